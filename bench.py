@@ -1,0 +1,200 @@
+"""Benchmark: SNICIT vs plain SpMM layer loop on one MI355X, edges/s and ms per batch.
+
+A step is one full forward (all L layers, clustering, recovery) over one batch of
+synthetic inputs already resident in HBM.  Default workload: the SDGC-like net
+N=16384, L=120, 32 nnz/row of 0.0625, bias -0.4, clip 32, batch 60000 binary blob
+images at ~15% density, threshold layer 30.
+
+Prints diagnostics on stderr and ONE JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    "c1": dict(N=1024, L=120, B=6000, bias=-0.3, t=30),
+    "c2": dict(N=16384, L=120, B=60000, bias=-0.4, t=30),
+    "c3": dict(N=65536, L=1920, B=60000, bias=-0.45, t=30),
+}
+NET_SEED = 2023
+IMG_SEED = 60000
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmm_layer_bytes(N, P, rows):
+    """Algorithmic HBM bytes of one panel-SpMM launch: read and write every processed
+    activation once, plus the layer's weights in panel form (weights, slot inputs, panel
+    offsets, output ids)."""
+    weights = N * 32 * 4 + P * 32 * 4 + (P + 1) * 4 + N * 4
+    return 8.0 * N * rows + weights
+
+
+def time_steps(plan, steps, warmup):
+    for _ in range(warmup):
+        plan.run()
+    t0 = time.perf_counter()
+    sts = [plan.run() for _ in range(steps)]
+    wall = (time.perf_counter() - t0) * 1000.0 / steps
+    ev = float(np.mean([s.ms_total for s in sts]))
+    return wall, ev, sts[-1]
+
+
+def cpu_baseline(wl, gpu_cat, target_s):
+    import oracle as orc
+    import snicit_amd as sa
+
+    N, L = wl["N"], wl["L"]
+    cores = min(16, os.cpu_count() or 1)
+    cols = sa.gen_sdgc_cols(N, L, NET_SEED)
+    vals = np.full(cols.shape, 0.0625, np.float32)
+    # calibrate on a small sample, then size the timed sample for ~target_s seconds
+    cal = 32
+    y0 = sa.gen_images(N, IMG_SEED, 0, cal)
+    t0 = time.perf_counter()
+    orc.plain_f32(cols, vals, wl["bias"], 32.0, y0, nthreads=cores, want_y=False)
+    per = (time.perf_counter() - t0) / cal
+    n = int(max(cal, min(wl["B"], target_s / max(per, 1e-6))))
+    n = max(cores, n // cores * cores)
+    y0 = sa.gen_images(N, IMG_SEED, 0, n)
+    t0 = time.perf_counter()
+    _, cat, _ = orc.plain_f32(cols, vals, wl["bias"], 32.0, y0, nthreads=cores, want_y=False)
+    dt = time.perf_counter() - t0
+    edges = float(n) * N * 32 * L
+    match = bool(np.array_equal(cat, gpu_cat[:n]))
+    return {"value": edges / dt, "unit": "edges/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} of the {wl['B']} samples x all {L} layers "
+                      f"(plain FP32 forward with dead-row skipping, OpenMP); {dt:.2f} s",
+            "seconds": dt, "categories_match_gpu": match}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--threshold", type=int, default=None)
+    ap.add_argument("--dense", action="store_true", help="also time the plain-dense form")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the layer loop "
+              "shards by samples with no exchange inside the loop, so N GPUs would run N "
+              "independent copies of this measurement.", file=sys.stderr)
+        sys.exit(2)
+
+    import snicit_amd as sa
+
+    wl = dict(WORKLOADS[args.config])
+    if args.threshold is not None:
+        wl["t"] = args.threshold
+    N, L, B, t = wl["N"], wl["L"], wl["B"], wl["t"]
+    ctx = sa.Context(0)
+    t0 = time.perf_counter()
+    net = sa.Net.generate_sdgc(ctx, N, L, NET_SEED, bias=wl["bias"])
+    log(f"net N={N} L={L}: {net.panels} panels, {net.device_bytes / 1e9:.2f} GB resident, "
+        f"{time.perf_counter() - t0:.1f} s")
+    P = net.panels // L
+
+    snic = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
+    snic.set_input_images(IMG_SEED, 0, B)
+    ms_s, ev_s, st_s = time_steps(snic, args.steps, args.warmup)
+    cat_s = snic.categories()
+    log(f"SNICIT: {ms_s:.3f} ms/step (events {ev_s:.3f}); pre {st_s.ms_pre:.3f} cluster "
+        f"{st_s.ms_cluster:.3f} post {st_s.ms_post:.3f} recover {st_s.ms_recover:.3f}; "
+        f"B'={st_s.live_at_threshold} k={st_s.centroids} nerow {st_s.nerow_first}->"
+        f"{st_s.nerow_last} active={st_s.active}")
+
+    bf = sa.Plan(net, sa.default_cfg("BF"), B)
+    bf.set_input_images(IMG_SEED, 0, B)
+    ms_b, ev_b, st_b = time_steps(bf, args.steps, args.warmup)
+    cat_b = bf.categories()
+    same = bool(np.array_equal(cat_s, cat_b))
+    log(f"plain BF: {ms_b:.3f} ms/step (events {ev_b:.3f}); categories identical: {same}")
+    del bf
+    ms_d = None
+    if args.dense:
+        dn = sa.Plan(net, sa.default_cfg("PLAIN_DENSE"), B)
+        dn.set_input_images(IMG_SEED, 0, B)
+        ms_d, _, _ = time_steps(dn, max(1, args.steps // 2), 1)
+        same = same and bool(np.array_equal(dn.categories(), cat_s))
+        log(f"plain dense: {ms_d:.3f} ms/step")
+        del dn
+
+    # live per-launch timing of the dominant kernel (plain-epilogue panel SpMM before t)
+    ms_l, rows_l, cms_l = snic.profile()
+    pre = list(range(t))
+    b_pre = sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre)
+    t_pre = sum(ms_l[l] for l in pre) / 1000.0
+    n_pre = max(1, len(pre))
+    roof = {"kernel": "panel_spmm_kernel<plain> (layers < t)", "bound": "hbm",
+            "achieved": round(b_pre / t_pre / 1e9, 1) if t_pre > 0 else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(b_pre / t_pre / 1e9 / HBM_PEAK_GBS, 4) if t_pre > 0 else None,
+            "traffic": None,
+            "bytes_per_launch": b_pre / n_pre, "avg_launch_ms": 1000.0 * t_pre / n_pre}
+    post = list(range(t, L))
+    k = st_s.centroids
+    b_post = sum(spmm_layer_bytes(N, P, rows_l[l]) + 8.0 * N * k + 4.0 * N * k for l in post)
+    t_post = sum(ms_l[l] + cms_l[l] for l in post) / 1000.0
+    delta_roof = {"kernel": "panel_spmm_kernel<centroid>+<delta> (layers >= t)",
+                  "achieved": round(b_post / t_post / 1e9, 1) if t_post > 0 else None,
+                  "frac": round(b_post / t_post / 1e9 / HBM_PEAK_GBS, 4) if t_post > 0 else None,
+                  "bytes_per_layer": b_post / max(1, len(post)),
+                  "avg_layer_ms": 1000.0 * t_post / max(1, len(post))}
+    log(f"roofline pre-t SpMM: {roof['achieved']} GB/s ({roof['frac']}) avg "
+        f"{roof['avg_launch_ms']:.3f} ms/launch; post-t: {delta_roof['achieved']} GB/s")
+
+    edges = float(B) * N * 32 * L
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(wl, cat_s, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} edges/s on {cpu['cores']} cores; {cpu['sample']}; "
+            f"categories match GPU: {cpu['categories_match_gpu']}")
+    out = {
+        "metric": "SNICIT edges/s (samples x nnz(W) x layers / time), one forward per step",
+        "value": edges / (ms_s / 1000.0),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SDGC-like banded+permuted net, binary blob images)",
+        "config": {"workload": f"SDGC-like SNICIT vs plain, N={N} L={L} batch={B} "
+                               f"bias={wl['bias']} t={t}", "N": N, "L": L, "batch": B,
+                   "threshold": t, "bias": wl["bias"], "clip": 32.0, "nnz_per_row": 32},
+        "plain_bf_ms_per_step": ms_b,
+        "plain_bf_edges_per_s": edges / (ms_b / 1000.0),
+        "plain_dense_ms_per_step": ms_d,
+        "speedup_vs_plain_bf": ms_b / ms_s,
+        "categories_identical_snicit_vs_plain": same,
+        "stages_ms": {"pre": st_s.ms_pre, "cluster": st_s.ms_cluster, "post": st_s.ms_post,
+                      "recover": st_s.ms_recover, "events_total": ev_s},
+        "snicit": {"live_at_threshold": st_s.live_at_threshold, "centroids": k,
+                   "nerow_first": st_s.nerow_first, "nerow_last": st_s.nerow_last,
+                   "active": st_s.active},
+        "roofline": roof,
+        "delta_roofline": delta_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

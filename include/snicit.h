@@ -1,0 +1,181 @@
+/*
+ * snicit.h -- C-ABI of the MI355X (gfx950) SNICIT sparse-DNN inference layer loop.
+ *
+ * Plain C types only (no HIP, no torch): every entry point takes host pointers and
+ * sizes, returns 0 on success and a negative SNICIT_E* code on failure, and never
+ * throws across the boundary.  snicit_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * The reference has no library API; its in-process boundary is a set of header-only
+ * classes whose constructor takes the network and whose infer() runs the layer loop:
+ *   SNICIT_SDGC::SNICIT(weight_path, bias, N, L, threshold).infer(input, golden, num_inputs, batch)
+ *        src/SDGC/SNICIT/SNICIT.hpp:43-56, 87-203
+ *   SNICIT_SDGC::BF(weight_path, bias, N, L).infer(input, golden, num_inputs)   src/SDGC/BF/BF.hpp:55-68,102-124
+ *   SNICIT_SDGC::SNIG(...).infer(...)                                            src/SDGC/SNIG/SNIG.hpp:51-66,101-131
+ *   SNICIT_SDGC::XY(...).infer(...)                                              src/SDGC/XY/XY.hpp:41-54,85-201
+ *   SNICIT_BEY::{SNICIT,BF,SNIG}(...).infer(input_path)                          src/beyond/SNICIT/SNICIT.hpp:59-74,134-172
+ * Each entry point below names the part of those classes it replaces.  Ownership
+ * follows the reference: a net owns its resident device weights (the reference
+ * re-uploads them every round, SNICIT.hpp:252-317; here they stay in HBM), a plan
+ * owns the activation buffers of one batch shape.  A context and everything made
+ * from it must be used from one host thread at a time.
+ */
+#ifndef SNICIT_H
+#define SNICIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------ */
+#define SNICIT_OK 0
+#define SNICIT_E_INVALID (-1)   /* bad argument (sizes, mode, threshold ...)            */
+#define SNICIT_E_HIP (-2)       /* HIP runtime error (message has the HIP error string)  */
+#define SNICIT_E_NOMEM (-3)     /* device or host allocation failed / over budget       */
+#define SNICIT_E_STATE (-4)     /* call out of order (e.g. run before set_input)        */
+#define SNICIT_E_UNSUPPORTED (-5)
+
+/* ---- run modes (main/SDGC.cu:15-20,69-110; main/beyond.cu:12-18,104-150) --------- */
+#define SNICIT_MODE_SNICIT 0      /* pre-threshold SpMM, clustering, delta SpMM, recovery */
+#define SNICIT_MODE_BF 1          /* plain SpMM, all-zero rows skipped every layer         */
+#define SNICIT_MODE_SNIG 2        /* plain SpMM, same results as BF                        */
+#define SNICIT_MODE_XY 3          /* plain SpMM, same results as BF                        */
+#define SNICIT_MODE_PLAIN_DENSE 4 /* plain SpMM over every row, no skipping                */
+
+/* ---- bias forms -------------------------------------------------------------- */
+#define SNICIT_BIAS_SCALAR 0      /* one value for the whole net (main/SDGC.cu:49-54)      */
+#define SNICIT_BIAS_PER_LAYER 1   /* one value per layer: bias[L]                          */
+#define SNICIT_BIAS_PER_NEURON 2  /* bias[L*N] (src/beyond/general.hpp:50,70)              */
+
+/* ---- clustering variants ----------------------------------------------------- */
+#define SNICIT_CLUSTER_EXACT 0    /* SDGC SNICIT: src/SDGC/SNICIT/kernel.hpp:8-218         */
+#define SNICIT_CLUSTER_LOSSY 1    /* beyond SNICIT: src/beyond/SNICIT/kernel.hpp:5-183     */
+
+typedef struct snicit_ctx snicit_ctx;
+typedef struct snicit_net snicit_net;
+typedef struct snicit_plan snicit_plan;
+
+/* Run configuration.  snicit_run_cfg_default() fills the reference's values:
+ * sample_size 32 and segments 16 (SNICIT.hpp:228-229), eps 0.03 (kernel.hpp:178,184),
+ * lossy: pool 128 (main/beyond.cu:38), assign/delta tolerance 0.04 and post prune
+ * 0.05 (src/beyond/SNICIT/kernel.hpp:71,90,161). */
+typedef struct snicit_run_cfg {
+  int32_t mode;            /* SNICIT_MODE_*                                            */
+  int32_t threshold;       /* layer t at which clustering runs, 0 <= t < L (SNICIT)   */
+  int32_t cluster;         /* SNICIT_CLUSTER_*                                         */
+  int32_t sample_size;     /* exact: pool = first min(sample_size, live) live rows    */
+  int32_t segments;        /* exact: segment sums per pool row (N % segments == 0)    */
+  float eps;               /* exact: pruning tolerance on segment sums                */
+  int32_t lossy_pool;      /* lossy: pool = lossy_pool strided rows                   */
+  float lossy_eps_pool;    /* lossy: pruning tolerance between pool rows (0.03)       */
+  float lossy_eps_assign;  /* lossy: L0 tolerance and delta pruning at clustering (0.04)*/
+  float lossy_eps_post;    /* lossy: delta pruning after each post layer (0.05)       */
+  int32_t use_graph;       /* 1: replay the layer loop as captured hipGraphs          */
+  int32_t reserved[7];
+} snicit_run_cfg;
+
+/* Per-run statistics.  Times are HIP-event milliseconds on the plan's stream with
+ * inputs and weights resident, split like the reference's stage report
+ * (SNICIT.hpp:659-664): pre = layers < t, cluster = clustering at t, post = layers
+ * >= t, recover = recovery + categories.  Plain modes report everything in pre. */
+typedef struct snicit_stats {
+  double ms_total, ms_pre, ms_cluster, ms_post, ms_recover;
+  double edges;              /* samples x nnz(W_l) x layers (nominal work)           */
+  int64_t batch;             /* samples in the run                                    */
+  int64_t live_at_threshold; /* B' (live rows entering clustering)                    */
+  int64_t nerow_first;       /* non-empty delta rows after clustering                 */
+  int64_t nerow_last;        /* non-empty delta rows after the last layer             */
+  int64_t active;            /* samples with category 1                               */
+  int32_t centroids;         /* k                                                     */
+  int32_t layers;
+  int64_t rows_processed;    /* sum over layers of rows the SpMM read                  */
+  int64_t reserved[6];
+} snicit_stats;
+
+/* ---- context --------------------------------------------------------------------
+ * Replaces the reference's GpuEnv (src/SDGC/gpu_lib/gpu_env.h:11-89): one device and one
+ * HIP stream (default priority). */
+int snicit_ctx_create(int device, snicit_ctx** out);
+int snicit_ctx_destroy(snicit_ctx* ctx);
+const char* snicit_last_error(void);
+void snicit_run_cfg_default(snicit_run_cfg* cfg);
+
+/* ---- nets -----------------------------------------------------------------------
+ * snicit_net_create_ell replaces the weight ingestion of Base::_load_weight
+ * (src/SDGC/base.hpp:89-141) and of the SNICIT/XY preprocessing (SNICIT.hpp:127-183):
+ * row o of layer l has K nonzeros cols[(l*N+o)*K + r] (input neuron) with value
+ * vals[(l*N+o)*K + r], i.e. Y_{l+1}[s,o] = f(sum_r vals*Y_l[s,cols] + b).  This is the
+ * transpose of the reference's TSV line "i j v" (input i -> output j, utility.hpp:109-113).
+ * Rows with fewer than K nonzeros pad with value 0.  The library packs rows that share
+ * one input set into panels (the 16-outputs-share-32-inputs structure the reference's XY
+ * kernels exploit, src/SDGC/XY/kernel.hpp:111-149) and keeps everything in HBM.
+ * f(z) = min(max(z,0),clip) (src/SDGC/XY/kernel.hpp:5-7; clip 32 for SDGC, 1 for beyond). */
+int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K,
+                          const int32_t* cols, const float* vals, int32_t bias_kind,
+                          const float* bias, float clip, snicit_net** out);
+
+/* Seeded SDGC-like net (32 nnz/row of `weight`, banded+permuted 16-chunk pattern),
+ * built directly in panel form with the weights filled on the device; no reference
+ * counterpart (the reference reads the Graph Challenge TSVs, src/SDGC/utility.hpp:65-157). */
+int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t seed,
+                             float weight, float bias, float clip, snicit_net** out);
+int snicit_net_destroy(snicit_net* net);
+int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,
+                    int64_t* panels_total, int64_t* device_bytes);
+
+/* ---- plans (one batch shape + mode on one net) ----------------------------------
+ * A plan holds the device activations of up to max_batch samples and the captured
+ * layer-loop graphs.  It replaces the per-round buffers of SNICIT::_infer
+ * (SNICIT.hpp:205-317) and BF::_infer (BF.hpp:157-220). */
+int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_batch,
+                       snicit_plan** out);
+int snicit_plan_destroy(snicit_plan* plan);
+/* Upload y0 (B rows of N floats, sample-major like the reference's input[b][n],
+ * utility.hpp:268-293) into the plan's resident input buffer. */
+int snicit_plan_set_input(snicit_plan* plan, const float* y0, int64_t B);
+/* Generate B seeded binary blob images (samples first_sample..first_sample+B-1) on the
+ * device into the resident input buffer; equals snicit_gen_images on the host. */
+int snicit_plan_set_input_images(snicit_plan* plan, uint64_t seed, int64_t first_sample,
+                                 int64_t B);
+/* Run the whole layer loop on the resident input.  Replaces SNICIT::_infer / BF::_infer. */
+int snicit_plan_run(snicit_plan* plan, snicit_stats* stats);
+/* Plain modes only: run layers [0, stop_layer) and stop (for layer-local checks). */
+int snicit_plan_run_layers(snicit_plan* plan, int32_t stop_layer, snicit_stats* stats);
+/* Categories of the last run: cat[b] = 1 iff any final activation of sample b > 0
+ * (identify, src/SDGC/utility.hpp:194-212; recover, src/SDGC/SNICIT/kernel.hpp:197-218). */
+int snicit_plan_categories(snicit_plan* plan, int32_t* cat);
+/* Final activations of the last run as B x N floats (sample-major).  SNICIT mode returns
+ * the recovered Yhat_i + Yhat_m(i) (src/SDGC/SNICIT/kernel.hpp:210). */
+int snicit_plan_activations(snicit_plan* plan, float* y);
+/* Introspection of the last run.  what: 0 live rows after each layer (int64[L]),
+ * 1 non-empty delta rows after each post layer (int64[L], 0 before t), 2 pool rows
+ * as original sample ids (int32[pool]), 3 pool segment sums (float[pool*segments]),
+ * 4 centroids as original sample ids (int32[k]), 5 per-sample assignment (int32[B]:
+ * centroid index, or -1 for rows dead before t).  Returns the element count. */
+int64_t snicit_plan_trace(snicit_plan* plan, int32_t what, void* out, int64_t cap);
+
+/* Per-launch timing: replays the plan's layer loop eagerly with a hipEvent pair around
+ * every SpMM launch on the plan's stream (the reference's per-layer event timing,
+ * SNICIT.hpp:351-354).  For layer l < cap: ms[l] = duration of the layer's plain or
+ * delta-row SpMM launch, rows[l] = columns it produced, cent_ms[l] = duration of the
+ * centroid-row launch (post-threshold layers, else 0).  Any pointer may be null. */
+int snicit_plan_profile(snicit_plan* plan, double* ms, int64_t* rows, double* cent_ms,
+                        int64_t cap);
+
+/* One-call form mirroring SNICIT::infer(input, golden, num_inputs, batch): rounds of
+ * cfg batch over B samples, categories for all B. */
+int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
+                 int64_t batch, int32_t* cat, snicit_stats* stats);
+
+/* ---- host-side seeded generators (splitmix64 streams) ---------------------------- */
+/* cols[(l*N+o)*32 + r] of the SDGC-like net (ascending per row); all values = weight. */
+int snicit_gen_sdgc_ell(int32_t N, int32_t L, uint64_t seed, int32_t* cols);
+/* B x N binary blob images for samples first_sample.. (row-major pixels). */
+int snicit_gen_images(int32_t N, uint64_t seed, int64_t first_sample, int64_t B, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNICIT_H */

@@ -1,0 +1,134 @@
+"""ctypes bindings of the CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py, as the checker / the timed CPU baseline.  The product never imports it.
+Arrays are sample-major (B, N); nets are per-row ELL (L, N, K).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_F32P = ctypes.POINTER(ctypes.c_float)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing; build the oracle (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_plain_f32.restype = ctypes.c_int
+        L.oracle_plain_f32.argtypes = [ctypes.c_int] * 3 + [_I32P, _F32P, ctypes.c_int, _F32P,
+                                                            ctypes.c_float, ctypes.c_int64, _F32P,
+                                                            _F32P, _I32P, _I64P, ctypes.c_int]
+        L.oracle_plain_f64.restype = ctypes.c_int
+        L.oracle_plain_f64.argtypes = [ctypes.c_int] * 3 + [_I32P, _F32P, ctypes.c_int, _F32P,
+                                                            ctypes.c_float, ctypes.c_int64, _F32P,
+                                                            _F64P, _I32P, ctypes.c_int]
+        L.oracle_layer_f64.restype = ctypes.c_int
+        L.oracle_layer_f64.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _F32P, ctypes.c_int,
+                                       _F32P, ctypes.c_float, ctypes.c_int64, _F32P, _F64P]
+        L.oracle_snicit_exact.restype = ctypes.c_int
+        L.oracle_snicit_exact.argtypes = [ctypes.c_int] * 3 + [
+            _I32P, _F32P, ctypes.c_int, _F32P, ctypes.c_float, ctypes.c_int64, _F32P,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _I32P, _F32P, _I32P, _F32P,
+            _I32P, _I32P, _I64P, _I64P, _I64P, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _bias(bias, L, N):
+    b = np.atleast_1d(np.ascontiguousarray(bias, dtype=np.float32)).ravel()
+    kind = 0 if b.size == 1 else 1 if b.size == L else 2
+    if kind == 2:
+        assert b.size == L * N
+    return kind, b
+
+
+def plain_f32(cols, vals, bias, clip, y0, nthreads=0, want_y=True):
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    y0 = np.ascontiguousarray(y0, np.float32)
+    L, N, K = cols.shape
+    B = y0.shape[0]
+    kind, b = _bias(bias, L, N)
+    yL = np.empty((B, N), np.float32) if want_y else None
+    cat = np.empty(B, np.int32)
+    live = np.empty(L, np.int64)
+    rc = lib().oracle_plain_f32(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind, _p(b, _F32P),
+                                clip, B, _p(y0, _F32P), _p(yL, _F32P), _p(cat, _I32P),
+                                _p(live, _I64P), nthreads)
+    assert rc == 0
+    return yL, cat, live
+
+
+def plain_f64(cols, vals, bias, clip, y0, nthreads=0):
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    y0 = np.ascontiguousarray(y0, np.float32)
+    L, N, K = cols.shape
+    B = y0.shape[0]
+    kind, b = _bias(bias, L, N)
+    yL = np.empty((B, N), np.float64)
+    cat = np.empty(B, np.int32)
+    rc = lib().oracle_plain_f64(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind, _p(b, _F32P),
+                                clip, B, _p(y0, _F32P), _p(yL, _F64P), _p(cat, _I32P), nthreads)
+    assert rc == 0
+    return yL, cat
+
+
+def layer_f64(cols_l, vals_l, bias_l, clip, yin):
+    cols_l = np.ascontiguousarray(cols_l, np.int32)
+    vals_l = np.ascontiguousarray(vals_l, np.float32)
+    yin = np.ascontiguousarray(yin, np.float32)
+    N, K = cols_l.shape
+    b = np.atleast_1d(np.ascontiguousarray(bias_l, np.float32)).ravel()
+    kind = 0 if b.size == 1 else 2
+    out = np.empty((yin.shape[0], N), np.float64)
+    rc = lib().oracle_layer_f64(N, K, _p(cols_l, _I32P), _p(vals_l, _F32P), kind, _p(b, _F32P),
+                                clip, yin.shape[0], _p(yin, _F32P), _p(out, _F64P))
+    assert rc == 0
+    return out
+
+
+def snicit_exact(cols, vals, bias, clip, y0, t, sample_size=32, segments=16, eps=0.03,
+                 nthreads=0, want_y=True):
+    """Exact SNICIT restatement.  Returns a dict with cat, yL, pool, F, centroids, assign,
+    live (per pre layer), nerow (per post layer) and info (B', pool, k, nerow at t)."""
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    y0 = np.ascontiguousarray(y0, np.float32)
+    L, N, K = cols.shape
+    B = y0.shape[0]
+    kind, b = _bias(bias, L, N)
+    cat = np.empty(B, np.int32)
+    yL = np.empty((B, N), np.float32) if want_y else None
+    pool = np.full(64, -1, np.int32)
+    F = np.zeros(64 * segments, np.float32)
+    cent = np.full(64, -1, np.int32)
+    assign = np.empty(B, np.int32)
+    live = np.full(L, -1, np.int64)
+    nerow = np.zeros(L, np.int64)
+    info = np.zeros(4, np.int64)
+    rc = lib().oracle_snicit_exact(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind, _p(b, _F32P),
+                                   clip, B, _p(y0, _F32P), t, sample_size, segments, eps,
+                                   _p(cat, _I32P), _p(yL, _F32P), _p(pool, _I32P), _p(F, _F32P),
+                                   _p(cent, _I32P), _p(assign, _I32P), _p(live, _I64P),
+                                   _p(nerow, _I64P), _p(info, _I64P), nthreads)
+    assert rc == 0, rc
+    Bp, P, k, ne0 = (int(x) for x in info)
+    return {"cat": cat, "yL": yL, "pool": pool[:P], "F": F[:P * segments].reshape(P, segments),
+            "centroids": cent[:k], "assign": assign, "live": live[:t], "nerow": nerow[t:],
+            "live_at_t": Bp, "k": k, "nerow_at_t": ne0}

@@ -1,0 +1,357 @@
+"""snicit_amd -- MI355X (gfx950) SNICIT sparse-DNN inference layer loop.
+
+Python mirror of the reference's in-process interface over the C-ABI of libsnicit.so
+(include/snicit.h).  The reference's classes take the net in the constructor and run the
+layer loop in ``infer``:
+
+    SNICIT_SDGC::SNICIT(weight_path, bias, N, L, threshold).infer(input, golden, num_inputs, batch)
+        (src/SDGC/SNICIT/SNICIT.hpp:43-56, 87-203)
+    SNICIT_SDGC::BF(weight_path, bias, N, L).infer(input, golden, num_inputs)
+        (src/SDGC/BF/BF.hpp:55-68, 102-124)
+    SNICIT_SDGC::SNIG / XY  (src/SDGC/SNIG/SNIG.hpp:51-66, src/SDGC/XY/XY.hpp:41-54)
+
+Here the classes take a resident :class:`Net` (weights stay in HBM) and ``infer`` takes the
+input rows, returning the per-sample categories.  All compute runs in the HIP library;
+there is no CPU fallback: importing works without a GPU, but every device call raises if
+the library or the device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsnicit.so")
+
+MODE_SNICIT, MODE_BF, MODE_SNIG, MODE_XY, MODE_PLAIN_DENSE = 0, 1, 2, 3, 4
+MODES = {"SNICIT": MODE_SNICIT, "BF": MODE_BF, "SNIG": MODE_SNIG, "XY": MODE_XY,
+         "PLAIN_DENSE": MODE_PLAIN_DENSE}
+BIAS_SCALAR, BIAS_PER_LAYER, BIAS_PER_NEURON = 0, 1, 2
+CLUSTER_EXACT, CLUSTER_LOSSY = 0, 1
+
+# Graph Challenge biases per N (main/SDGC.cu:49-54).
+SDGC_BIAS = {1024: -0.3, 4096: -0.35, 16384: -0.4, 65536: -0.45}
+
+
+class SnicitError(RuntimeError):
+    pass
+
+
+class RunCfg(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32), ("threshold", ctypes.c_int32), ("cluster", ctypes.c_int32),
+        ("sample_size", ctypes.c_int32), ("segments", ctypes.c_int32), ("eps", ctypes.c_float),
+        ("lossy_pool", ctypes.c_int32), ("lossy_eps_pool", ctypes.c_float),
+        ("lossy_eps_assign", ctypes.c_float), ("lossy_eps_post", ctypes.c_float),
+        ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double), ("ms_pre", ctypes.c_double),
+        ("ms_cluster", ctypes.c_double), ("ms_post", ctypes.c_double),
+        ("ms_recover", ctypes.c_double), ("edges", ctypes.c_double),
+        ("batch", ctypes.c_int64), ("live_at_threshold", ctypes.c_int64),
+        ("nerow_first", ctypes.c_int64), ("nerow_last", ctypes.c_int64),
+        ("active", ctypes.c_int64), ("centroids", ctypes.c_int32), ("layers", ctypes.c_int32),
+        ("rows_processed", ctypes.c_int64), ("reserved", ctypes.c_int64 * 6),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+# Every symbol the header declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_F32P = ctypes.POINTER(ctypes.c_float)
+SIGNATURES = {
+    "snicit_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "snicit_ctx_destroy": (ctypes.c_int, [_P]),
+    "snicit_last_error": (ctypes.c_char_p, []),
+    "snicit_run_cfg_default": (None, [ctypes.POINTER(RunCfg)]),
+    "snicit_net_create_ell": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                             _I32P, _F32P, ctypes.c_int32, _F32P, ctypes.c_float,
+                                             ctypes.POINTER(_P)]),
+    "snicit_net_generate_sdgc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
+                                                ctypes.c_float, ctypes.POINTER(_P)]),
+    "snicit_net_destroy": (ctypes.c_int, [_P]),
+    "snicit_net_info": (ctypes.c_int, [_P, _I32P, _I32P, _I32P,
+                                       ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64)]),
+    "snicit_plan_create": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), ctypes.c_int64,
+                                          ctypes.POINTER(_P)]),
+    "snicit_plan_destroy": (ctypes.c_int, [_P]),
+    "snicit_plan_set_input": (ctypes.c_int, [_P, _F32P, ctypes.c_int64]),
+    "snicit_plan_set_input_images": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int64,
+                                                    ctypes.c_int64]),
+    "snicit_plan_run": (ctypes.c_int, [_P, ctypes.POINTER(Stats)]),
+    "snicit_plan_run_layers": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(Stats)]),
+    "snicit_plan_categories": (ctypes.c_int, [_P, _I32P]),
+    "snicit_plan_activations": (ctypes.c_int, [_P, _F32P]),
+    "snicit_plan_trace": (ctypes.c_int64, [_P, ctypes.c_int32, _P, ctypes.c_int64]),
+    "snicit_plan_profile": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_int64]),
+    "snicit_infer": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
+                                    ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
+    "snicit_gen_sdgc_ell": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, _I32P]),
+    "snicit_gen_images": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                         ctypes.c_int64, _F32P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libsnicit.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SnicitError(f"{LIB_PATH} is missing; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().snicit_last_error().decode(errors="replace")
+        raise SnicitError(f"snicit error {rc}: {msg}")
+
+
+def _i32(a):
+    return a.ctypes.data_as(_I32P)
+
+
+def _f32(a):
+    return a.ctypes.data_as(_F32P)
+
+
+def default_cfg(mode="SNICIT", threshold=30, **kw):
+    cfg = RunCfg()
+    lib().snicit_run_cfg_default(ctypes.byref(cfg))
+    cfg.mode = MODES[mode] if isinstance(mode, str) else int(mode)
+    cfg.threshold = int(threshold)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+# ---- host generators ---------------------------------------------------------------
+def gen_sdgc_cols(N, L, seed):
+    """Columns (L, N, 32) of the seeded SDGC-like net (values are all the same weight)."""
+    cols = np.empty((L, N, 32), dtype=np.int32)
+    _check(lib().snicit_gen_sdgc_ell(N, L, seed, _i32(cols)))
+    return cols
+
+
+def gen_images(N, seed, first_sample, B):
+    """(B, N) float32 binary blob images of samples first_sample .. first_sample+B-1."""
+    out = np.empty((B, N), dtype=np.float32)
+    _check(lib().snicit_gen_images(N, seed, first_sample, B, _f32(out)))
+    return out
+
+
+# ---- device objects --------------------------------------------------------------------
+class Context:
+    """One device and one HIP stream (replaces GpuEnv, src/SDGC/gpu_lib/gpu_env.h:11-89)."""
+
+    def __init__(self, device=0):
+        h = _P()
+        _check(lib().snicit_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().snicit_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Net:
+    """A resident net.  ``from_ell`` replaces the TSV weight ingestion of
+    Base::_load_weight (src/SDGC/base.hpp:89-141); ``generate_sdgc`` builds the seeded
+    SDGC-like net on the device."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        n, l, k = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        p, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().snicit_net_info(self.h, ctypes.byref(n), ctypes.byref(l), ctypes.byref(k),
+                                     ctypes.byref(p), ctypes.byref(b)))
+        self.N, self.L, self.K = n.value, l.value, k.value
+        self.panels, self.device_bytes = p.value, b.value
+
+    @classmethod
+    def from_ell(cls, ctx, cols, vals, bias, clip=32.0):
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        L, N, K = cols.shape
+        assert vals.shape == cols.shape
+        b = np.atleast_1d(np.ascontiguousarray(bias, dtype=np.float32))
+        if b.size == 1:
+            kind = BIAS_SCALAR
+        elif b.size == L:
+            kind = BIAS_PER_LAYER
+        elif b.size == L * N:
+            kind = BIAS_PER_NEURON
+        else:
+            raise ValueError("bias must have 1, L or L*N entries")
+        h = _P()
+        _check(lib().snicit_net_create_ell(ctx.h, N, L, K, _i32(cols), _f32(vals), kind,
+                                           _f32(b.ravel()), clip, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def generate_sdgc(cls, ctx, N, L, seed, bias=None, weight=0.0625, clip=32.0):
+        if bias is None:
+            bias = SDGC_BIAS[N]
+        h = _P()
+        _check(lib().snicit_net_generate_sdgc(ctx.h, N, L, seed, weight, bias, clip,
+                                              ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def close(self):
+        if self.h:
+            lib().snicit_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Activations of up to ``max_batch`` samples for one mode on one net."""
+
+    TRACE_LIVE, TRACE_NEROW, TRACE_POOL, TRACE_F, TRACE_CENTROIDS, TRACE_ASSIGN = range(6)
+
+    def __init__(self, net, cfg, max_batch):
+        self.net = net
+        self.cfg = cfg
+        h = _P()
+        _check(lib().snicit_plan_create(net.h, ctypes.byref(cfg), max_batch, ctypes.byref(h)))
+        self.h = h
+        self.B = 0
+
+    def set_input(self, y0):
+        y0 = np.ascontiguousarray(y0, dtype=np.float32)
+        assert y0.ndim == 2 and y0.shape[1] == self.net.N
+        _check(lib().snicit_plan_set_input(self.h, _f32(y0), y0.shape[0]))
+        self.B = y0.shape[0]
+
+    def set_input_images(self, seed, first_sample, B):
+        _check(lib().snicit_plan_set_input_images(self.h, seed, first_sample, B))
+        self.B = B
+
+    def run(self, stop_layer=None):
+        st = Stats()
+        if stop_layer is None:
+            _check(lib().snicit_plan_run(self.h, ctypes.byref(st)))
+        else:
+            _check(lib().snicit_plan_run_layers(self.h, stop_layer, ctypes.byref(st)))
+        return st
+
+    def categories(self):
+        out = np.empty(self.B, dtype=np.int32)
+        _check(lib().snicit_plan_categories(self.h, _i32(out)))
+        return out
+
+    def activations(self):
+        out = np.empty((self.B, self.net.N), dtype=np.float32)
+        _check(lib().snicit_plan_activations(self.h, _f32(out)))
+        return out
+
+    def trace(self, what):
+        n = lib().snicit_plan_trace(self.h, what, None, 0)
+        if n < 0:
+            _check(int(n))
+        dtype = {0: np.int64, 1: np.int64, 2: np.int32, 3: np.float32, 4: np.int32,
+                 5: np.int32}[what]
+        out = np.empty(max(int(n), 1), dtype=dtype)
+        m = lib().snicit_plan_trace(self.h, what, out.ctypes.data_as(_P), out.size)
+        if m < 0:
+            _check(int(m))
+        return out[:int(n)]
+
+    def profile(self):
+        """Per-layer SpMM launch durations (ms), rows and centroid-launch ms (eager replay)."""
+        L = self.net.L
+        ms = np.zeros(L, np.float64)
+        rows = np.zeros(L, np.int64)
+        cms = np.zeros(L, np.float64)
+        _check(lib().snicit_plan_profile(self.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                         cms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), L))
+        return ms, rows, cms
+
+    def close(self):
+        if self.h:
+            lib().snicit_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Method:
+    """Shared body of the mode classes: a resident net and ``infer`` in rounds of
+    ``batch`` (SNICIT::infer, src/SDGC/SNICIT/SNICIT.hpp:121 rounds of batch)."""
+
+    mode = "BF"
+
+    def __init__(self, net, threshold=30, **cfg):
+        self.net = net
+        self.cfg = default_cfg(self.mode, threshold, **cfg)
+        self.stats = None
+
+    def infer(self, inputs, batch=None):
+        inputs = np.ascontiguousarray(inputs, dtype=np.float32)
+        B = inputs.shape[0]
+        batch = B if batch is None else int(batch)
+        if B % batch:
+            raise ValueError("batch must divide the number of inputs")
+        cat = np.empty(B, dtype=np.int32)
+        st = Stats()
+        _check(lib().snicit_infer(self.net.h, ctypes.byref(self.cfg), _f32(inputs), B, batch,
+                                  _i32(cat), ctypes.byref(st)))
+        self.stats = st
+        return cat
+
+
+class SNICIT(_Method):
+    """Exact SNICIT (pre-threshold SpMM, clustering at ``threshold``, delta SpMM, recovery)."""
+    mode = "SNICIT"
+
+
+class BF(_Method):
+    """Plain layer-by-layer SpMM with all-zero rows skipped (BF-2019)."""
+    mode = "BF"
+
+
+class SNIG(_Method):
+    """Plain SpMM under the SNIG mode name (same results as BF)."""
+    mode = "SNIG"
+
+
+class XY(_Method):
+    """Plain SpMM under the XY mode name (same results as BF)."""
+    mode = "XY"

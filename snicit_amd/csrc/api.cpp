@@ -1,0 +1,1031 @@
+// C-ABI of libsnicit: contexts, nets, plans and the layer loop.
+//
+// The layer loop follows SNICIT::_infer (src/SDGC/SNICIT/SNICIT.hpp:205-692) and
+// BF::_infer (src/SDGC/BF/BF.hpp:157-220) in what it computes, with every piece of
+// bookkeeping on the device: live-row lists, the centroid pool, assignment and the
+// non-empty delta rows are built by kernels, counts stay in device memory, and each
+// stage (pre / cluster / post / recover) can be replayed as one captured hipGraph
+// because no launch size depends on a host read-back.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+
+#include "kernels.hpp"
+
+using namespace snicit;
+
+struct snicit_ctx : Ctx {};
+struct snicit_net : Net {};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return SNICIT_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SNICIT_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SNICIT_E_INVALID;
+  }
+}
+
+template <class T>
+T* dev_alloc(size_t n, int64_t* acct = nullptr) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess)
+    throw Error(e == hipErrorOutOfMemory ? SNICIT_E_NOMEM : SNICIT_E_HIP,
+                "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+                    hipGetErrorString(e));
+  if (acct) *acct += (int64_t)(n * sizeof(T));
+  return static_cast<T*>(p);
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Upload one layer's host panels into the net's concatenated arrays.
+struct NetBuilder {
+  Net* net;
+  std::vector<int32_t> pcols, pbeg, outs;
+  std::vector<float> w;
+  bool weights_given = true;
+  void add(const HostLayerPanels& hp) {
+    net->panel_off.push_back(net->panel_off.back() + (int64_t)hp.pbeg.size() - 1);
+    pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
+    pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
+    outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());
+    if (weights_given) w.insert(w.end(), hp.w.begin(), hp.w.end());
+  }
+};
+
+void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
+  hipStream_t s = net->ctx->stream;
+  const int64_t Ptot = net->panel_off.back();
+  net->d_pcols = dev_alloc<int32_t>((size_t)Ptot * kSlots, &net->device_bytes);
+  net->d_pbeg = dev_alloc<int32_t>((size_t)Ptot + net->L, &net->device_bytes);
+  net->d_outs = dev_alloc<int32_t>((size_t)net->L * net->N, &net->device_bytes);
+  net->d_w = dev_alloc<float>((size_t)net->L * net->N * kSlots, &net->device_bytes);
+  SNICIT_HIP(hipMemcpy(net->d_pcols, nb.pcols.data(), nb.pcols.size() * 4, hipMemcpyHostToDevice));
+  SNICIT_HIP(hipMemcpy(net->d_pbeg, nb.pbeg.data(), nb.pbeg.size() * 4, hipMemcpyHostToDevice));
+  SNICIT_HIP(hipMemcpy(net->d_outs, nb.outs.data(), nb.outs.size() * 4, hipMemcpyHostToDevice));
+  if (nb.weights_given) {
+    SNICIT_HIP(hipMemcpy(net->d_w, nb.w.data(), nb.w.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    launch_fill_f32(net->d_w, fill_weight, (int64_t)net->L * net->N * kSlots, s);
+  }
+  if (net->bias_kind == SNICIT_BIAS_PER_NEURON) {
+    net->d_bias = dev_alloc<float>((size_t)net->L * net->N, &net->device_bytes);
+    SNICIT_HIP(hipMemcpy(net->d_bias, net->bias_host.data(), net->bias_host.size() * 4,
+                         hipMemcpyHostToDevice));
+  }
+  SNICIT_HIP(hipStreamSynchronize(s));
+}
+
+void net_set_bias(Net* net, int32_t bias_kind, const float* bias) {
+  SNICIT_REQUIRE(bias != nullptr, "bias must not be null");
+  net->bias_kind = bias_kind;
+  size_t nb = bias_kind == SNICIT_BIAS_SCALAR      ? 1
+              : bias_kind == SNICIT_BIAS_PER_LAYER ? (size_t)net->L
+              : bias_kind == SNICIT_BIAS_PER_NEURON ? (size_t)net->L * net->N
+                                                     : 0;
+  SNICIT_REQUIRE(nb > 0, "bad bias_kind");
+  net->bias_host.assign(bias, bias + nb);
+  net->skip_dead = true;
+  for (float b : net->bias_host)
+    if (!(b <= 0.f)) net->skip_dead = false;
+}
+
+void destroy_net(Net* net) {
+  dev_free(net->d_pcols);
+  dev_free(net->d_pbeg);
+  dev_free(net->d_outs);
+  dev_free(net->d_w);
+  dev_free(net->d_bias);
+}
+
+}  // namespace
+
+// ---- plan ------------------------------------------------------------------------------
+struct snicit_plan {
+  Net* net = nullptr;
+  snicit_run_cfg cfg{};
+  int64_t cap = 0, ld = 0;
+  int64_t B = 0;
+  bool have_input = false, have_run = false;
+  int32_t kcap = kMaxCentroidsExact;
+  int32_t stop_layer = -1;  // layers actually run by the last run
+  // activations
+  float* d_in = nullptr;
+  float* d_y[2] = {nullptr, nullptr};
+  // bookkeeping
+  int32_t* d_flags = nullptr;
+  int32_t* d_map = nullptr;   // pre-threshold / plain live list
+  int32_t* d_mapD = nullptr;  // post-threshold delta list
+  int32_t* d_orig[2] = {nullptr, nullptr};
+  int32_t* d_mcar[2] = {nullptr, nullptr};
+  int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
+  int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
+  int32_t* d_k = nullptr;
+  float* d_F = nullptr;
+  int32_t* d_cent_slot = nullptr;
+  float* d_C[2] = {nullptr, nullptr};
+  float* d_zc = nullptr;
+  int32_t* d_rec = nullptr;
+  int32_t* d_catc = nullptr;
+  int32_t* d_act = nullptr;
+  int32_t* d_cat = nullptr;
+  int32_t* d_img_params = nullptr;
+  int64_t img_params_cap = 0;
+  int32_t* d_img_blobs = nullptr;
+  int32_t* d_img_proto = nullptr;
+  float* d_stage = nullptr;  // [cap][N] staging for host transfers
+  // where the last run left its final state
+  int last_orig = 0, last_mcar = 0;
+  const float* last_D = nullptr;
+  const float* last_Y = nullptr;
+  const float* last_C = nullptr;
+  bool last_list = false;  // final columns selected by d_map (plain) / d_mapD (SNICIT)
+  // graphs: stage 0 pre, 1 cluster, 2 post, 3 recover
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t graph_stop = -2;
+  hipEvent_t ev[5] = {};
+  int64_t device_bytes = 0;
+  // per-launch profiling (eager only): events [4*l .. 4*l+3] = main start/stop, centroid
+  bool profiling = false;
+  std::vector<hipEvent_t> prof_ev;
+};
+
+namespace {
+
+bool plain_mode(int32_t mode) { return mode != SNICIT_MODE_SNICIT; }
+bool compacting(const snicit_plan* p) {
+  return p->net->skip_dead && p->cfg.mode != SNICIT_MODE_PLAIN_DENSE;
+}
+
+void prof_mark(snicit_plan* p, int l, int which) {
+  if (!p->profiling) return;
+  SNICIT_HIP(hipEventRecord(p->prof_ev[(size_t)4 * l + which], p->net->ctx->stream));
+}
+
+SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
+  SpmmLaunch a{};
+  a.net = p->net;
+  a.layer = l;
+  a.epilogue = epi;
+  a.kcap = p->kcap;
+  a.eps_post = p->cfg.lossy_eps_post;
+  a.max_cols = p->cap;
+  return a;
+}
+
+// Stage "pre": layers [0, l_end) of the plain loop.  With compaction every layer reads its
+// live columns through d_map and writes them densely, then a compaction builds the next
+// list (BF's rowsY, src/SDGC/BF/BF.hpp:183-200).  Leaves the live list for layer l_end in
+// d_map / d_counts[l_end] / d_orig[return value].
+int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
+  hipStream_t s = p->net->ctx->stream;
+  const bool cmp = compacting(p);
+  launch_iota_i32(p->d_orig[0], p->cap, s);
+  int cur = 0;
+  for (int l = 0; l < l_end; ++l) {
+    SpmmLaunch a = spmm_base(p, l, EPI_PLAIN);
+    a.yin = l == 0 ? p->d_in : p->d_y[(l - 1) & 1];
+    a.ldin = p->ld;
+    a.yout = p->d_y[l & 1];
+    a.ldout = p->ld;
+    if (cmp) {
+      a.map = l == 0 ? nullptr : p->d_map;
+      a.count = p->d_counts + l;
+      a.flags = p->d_flags;
+    } else {
+      a.map = nullptr;
+      a.count = p->d_counts;
+      a.flags = (final_flags && l == l_end - 1) ? p->d_flags : nullptr;
+    }
+    prof_mark(p, l, 0);
+    launch_panel_spmm(a, s);
+    prof_mark(p, l, 1);
+    if (cmp) {
+      launch_compact(p->d_flags, p->d_counts + l, p->d_counts + l + 1, p->d_map, nullptr,
+                     p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, s);
+      cur ^= 1;
+    }
+  }
+  return cur;
+}
+
+void enqueue_plain_categories(snicit_plan* p, int l_end, int cur) {
+  hipStream_t s = p->net->ctx->stream;
+  launch_fill_i32(p->d_cat, 0, p->cap, s);
+  if (compacting(p)) {
+    launch_cat_from_list(p->d_orig[cur], p->d_counts + l_end, p->cap, p->d_cat, s);
+  } else {
+    launch_cat_from_flags(p->d_flags, p->d_counts, p->cap, p->d_cat, s);
+  }
+}
+
+struct SnicitState {
+  const float* Yt = nullptr;  // activations entering layer t
+  const int32_t* map = nullptr;
+  const int32_t* count = nullptr;
+  int orig = 0;
+  float* Da = nullptr;
+  float* Db = nullptr;
+};
+
+SnicitState snicit_state(snicit_plan* p) {
+  SnicitState st;
+  const int t = p->cfg.threshold;
+  st.Yt = t == 0 ? p->d_in : p->d_y[(t - 1) & 1];
+  st.Da = t == 0 ? p->d_y[0] : p->d_y[t & 1];
+  st.Db = t == 0 ? p->d_y[1] : p->d_y[(t + 1) & 1];
+  return st;
+}
+
+// Stage "cluster" (src/SDGC/SNICIT/SNICIT.hpp:406-515): live list at t, pool segment sums,
+// Alg. 1 pruning, nearest-centroid assignment, non-empty delta list, delta rows.
+// Slot of d_orig holding the live list that enters clustering: t compactions starting
+// from slot 0, or one compaction of the input when t == 0; slot 0 without compaction.
+int cluster_cur(const snicit_plan* p) {
+  const int t = p->cfg.threshold;
+  if (!compacting(p)) return 0;
+  return t == 0 ? 1 : (t & 1);
+}
+
+void enqueue_cluster(snicit_plan* p) {
+  hipStream_t s = p->net->ctx->stream;
+  const int t = p->cfg.threshold;
+  const int N = p->net->N;
+  SnicitState st = snicit_state(p);
+  const int cur = cluster_cur(p);
+  const int32_t* map = nullptr;
+  const int32_t* count = p->d_counts;
+  if (compacting(p)) {
+    if (t == 0) {
+      // Rows of the input that are all zero stay zero (f(b) = 0); drop them before pooling.
+      launch_iota_i32(p->d_orig[0], p->cap, s);
+      launch_row_any(p->d_in, p->ld, p->d_counts, p->cap, N, p->d_flags, s);
+      launch_compact(p->d_flags, p->d_counts, p->d_counts + 1, p->d_map, nullptr, p->d_orig[0],
+                     p->d_orig[1], nullptr, nullptr, s);
+      count = p->d_counts + 1;
+    } else {
+      count = p->d_counts + t;
+    }
+    map = p->d_map;
+  } else {
+    if (t == 0) launch_iota_i32(p->d_orig[0], p->cap, s);
+    count = p->d_counts;
+    map = nullptr;
+  }
+  launch_fill_i32(p->d_rec, -1, p->cap, s);
+  launch_seg_sums(st.Yt, p->ld, map, count, p->cfg.sample_size, p->cfg.segments, N, p->d_F, s);
+  launch_select_exact(p->d_F, count, p->cfg.sample_size, p->cfg.segments, p->cfg.eps,
+                      p->d_cent_slot, p->d_k, s);
+  launch_gather_centroids(st.Yt, p->ld, map, p->d_cent_slot, p->d_k, N, p->d_C[0], p->kcap, s);
+  launch_assign_exact(st.Yt, p->ld, map, count, p->cap, p->d_C[0], p->kcap, p->d_k,
+                      p->d_cent_slot, N, p->d_flags, p->d_mcar[0], p->d_orig[cur], p->d_rec, s);
+  // Non-empty delta rows: list of (column in Yt, centroid, original id).
+  const int32_t* ident = nullptr;
+  launch_compact(p->d_flags, count, p->d_nerow + t, p->d_mapD, map ? map : ident, p->d_orig[cur],
+                 p->d_orig[cur ^ 1], p->d_mcar[0], p->d_mcar[1], s);
+  launch_make_delta(st.Yt, p->ld, p->d_mapD, p->d_C[0], p->kcap, p->d_mcar[1], p->d_nerow + t,
+                    p->cap, N, st.Da, p->ld, s);
+}
+
+// Stage "post" (src/SDGC/SNICIT/SNICIT.hpp:526-538,603-614): per layer the centroid rows,
+// then the delta rows with the fused residue epilogue, then the device-side recompaction.
+void enqueue_post(snicit_plan* p) {
+  hipStream_t s = p->net->ctx->stream;
+  const int t = p->cfg.threshold, L = p->net->L;
+  SnicitState st = snicit_state(p);
+  // clustering leaves the delta list's original ids in slot cur^1 and centroids in slot 1
+  int orig = cluster_cur(p) ^ 1, mc = 1;
+  for (int l = t; l < L; ++l) {
+    const int q = (l - t) & 1;
+    SpmmLaunch c = spmm_base(p, l, EPI_CENTROID);
+    c.yin = p->d_C[q];
+    c.ldin = p->kcap;
+    c.yout = p->d_C[q ^ 1];
+    c.ldout = p->kcap;
+    c.count = p->d_k;
+    c.zc = p->d_zc;
+    c.max_cols = p->kcap;
+    prof_mark(p, l, 2);
+    launch_panel_spmm(c, s);
+    prof_mark(p, l, 3);
+    SpmmLaunch d = spmm_base(p, l, p->cfg.cluster == SNICIT_CLUSTER_LOSSY ? EPI_DELTA_LOSSY
+                                                                          : EPI_DELTA);
+    d.yin = q == 0 ? st.Da : st.Db;
+    d.ldin = p->ld;
+    d.yout = q == 0 ? st.Db : st.Da;
+    d.ldout = p->ld;
+    d.map = l == t ? nullptr : p->d_mapD;
+    d.count = p->d_nerow + l;
+    d.flags = p->d_flags;
+    d.zc = p->d_zc;
+    d.mcent = p->d_mcar[mc];
+    prof_mark(p, l, 0);
+    launch_panel_spmm(d, s);
+    prof_mark(p, l, 1);
+    launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
+                   p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1], s);
+    orig ^= 1;
+    mc ^= 1;
+  }
+}
+
+void enqueue_recover(snicit_plan* p) {
+  hipStream_t s = p->net->ctx->stream;
+  const int L = p->net->L, N = p->net->N;
+  launch_centroid_cats(p->last_C, p->kcap, p->d_k, N, p->d_catc, s);
+  launch_recover_base(p->d_rec, p->d_counts, p->cap, p->d_catc, p->d_cat, s);
+  launch_recover_rows(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar], p->d_orig[p->last_orig],
+                      p->d_nerow + L, p->cap, p->last_C, p->kcap, N, p->d_act, p->d_cat, s);
+}
+
+// Runs `body` directly or through a captured graph cached in slot `g`.
+void run_stage(snicit_plan* p, int g, const std::function<void()>& body) {
+  hipStream_t s = p->net->ctx->stream;
+  if (!p->cfg.use_graph) {
+    body();
+    return;
+  }
+  if (!p->graph[g]) {
+    hipGraph_t graph = nullptr;
+    SNICIT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      hipGraph_t junk = nullptr;
+      (void)hipStreamEndCapture(s, &junk);
+      if (junk) (void)hipGraphDestroy(junk);
+      throw;
+    }
+    SNICIT_HIP(hipStreamEndCapture(s, &graph));
+    hipError_t e = hipGraphInstantiate(&p->graph[g], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SNICIT_HIP(e);
+  }
+  SNICIT_HIP(hipGraphLaunch(p->graph[g], s));
+}
+
+void drop_graphs(snicit_plan* p) {
+  for (auto& g : p->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
+  if (!p->have_input) throw Error(SNICIT_E_STATE, "plan has no input (call set_input first)");
+  Net* net = p->net;
+  hipStream_t s = net->ctx->stream;
+  const int L = net->L;
+  if (p->graph_stop != stop) {
+    drop_graphs(p);
+    p->graph_stop = stop;
+  }
+  const bool plain = plain_mode(p->cfg.mode);
+  SNICIT_HIP(hipEventRecord(p->ev[0], s));
+  if (plain) {
+    const int l_end = stop < 0 ? L : stop;
+    run_stage(p, 0, [&] {
+      const int c = enqueue_plain_layers(p, l_end, true);
+      enqueue_plain_categories(p, l_end, c);
+    });
+    // the stage body runs at capture only; where the final state lives follows from l_end
+    const int cur = compacting(p) ? (l_end & 1) : 0;
+    p->last_orig = cur;
+    p->last_Y = l_end == 0 ? p->d_in : p->d_y[(l_end - 1) & 1];
+    p->last_list = compacting(p) && l_end > 0;
+    p->stop_layer = l_end;
+    for (int i = 1; i < 5; ++i) SNICIT_HIP(hipEventRecord(p->ev[i], s));
+  } else {
+    const int t = p->cfg.threshold;
+    run_stage(p, 0, [&] { enqueue_plain_layers(p, t, false); });
+    SNICIT_HIP(hipEventRecord(p->ev[1], s));
+    run_stage(p, 1, [&] { enqueue_cluster(p); });
+    SNICIT_HIP(hipEventRecord(p->ev[2], s));
+    run_stage(p, 2, [&] { enqueue_post(p); });
+    {
+      // where the post stage leaves the final state (each post layer flips every slot)
+      SnicitState st = snicit_state(p);
+      const int n_post = L - t;
+      p->last_orig = (cluster_cur(p) ^ 1) ^ (n_post & 1);
+      p->last_mcar = 1 ^ (n_post & 1);
+      p->last_C = p->d_C[n_post & 1];
+      p->last_D = (((n_post - 1) & 1) == 0) ? st.Db : st.Da;
+    }
+    SNICIT_HIP(hipEventRecord(p->ev[3], s));
+    run_stage(p, 3, [&] { enqueue_recover(p); });
+    SNICIT_HIP(hipEventRecord(p->ev[4], s));
+    p->stop_layer = L;
+  }
+  SNICIT_HIP(hipEventSynchronize(p->ev[4]));
+  p->have_run = true;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float ms[4];
+    for (int i = 0; i < 4; ++i) SNICIT_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
+    stats->ms_pre = ms[0];
+    stats->ms_cluster = ms[1];
+    stats->ms_post = ms[2];
+    stats->ms_recover = ms[3];
+    float tot;
+    SNICIT_HIP(hipEventElapsedTime(&tot, p->ev[0], p->ev[4]));
+    stats->ms_total = tot;
+    stats->batch = p->B;
+    stats->layers = p->stop_layer;
+    stats->edges = (double)p->B * (double)net->N * (double)net->K * (double)p->stop_layer;
+    std::vector<int32_t> counts(L + 4), nerow(L + 2);
+    SNICIT_HIP(hipMemcpy(counts.data(), p->d_counts, counts.size() * 4, hipMemcpyDeviceToHost));
+    SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
+    int64_t rows = 0;
+    if (plain) {
+      for (int l = 0; l < p->stop_layer; ++l) rows += compacting(p) ? counts[l] : counts[0];
+    } else {
+      const int t = p->cfg.threshold;
+      for (int l = 0; l < t; ++l) rows += compacting(p) ? counts[l] : counts[0];
+      int32_t k = 0;
+      SNICIT_HIP(hipMemcpy(&k, p->d_k, 4, hipMemcpyDeviceToHost));
+      stats->centroids = k;
+      stats->live_at_threshold = compacting(p) ? counts[t == 0 ? 1 : t] : counts[0];
+      stats->nerow_first = nerow[t];
+      stats->nerow_last = nerow[L];
+      for (int l = t; l < L; ++l) rows += nerow[l] + k;
+    }
+    stats->rows_processed = rows;
+    std::vector<int32_t> cat(p->B);
+    SNICIT_HIP(hipMemcpy(cat.data(), p->d_cat, p->B * 4, hipMemcpyDeviceToHost));
+    int64_t act = 0;
+    for (int32_t c : cat) act += c != 0;
+    stats->active = act;
+  }
+}
+
+void plan_free(snicit_plan* p) {
+  drop_graphs(p);
+  for (auto& e : p->prof_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : p->ev)
+    if (e) (void)hipEventDestroy(e);
+  void* ptrs[] = {p->d_in,     p->d_y[0],    p->d_y[1],     p->d_flags,   p->d_map,
+                  p->d_mapD,   p->d_orig[0], p->d_orig[1],  p->d_mcar[0], p->d_mcar[1],
+                  p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
+                  p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
+                  p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
+                  p->d_stage};
+  for (void* q : ptrs) dev_free(q);
+}
+
+}  // namespace
+
+// ---- C-ABI ------------------------------------------------------------------------------
+extern "C" {
+
+const char* snicit_last_error(void) { return g_last_error.c_str(); }
+
+void snicit_run_cfg_default(snicit_run_cfg* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->mode = SNICIT_MODE_SNICIT;
+  cfg->threshold = 30;
+  cfg->cluster = SNICIT_CLUSTER_EXACT;
+  cfg->sample_size = 32;
+  cfg->segments = 16;
+  cfg->eps = 0.03f;
+  cfg->lossy_pool = 128;
+  cfg->lossy_eps_pool = 0.03f;
+  cfg->lossy_eps_assign = 0.04f;
+  cfg->lossy_eps_post = 0.05f;
+  cfg->use_graph = 1;
+}
+
+int snicit_ctx_create(int device, snicit_ctx** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(out != nullptr, "out is null");
+    int n = 0;
+    SNICIT_HIP(hipGetDeviceCount(&n));
+    SNICIT_REQUIRE(device >= 0 && device < n, "no such device " + std::to_string(device));
+    SNICIT_HIP(hipSetDevice(device));
+    auto c = std::make_unique<snicit_ctx>();
+    c->device = device;
+    SNICIT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    SNICIT_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cus = prop.multiProcessorCount;
+    *out = c.release();
+  });
+}
+
+int snicit_ctx_destroy(snicit_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, const int32_t* cols,
+                          const float* vals, int32_t bias_kind, const float* bias, float clip,
+                          snicit_net** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(ctx && out && cols && vals, "null argument");
+    SNICIT_REQUIRE(N > 0 && L > 0 && K > 0 && K <= kSlots,
+                   "need N > 0, L > 0 and 0 < K <= 32");
+    SNICIT_REQUIRE(clip > 0.f, "clip must be positive");
+    SNICIT_HIP(hipSetDevice(ctx->device));
+    auto net = std::make_unique<snicit_net>();
+    net->ctx = ctx;
+    net->N = N;
+    net->L = L;
+    net->K = K;
+    net->clip = clip;
+    net_set_bias(net.get(), bias_kind, bias);
+    net->panel_off.push_back(0);
+    NetBuilder nb{net.get()};
+    HostLayerPanels hp;
+    for (int32_t l = 0; l < L; ++l) {
+      pack_layer_ell(N, K, cols + (size_t)l * N * K, vals + (size_t)l * N * K, hp);
+      nb.add(hp);
+    }
+    try {
+      upload_net(net.get(), nb, 0.f);
+    } catch (...) {
+      destroy_net(net.get());
+      throw;
+    }
+    *out = net.release();
+  });
+}
+
+int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t seed, float weight,
+                             float bias, float clip, snicit_net** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(ctx && out, "null argument");
+    SNICIT_REQUIRE(L > 0, "L must be positive");
+    SNICIT_REQUIRE(clip > 0.f, "clip must be positive");
+    SNICIT_HIP(hipSetDevice(ctx->device));
+    auto net = std::make_unique<snicit_net>();
+    net->ctx = ctx;
+    net->N = N;
+    net->L = L;
+    net->K = kSlots;
+    net->clip = clip;
+    net_set_bias(net.get(), SNICIT_BIAS_SCALAR, &bias);
+    net->panel_off.push_back(0);
+    NetBuilder nb{net.get()};
+    nb.weights_given = false;
+    HostLayerPanels hp;
+    for (int32_t l = 0; l < L; ++l) {
+      sdgc_layer_panels(N, l, seed, hp);
+      nb.add(hp);
+    }
+    try {
+      upload_net(net.get(), nb, weight);
+    } catch (...) {
+      destroy_net(net.get());
+      throw;
+    }
+    *out = net.release();
+  });
+}
+
+int snicit_net_destroy(snicit_net* net) {
+  return guarded([&] {
+    if (!net) return;
+    destroy_net(net);
+    delete net;
+  });
+}
+
+int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,
+                    int64_t* panels_total, int64_t* device_bytes) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net != nullptr, "null net");
+    if (N) *N = net->N;
+    if (L) *L = net->L;
+    if (K) *K = net->K;
+    if (panels_total) *panels_total = net->panel_off.back();
+    if (device_bytes) *device_bytes = net->device_bytes;
+  });
+}
+
+int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_batch,
+                       snicit_plan** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net && cfg && out, "null argument");
+    SNICIT_REQUIRE(max_batch > 0 && max_batch < (int64_t(1) << 31) - 1024,
+                   "max_batch out of range");
+    SNICIT_REQUIRE(cfg->mode >= SNICIT_MODE_SNICIT && cfg->mode <= SNICIT_MODE_PLAIN_DENSE,
+                   "bad mode");
+    if (cfg->mode == SNICIT_MODE_SNICIT) {
+      SNICIT_REQUIRE(cfg->threshold >= 0 && cfg->threshold < net->L,
+                     "threshold must satisfy 0 <= t < L (got " + std::to_string(cfg->threshold) +
+                         ", L = " + std::to_string(net->L) + ")");
+      SNICIT_REQUIRE(cfg->cluster == SNICIT_CLUSTER_EXACT,
+                     "lossy clustering is not available for panel nets");
+      SNICIT_REQUIRE(cfg->sample_size >= 1 && cfg->sample_size <= kMaxCentroidsExact,
+                     "sample_size must be in [1, 32]");
+      SNICIT_REQUIRE(cfg->segments >= 1 && cfg->segments <= 32 && net->N % cfg->segments == 0,
+                     "segments must divide N and be <= 32");
+    }
+    SNICIT_HIP(hipSetDevice(net->ctx->device));
+    auto p = std::make_unique<snicit_plan>();
+    p->net = net;
+    p->cfg = *cfg;
+    p->cap = max_batch;
+    p->ld = (max_batch + 63) / 64 * 64;
+    const int N = net->N, L = net->L;
+    try {
+      int64_t* acct = &p->device_bytes;
+      p->d_in = dev_alloc<float>((size_t)N * p->ld, acct);
+      p->d_y[0] = dev_alloc<float>((size_t)N * p->ld, acct);
+      p->d_y[1] = dev_alloc<float>((size_t)N * p->ld, acct);
+      p->d_flags = dev_alloc<int32_t>(p->ld, acct);
+      p->d_map = dev_alloc<int32_t>(p->ld, acct);
+      p->d_mapD = dev_alloc<int32_t>(p->ld, acct);
+      for (int i = 0; i < 2; ++i) {
+        p->d_orig[i] = dev_alloc<int32_t>(p->ld, acct);
+        p->d_mcar[i] = dev_alloc<int32_t>(p->ld, acct);
+        p->d_C[i] = dev_alloc<float>((size_t)N * p->kcap, acct);
+      }
+      p->d_counts = dev_alloc<int32_t>(L + 4, acct);
+      p->d_nerow = dev_alloc<int32_t>(L + 2, acct);
+      p->d_k = dev_alloc<int32_t>(1, acct);
+      p->d_F = dev_alloc<float>(32 * 32, acct);
+      p->d_cent_slot = dev_alloc<int32_t>(64, acct);
+      p->d_zc = dev_alloc<float>((size_t)N * p->kcap, acct);
+      p->d_rec = dev_alloc<int32_t>(p->ld, acct);
+      p->d_catc = dev_alloc<int32_t>(p->kcap, acct);
+      p->d_act = dev_alloc<int32_t>(p->ld, acct);
+      p->d_cat = dev_alloc<int32_t>(p->ld, acct);
+      SNICIT_HIP(hipMemset(p->d_flags, 0, p->ld * 4));
+      SNICIT_HIP(hipMemset(p->d_act, 0, p->ld * 4));
+      SNICIT_HIP(hipMemset(p->d_counts, 0, (L + 4) * 4));
+      SNICIT_HIP(hipMemset(p->d_nerow, 0, (L + 2) * 4));
+      SNICIT_HIP(hipMemset(p->d_k, 0, 4));
+      SNICIT_HIP(hipMemset(p->d_C[0], 0, (size_t)N * p->kcap * 4));
+      SNICIT_HIP(hipMemset(p->d_C[1], 0, (size_t)N * p->kcap * 4));
+      SNICIT_HIP(hipMemset(p->d_zc, 0, (size_t)N * p->kcap * 4));
+      for (auto& e : p->ev) SNICIT_HIP(hipEventCreate(&e));
+    } catch (...) {
+      plan_free(p.get());
+      throw;
+    }
+    *out = p.release();
+  });
+}
+
+int snicit_plan_destroy(snicit_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    (void)hipStreamSynchronize(plan->net->ctx->stream);
+    plan_free(plan);
+    delete plan;
+  });
+}
+
+namespace {
+void ensure_stage(snicit_plan* p) {
+  if (!p->d_stage) p->d_stage = dev_alloc<float>((size_t)p->cap * p->net->N, &p->device_bytes);
+}
+void set_count(snicit_plan* p, int64_t B) {
+  int32_t b32 = (int32_t)B;
+  hipStream_t s = p->net->ctx->stream;
+  SNICIT_HIP(hipMemcpyAsync(p->d_counts, &b32, 4, hipMemcpyHostToDevice, s));
+  SNICIT_HIP(hipStreamSynchronize(s));
+  p->B = B;
+  p->have_input = true;
+  p->have_run = false;
+}
+}  // namespace
+
+int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p && (y0 || B == 0), "null argument");
+    SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
+    const int N = p->net->N;
+    hipStream_t s = p->net->ctx->stream;
+    if (B > 0) {
+      ensure_stage(p);
+      SNICIT_HIP(hipMemcpyAsync(p->d_stage, y0, (size_t)B * N * 4, hipMemcpyHostToDevice, s));
+      launch_transpose_in(p->d_stage, B, N, p->d_in, p->ld, s);
+    }
+    set_count(p, B);
+  });
+}
+
+int snicit_plan_set_input_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p != nullptr, "null plan");
+    SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
+    const int N = p->net->N;
+    hipStream_t s = p->net->ctx->stream;
+    ImageGeom g = image_geometry(N, seed);
+    if (B > 0) {
+      std::vector<int32_t> params((size_t)B * 4);
+      for (int64_t b = 0; b < B; ++b) {
+        SampleParams sp = image_sample_params(g, seed, first_sample + b);
+        params[b * 4 + 0] = sp.cls;
+        params[b * 4 + 1] = sp.dx;
+        params[b * 4 + 2] = sp.dy;
+        params[b * 4 + 3] = sp.keep;
+      }
+      std::vector<int32_t> blobs(g.blobs.size() * 4), proto(20);
+      for (size_t i = 0; i < g.blobs.size(); ++i) {
+        blobs[i * 4 + 0] = g.blobs[i].cx;
+        blobs[i * 4 + 1] = g.blobs[i].cy;
+        blobs[i * 4 + 2] = g.blobs[i].r;
+        blobs[i * 4 + 3] = 0;
+      }
+      for (int c = 0; c < 10; ++c) {
+        proto[c] = g.first_blob[c];
+        proto[10 + c] = g.nproto_blobs[c];
+      }
+      if (p->img_params_cap < B) {
+        dev_free(p->d_img_params);
+        p->d_img_params = dev_alloc<int32_t>((size_t)B * 4);
+        p->img_params_cap = B;
+      }
+      if (!p->d_img_blobs) p->d_img_blobs = dev_alloc<int32_t>(64 * 4);
+      if (!p->d_img_proto) p->d_img_proto = dev_alloc<int32_t>(20);
+      SNICIT_REQUIRE(blobs.size() <= 64 * 4, "too many blobs");
+      SNICIT_HIP(hipMemcpy(p->d_img_params, params.data(), params.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(p->d_img_blobs, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(p->d_img_proto, proto.data(), proto.size() * 4, hipMemcpyHostToDevice));
+      launch_images(g, p->d_img_params, p->d_img_blobs, p->d_img_proto, B, N, p->d_in, p->ld, s);
+    }
+    set_count(p, B);
+  });
+}
+
+int snicit_plan_run(snicit_plan* p, snicit_stats* stats) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p != nullptr, "null plan");
+    SNICIT_HIP(hipSetDevice(p->net->ctx->device));
+    plan_run(p, -1, stats);
+  });
+}
+
+int snicit_plan_run_layers(snicit_plan* p, int32_t stop_layer, snicit_stats* stats) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p != nullptr, "null plan");
+    SNICIT_REQUIRE(plain_mode(p->cfg.mode), "run_layers needs a plain mode");
+    SNICIT_REQUIRE(stop_layer >= 0 && stop_layer <= p->net->L, "stop_layer out of range");
+    SNICIT_HIP(hipSetDevice(p->net->ctx->device));
+    plan_run(p, stop_layer, stats);
+  });
+}
+
+int snicit_plan_categories(snicit_plan* p, int32_t* cat) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p && (cat || p->B == 0), "null argument");
+    if (!p->have_run) throw Error(SNICIT_E_STATE, "no completed run");
+    if (p->B) SNICIT_HIP(hipMemcpy(cat, p->d_cat, p->B * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int snicit_plan_activations(snicit_plan* p, float* y) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p && (y || p->B == 0), "null argument");
+    if (!p->have_run) throw Error(SNICIT_E_STATE, "no completed run");
+    if (p->B == 0) return;
+    ensure_stage(p);
+    Net* net = p->net;
+    const int N = net->N;
+    hipStream_t s = net->ctx->stream;
+    // Assemble Y_final[n * ld + b] in original sample order inside d_stage's companion
+    // scratch (the first plan buffer not holding final state), then transpose to [B][N].
+    float* full = nullptr;
+    full = dev_alloc<float>((size_t)N * p->ld);
+    try {
+      if (plain_mode(p->cfg.mode)) {
+        if (p->last_list) {
+          launch_fill_f32(full, 0.f, (int64_t)N * p->ld, s);
+          launch_scatter_cols(p->last_Y, p->ld, p->d_map, p->d_orig[p->last_orig],
+                              p->d_counts + p->stop_layer, p->cap, N, full, p->ld, s);
+        } else {
+          SNICIT_HIP(hipMemcpyAsync(full, p->last_Y, (size_t)N * p->ld * 4,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+      } else {
+        launch_expand_centroids(p->d_rec, p->d_counts, p->cap, p->last_C, p->kcap, N, full, p->ld,
+                                s);
+        launch_scatter_delta(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar],
+                             p->d_orig[p->last_orig], p->d_nerow + net->L, p->cap, p->last_C,
+                             p->kcap, N, full, p->ld, s);
+      }
+      launch_transpose_out(full, p->ld, p->B, N, p->d_stage, s);
+      SNICIT_HIP(hipMemcpyAsync(y, p->d_stage, (size_t)p->B * N * 4, hipMemcpyDeviceToHost, s));
+      SNICIT_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      dev_free(full);
+      throw;
+    }
+    dev_free(full);
+  });
+}
+
+int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) {
+  int64_t n = -1;
+  int rc = guarded([&] {
+    SNICIT_REQUIRE(p != nullptr, "null plan");
+    if (!p->have_run) throw Error(SNICIT_E_STATE, "no completed run");
+    const int L = p->net->L;
+    const int t = p->cfg.threshold;
+    const bool snic = !plain_mode(p->cfg.mode);
+    std::vector<int32_t> counts(L + 4), nerow(L + 2);
+    SNICIT_HIP(hipMemcpy(counts.data(), p->d_counts, counts.size() * 4, hipMemcpyDeviceToHost));
+    SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
+    int32_t k = 0;
+    SNICIT_HIP(hipMemcpy(&k, p->d_k, 4, hipMemcpyDeviceToHost));
+    const int64_t live_t =
+        snic ? (compacting(p) ? counts[t == 0 ? 1 : t] : counts[0]) : 0;
+    const int pool = snic ? (int)std::min<int64_t>(p->cfg.sample_size, live_t) : 0;
+    switch (what) {
+      case 0: {  // live rows after each layer
+        n = L;
+        if (out) {
+          int64_t* o = static_cast<int64_t*>(out);
+          for (int l = 0; l < std::min<int64_t>(L, cap); ++l) {
+            int64_t v;
+            if (!compacting(p)) v = -1;
+            else if (snic && l >= t) v = -1;
+            else if (l + 1 <= p->stop_layer || snic) v = counts[l + 1];
+            else v = -1;
+            o[l] = v;
+          }
+        }
+        break;
+      }
+      case 1: {
+        n = L;
+        if (out) {
+          int64_t* o = static_cast<int64_t*>(out);
+          for (int l = 0; l < std::min<int64_t>(L, cap); ++l)
+            o[l] = (snic && l >= t) ? nerow[l + 1] : 0;
+        }
+        break;
+      }
+      case 2:
+      case 4: {
+        if (!snic) throw Error(SNICIT_E_STATE, "trace needs a SNICIT run");
+        // pool rows / centroids as original ids: the orig list at t
+        std::vector<int32_t> slots;
+        if (what == 2) {
+          for (int s = 0; s < pool; ++s) slots.push_back(s);
+        } else {
+          std::vector<int32_t> cs(std::max(k, 1));
+          SNICIT_HIP(hipMemcpy(cs.data(), p->d_cent_slot, k * 4, hipMemcpyDeviceToHost));
+          slots.assign(cs.begin(), cs.begin() + k);
+        }
+        // rebuild the orig list at t on the host from the per-sample record: live rows at t
+        // in ascending original id are exactly the compacted list.
+        std::vector<int32_t> rec(p->B);
+        SNICIT_HIP(hipMemcpy(rec.data(), p->d_rec, p->B * 4, hipMemcpyDeviceToHost));
+        std::vector<int32_t> live;
+        for (int64_t b = 0; b < p->B; ++b)
+          if (rec[b] >= 0) live.push_back((int32_t)b);
+        n = (int64_t)slots.size();
+        if (out) {
+          int32_t* o = static_cast<int32_t*>(out);
+          for (int64_t i = 0; i < std::min<int64_t>(n, cap); ++i) o[i] = live[slots[i]];
+        }
+        break;
+      }
+      case 3: {
+        if (!snic) throw Error(SNICIT_E_STATE, "trace needs a SNICIT run");
+        n = (int64_t)pool * p->cfg.segments;
+        std::vector<float> F(32 * 32);
+        SNICIT_HIP(hipMemcpy(F.data(), p->d_F, F.size() * 4, hipMemcpyDeviceToHost));
+        if (out) std::memcpy(out, F.data(), std::min<int64_t>(n, cap) * 4);
+        break;
+      }
+      case 5: {
+        if (!snic) throw Error(SNICIT_E_STATE, "trace needs a SNICIT run");
+        n = p->B;
+        if (out && p->B) {
+          std::vector<int32_t> rec(p->B);
+          SNICIT_HIP(hipMemcpy(rec.data(), p->d_rec, p->B * 4, hipMemcpyDeviceToHost));
+          std::memcpy(out, rec.data(), std::min<int64_t>(n, cap) * 4);
+        }
+        break;
+      }
+      default:
+        throw Error(SNICIT_E_INVALID, "unknown trace kind");
+    }
+  });
+  return rc == SNICIT_OK ? n : rc;
+}
+
+int snicit_plan_profile(snicit_plan* p, double* ms, int64_t* rows, double* cent_ms,
+                        int64_t cap) {
+  return guarded([&] {
+    SNICIT_REQUIRE(p != nullptr, "null plan");
+    SNICIT_HIP(hipSetDevice(p->net->ctx->device));
+    const int L = p->net->L;
+    if (p->prof_ev.empty()) {
+      p->prof_ev.resize((size_t)4 * L);
+      for (auto& e : p->prof_ev) SNICIT_HIP(hipEventCreate(&e));
+    }
+    const int32_t saved_graph = p->cfg.use_graph;
+    p->cfg.use_graph = 0;
+    p->profiling = true;
+    try {
+      plan_run(p, -1, nullptr);
+    } catch (...) {
+      p->profiling = false;
+      p->cfg.use_graph = saved_graph;
+      throw;
+    }
+    p->profiling = false;
+    p->cfg.use_graph = saved_graph;
+    std::vector<int32_t> counts(L + 4), nerow(L + 2);
+    SNICIT_HIP(hipMemcpy(counts.data(), p->d_counts, counts.size() * 4, hipMemcpyDeviceToHost));
+    SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
+    const bool snic = !plain_mode(p->cfg.mode);
+    const int t = snic ? p->cfg.threshold : L;
+    for (int l = 0; l < std::min<int64_t>(L, cap); ++l) {
+      float a = 0.f, c = 0.f;
+      SNICIT_HIP(hipEventElapsedTime(&a, p->prof_ev[4 * l], p->prof_ev[4 * l + 1]));
+      if (snic && l >= t)
+        SNICIT_HIP(hipEventElapsedTime(&c, p->prof_ev[4 * l + 2], p->prof_ev[4 * l + 3]));
+      if (ms) ms[l] = a;
+      if (cent_ms) cent_ms[l] = c;
+      if (rows) rows[l] = (snic && l >= t) ? nerow[l] : (compacting(p) ? counts[l] : counts[0]);
+    }
+  });
+}
+
+int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
+                 int64_t batch, int32_t* cat, snicit_stats* stats) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net && cfg && (y0 || B == 0) && (cat || B == 0), "null argument");
+    SNICIT_REQUIRE(batch > 0, "batch must be positive");
+    SNICIT_REQUIRE(B % batch == 0, "batch must divide the number of inputs");
+    snicit_plan* p = nullptr;
+    int rc = snicit_plan_create(net, cfg, batch, &p);
+    if (rc) throw Error(rc, g_last_error);
+    std::unique_ptr<snicit_plan, int (*)(snicit_plan*)> hold(p, snicit_plan_destroy);
+    snicit_stats acc{};
+    for (int64_t off = 0; off < B; off += batch) {
+      rc = snicit_plan_set_input(p, y0 + (size_t)off * net->N, batch);
+      if (rc) throw Error(rc, g_last_error);
+      snicit_stats st{};
+      rc = snicit_plan_run(p, &st);
+      if (rc) throw Error(rc, g_last_error);
+      rc = snicit_plan_categories(p, cat + off);
+      if (rc) throw Error(rc, g_last_error);
+      acc.ms_total += st.ms_total;
+      acc.ms_pre += st.ms_pre;
+      acc.ms_cluster += st.ms_cluster;
+      acc.ms_post += st.ms_post;
+      acc.ms_recover += st.ms_recover;
+      acc.edges += st.edges;
+      acc.batch += st.batch;
+      acc.active += st.active;
+      acc.rows_processed += st.rows_processed;
+      acc.layers = st.layers;
+      acc.centroids = std::max(acc.centroids, st.centroids);
+      acc.live_at_threshold += st.live_at_threshold;
+      acc.nerow_first += st.nerow_first;
+      acc.nerow_last += st.nerow_last;
+    }
+    if (stats) *stats = acc;
+  });
+}
+
+int snicit_gen_sdgc_ell(int32_t N, int32_t L, uint64_t seed, int32_t* cols) {
+  return guarded([&] {
+    SNICIT_REQUIRE(cols != nullptr && L > 0, "bad argument");
+    for (int32_t l = 0; l < L; ++l) sdgc_layer_ell(N, l, seed, cols + (size_t)l * N * kSlots);
+  });
+}
+
+int snicit_gen_images(int32_t N, uint64_t seed, int64_t first_sample, int64_t B, float* out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(out != nullptr || B == 0, "null out");
+    ImageGeom g = image_geometry(N, seed);
+    for (int64_t b = 0; b < B; ++b)
+      image_render(g, image_sample_params(g, seed, first_sample + b), N, out + (size_t)b * N);
+  });
+}
+
+}  // extern "C"

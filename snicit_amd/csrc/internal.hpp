@@ -1,0 +1,119 @@
+// Internal declarations shared by the host C++ and the HIP kernels of libsnicit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/snicit.h"
+
+namespace snicit {
+
+// Every failure inside the library throws Error; the C-ABI layer converts it to a
+// status code and records the message for snicit_last_error().
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SNICIT_HIP(expr)                                                               \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::snicit::Error(SNICIT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + \
+                                               " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define SNICIT_REQUIRE(cond, msg)                                                      \
+  do {                                                                                 \
+    if (!(cond)) throw ::snicit::Error(SNICIT_E_INVALID, std::string(msg));            \
+  } while (0)
+
+// Panel slots: every panel lists exactly kSlots input neurons (rows with fewer nonzeros
+// are padded with value-0 slots).  32 = the fixed fan-in of the Graph Challenge nets.
+constexpr int kSlots = 32;
+// Maximum output rows per panel; larger sharing groups are split.
+constexpr int kMaxPanelRows = 64;
+// Centroid capacity of the exact variant (pool <= 32 rows => k <= 32).
+constexpr int kMaxCentroidsExact = 32;
+// Centroid capacity of the lossy variant (pool of 128 strided rows).
+constexpr int kMaxCentroidsLossy = 128;
+
+// A fan-in panel: rows of one layer that read the same kSlots input neurons.
+// Host form of one layer, used by the packer and the generator.
+struct HostLayerPanels {
+  std::vector<int32_t> pcols;  // [P][kSlots] input neuron per slot (ascending)
+  std::vector<int32_t> pbeg;   // [P+1] first output row (in panel order) of each panel
+  std::vector<int32_t> outs;   // [N] output neuron of each panel-order row
+  std::vector<float> w;        // [N][kSlots] weight of each slot, panel order (may be empty
+                               //  when the net is generated: all weights equal)
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+};
+
+struct Net {
+  Ctx* ctx = nullptr;
+  int32_t N = 0, L = 0, K = 0;
+  int32_t bias_kind = SNICIT_BIAS_SCALAR;
+  std::vector<float> bias_host;  // scalar: [1]; per layer: [L]; per neuron: [L*N]
+  float clip = 32.f;
+  bool skip_dead = true;  // every bias <= 0 -> an all-zero row stays all-zero
+  // Panel data, all layers concatenated.
+  std::vector<int64_t> panel_off;  // [L+1] first panel of each layer
+  int32_t* d_pcols = nullptr;      // [Ptot][kSlots]
+  int32_t* d_pbeg = nullptr;       // [Ptot + L] (P_l + 1 entries per layer)
+  int32_t* d_outs = nullptr;       // [L][N]
+  float* d_w = nullptr;            // [L][N][kSlots]
+  float* d_bias = nullptr;         // [L][N] when bias_kind == PER_NEURON
+  int64_t device_bytes = 0;
+
+  int64_t panels(int l) const { return panel_off[l + 1] - panel_off[l]; }
+  float bias_scalar(int l) const {
+    if (bias_kind == SNICIT_BIAS_SCALAR) return bias_host[0];
+    if (bias_kind == SNICIT_BIAS_PER_LAYER) return bias_host[l];
+    return 0.f;
+  }
+  const float* bias_vec(int l) const {
+    return bias_kind == SNICIT_BIAS_PER_NEURON ? d_bias + (size_t)l * N : nullptr;
+  }
+};
+
+// ---- host-side generators (gen.cpp) -------------------------------------------
+uint64_t sm_mix(uint64_t z);
+struct Rng {
+  uint64_t s;
+  uint64_t next();
+};
+void sdgc_layer_structure(int32_t N, int32_t l, uint64_t seed, int32_t* shift, int32_t* bit,
+                          std::vector<int32_t>& perm);
+void sdgc_layer_panels(int32_t N, int32_t l, uint64_t seed, HostLayerPanels& out);
+void sdgc_layer_ell(int32_t N, int32_t l, uint64_t seed, int32_t* cols /*N*32*/);
+
+struct ImageProto {
+  int32_t cx, cy, r, pad;
+};
+struct ImageGeom {
+  int32_t W, H, nproto_blobs[10], first_blob[10];
+  std::vector<ImageProto> blobs;
+};
+ImageGeom image_geometry(int32_t N, uint64_t seed);
+// Per-sample parameters: class, dx, dy, keep mask (bit b = blob b kept).
+struct SampleParams {
+  int32_t cls, dx, dy, keep;
+};
+SampleParams image_sample_params(const ImageGeom& g, uint64_t seed, int64_t sample);
+void image_render(const ImageGeom& g, const SampleParams& p, int32_t N, float* row);
+
+// ---- packing (net.cpp) -----------------------------------------------------------
+void pack_layer_ell(int32_t N, int32_t K, const int32_t* cols, const float* vals,
+                    HostLayerPanels& out);
+
+}  // namespace snicit

@@ -1,0 +1,670 @@
+// gfx950 kernels of the SNICIT layer loop.
+//
+// Layout: activations are feature-major, Y[n * ld + pos] (neuron n of the sample held in
+// column pos).  A wavefront owns 64 consecutive columns, so each neuron access is one
+// contiguous 256-byte run; weights, panel inputs and output ids are wavefront-uniform and
+// are fetched with scalar loads.  The bound is HBM bandwidth (one read and one write of
+// every live activation per layer); there is no MFMA.
+#include "kernels.hpp"
+
+namespace snicit {
+
+namespace {
+
+constexpr int kBlock = 256;   // 4 wavefronts of 64
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float act_clip(float z, float clip) {
+  // f(z) = min(max(z, 0), clip): src/SDGC/XY/kernel.hpp:5-7, src/SDGC/BF/kernel.hpp:79-82.
+  return fminf(fmaxf(z, 0.f), clip);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ int lane_prefix(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+// ---- panel SpMM ---------------------------------------------------------------------
+// Block (panel p, column tile): each wavefront gathers the panel's 32 inputs of its 64
+// columns into registers, then produces every output row of the panel with a fixed-order
+// fused multiply-add chain over the slots (ascending input neuron):
+//   acc = fma(w[31], x[31], ... fma(w[0], x[0], 0)).
+// Plain (src/SDGC/BF/kernel.hpp:57-88): y = f(acc + b), flag = any(y > 0).
+// Centroid / delta (src/SDGC/SNICIT/kernel.hpp:71-136): zc = acc + b for centroid rows;
+// delta rows y = f(zc[m] + acc) - f(zc[m]), flag = any(y != 0).
+// Lossy delta (src/beyond/SNICIT/kernel.hpp:141-168): |y| <= eps -> 0.
+// A wavefront whose 64 x 32 inputs are all zero skips the multiply-adds (acc = +0 exactly,
+// what the chain would give), the zero-input skipping of src/SDGC/SNICIT/kernel.hpp:96.
+template <int EPI>
+__global__ __launch_bounds__(kBlock) void panel_spmm_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int beg = pbeg[p], end = pbeg[p + 1];
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
+       base += (int64_t)gridDim.y * kBlock) {
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    float x[kSlots];
+    bool nz = false;
+    if (valid) {
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) {
+        x[r] = yin[(int64_t)pc[r] * ldin + src];
+        nz |= (x[r] != 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) x[r] = 0.f;
+    }
+    const bool wave_nz = __ballot(nz) != 0ull;
+    int32_t mi = 0;
+    if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = valid ? mcent[pos] : 0;
+    bool flag = false;
+    for (int j = beg; j < end; ++j) {
+      const float* __restrict__ wj = w + (size_t)j * kSlots;
+      float acc = 0.f;
+      if (wave_nz) {
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) acc = __builtin_fmaf(wj[r], x[r], acc);
+      }
+      const int o = outs[j];
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+      float y;
+      if (EPI == EPI_PLAIN) {
+        y = act_clip(acc + b, clip);
+        flag |= (y > 0.f);
+      } else if (EPI == EPI_CENTROID) {
+        const float z = acc + b;
+        if (valid) zc[(int64_t)o * kcap + pos] = z;
+        y = act_clip(z, clip);
+      } else {
+        const float z = valid ? zc[(int64_t)o * kcap + mi] : 0.f;
+        y = act_clip(z + acc, clip) - act_clip(z, clip);
+        if (EPI == EPI_DELTA_LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
+        flag |= (y != 0.f);
+      }
+      if (valid) yout[(int64_t)o * ldout + pos] = y;
+    }
+    if (flags && valid && flag) flags[pos] = 1;
+  }
+}
+
+// ---- stable compaction ----------------------------------------------------------------
+// One workgroup walks the flags in rounds of 1024 columns; a wavefront ballot plus a
+// 16-entry prefix gives each set flag its output slot, so the new list keeps column order.
+// Flags are cleared as they are consumed.  Replaces the per-layer host loops over managed
+// memory (src/SDGC/BF/BF.hpp:222-229, src/SDGC/SNICIT/SNICIT.hpp:556-615).
+__global__ __launch_bounds__(1024) void compact_kernel(
+    int32_t* __restrict__ flags, const int32_t* __restrict__ count_in,
+    int32_t* __restrict__ count_out, int32_t* __restrict__ map_out,
+    const int32_t* __restrict__ compose, const int32_t* __restrict__ c0_in,
+    int32_t* __restrict__ c0_out, const int32_t* __restrict__ c1_in,
+    int32_t* __restrict__ c1_out) {
+  __shared__ int32_t s_wave[16];
+  const int n = *count_in;
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
+  int running = 0;
+  for (int base = 0; base < n; base += 1024) {
+    const int i = base + t;
+    bool f = false;
+    int32_t cm = 0, v0 = 0, v1 = 0;
+    if (i < n) {
+      f = flags[i] != 0;
+      if (f) {
+        cm = compose ? compose[i] : i;
+        if (c0_in) v0 = c0_in[i];
+        if (c1_in) v1 = c1_in[i];
+        flags[i] = 0;
+      }
+    }
+    const uint64_t m = __ballot(f);
+    const int lp = lane_prefix(m);
+    if (lane == 0) s_wave[wv] = __popcll(m);
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = s_wave[q];
+      woff += (q < wv) ? c : 0;
+      tot += c;
+    }
+    if (f) {
+      const int o = running + woff + lp;
+      map_out[o] = cm;
+      if (c0_out) c0_out[o] = v0;
+      if (c1_out) c1_out[o] = v1;
+    }
+    running += tot;
+    __syncthreads();
+  }
+  if (t == 0) *count_out = running;
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int32_t v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void iota_i32_kernel(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+__global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// 64 x 64 tile transpose through LDS (65-float rows avoid bank conflicts).
+// in: [rows][cols] row-major, out[c * ldo + r].
+__global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
+                                                         int64_t rows, int64_t cols,
+                                                         int64_t ldi, float* __restrict__ out,
+                                                         int64_t ldo) {
+  __shared__ float tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t r = r0 + k, c = c0 + tx;
+    tile[k][tx] = (r < rows && c < cols) ? in[r * ldi + c] : 0.f;
+  }
+  __syncthreads();
+  for (int k = ty; k < 64; k += 4) {
+    const int64_t c = c0 + k, r = r0 + tx;
+    if (c < cols && r < rows) out[c * ldo + r] = tile[tx][k];
+  }
+}
+
+__global__ __launch_bounds__(256) void scatter_cols_kernel(
+    const float* __restrict__ src, int64_t ldsrc, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ dstcol, const int32_t* __restrict__ count_ptr, int32_t N,
+    float* __restrict__ dst, int64_t lddst) {
+  const int64_t count = *count_ptr;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int64_t s = map ? (int64_t)map[j] : j;
+  const int64_t d = dstcol[j];
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+  for (int n = n0; n < n1; ++n) dst[(int64_t)n * lddst + d] = src[(int64_t)n * ldsrc + s];
+}
+
+// Binary blob images, one column per thread: pixel n of sample b is on iff it lies in a
+// kept disc of the sample's prototype shifted by (dx, dy).
+__global__ __launch_bounds__(256) void images_kernel(const int32_t* __restrict__ params,
+                                                      const int32_t* __restrict__ blobs,
+                                                      const int32_t* __restrict__ proto,
+                                                      int32_t W, int32_t H, int64_t B,
+                                                      int32_t N, float* __restrict__ Y,
+                                                      int64_t ld) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int cls = params[b * 4 + 0], dx = params[b * 4 + 1], dy = params[b * 4 + 2],
+            keep = params[b * 4 + 3];
+  const int first = proto[cls], nb = proto[10 + cls];
+  const int n0 = blockIdx.y * 256, n1 = min(N, n0 + 256);
+  for (int n = n0; n < n1; ++n) {
+    const int x = n % W, y = n / W;
+    float v = 0.f;
+    if (y < H) {
+      for (int q = 0; q < nb; ++q) {
+        if (!((keep >> q) & 1)) continue;
+        const int32_t* bl = blobs + (first + q) * 4;
+        const int ddx = x - bl[0] - dx, ddy = y - bl[1] - dy;
+        if (ddx * ddx + ddy * ddy <= bl[2] * bl[2]) {
+          v = 1.f;
+          break;
+        }
+      }
+    }
+    Y[(int64_t)n * ld + b] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void row_any_kernel(const float* __restrict__ Y, int64_t ld,
+                                                       const int32_t* __restrict__ count_ptr,
+                                                       int32_t N, int32_t* __restrict__ flags) {
+  const int64_t count = *count_ptr;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count) return;
+  const int n0 = blockIdx.y * 256, n1 = min(N, n0 + 256);
+  bool any = false;
+  for (int n = n0; n < n1; ++n) any |= Y[(int64_t)n * ld + b] != 0.f;
+  if (any) flags[b] = 1;
+}
+
+__global__ void cat_from_list_kernel(const int32_t* __restrict__ orig,
+                                     const int32_t* __restrict__ count_ptr,
+                                     int32_t* __restrict__ cat) {
+  const int64_t count = *count_ptr;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += (int64_t)gridDim.x * blockDim.x)
+    cat[orig[j]] = 1;
+}
+__global__ void cat_from_flags_kernel(int32_t* __restrict__ flags,
+                                      const int32_t* __restrict__ count_ptr,
+                                      int32_t* __restrict__ cat) {
+  const int64_t count = *count_ptr;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    cat[j] = flags[j] != 0;
+    flags[j] = 0;
+  }
+}
+
+// ---- exact clustering ----------------------------------------------------------------
+// Pool row s (the s-th live row, s < min(pool, live)) gets `segments` sums over consecutive
+// blocks of N/segments neurons, each summed in ascending neuron order (the intended
+// segments of src/SDGC/SNICIT/kernel.hpp:138-153; see the oracle for the order).
+__global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict__ Y, int64_t ld,
+                                                         const int32_t* __restrict__ map,
+                                                         const int32_t* __restrict__ count_ptr,
+                                                         int32_t pool, int32_t segments,
+                                                         int32_t N, float* __restrict__ F) {
+  const int count = *count_ptr;
+  const int P = min(pool, count);
+  const int seglen = N / segments;
+  for (int it = threadIdx.x; it < pool * segments; it += blockDim.x) {
+    const int s = it / segments, g = it % segments;
+    float sum = 0.f;
+    if (s < P) {
+      const int64_t col = map ? (int64_t)map[s] : s;
+      const float* __restrict__ y = Y + (int64_t)g * seglen * ld + col;
+      for (int n = 0; n < seglen; ++n) sum += y[(int64_t)n * ld];
+    }
+    F[it] = sum;
+  }
+}
+
+// Alg. 1 sample pruning (src/SDGC/SNICIT/kernel.hpp:156-194): in pool order, every still
+// present base c removes each other present row y whose segment sums all lie within eps
+// of c's (count of |F_y - F_c| > eps below segments*eps).  Survivors are the centroids,
+// listed in pool order.  One wavefront, lane = pool row.
+__global__ __launch_bounds__(64) void select_exact_kernel(const float* __restrict__ F,
+                                                           const int32_t* __restrict__ count_ptr,
+                                                           int32_t pool, int32_t segments,
+                                                           float eps,
+                                                           int32_t* __restrict__ cent_slot,
+                                                           int32_t* __restrict__ k_out) {
+  const int lane = threadIdx.x;
+  const int P = min(pool, *count_ptr);
+  bool alive = lane < P;
+  for (int c = 0; c < P; ++c) {
+    const uint64_t am = __ballot(alive);
+    if (!((am >> c) & 1ull)) continue;
+    if (alive && lane != c) {
+      int cnt = 0;
+      for (int g = 0; g < segments; ++g)
+        cnt += fabsf(F[lane * segments + g] - F[c * segments + g]) > eps;
+      if ((double)cnt < (double)segments * (double)eps) alive = false;
+    }
+  }
+  const uint64_t am = __ballot(alive);
+  if (alive) cent_slot[lane_prefix(am)] = lane;
+  if (lane == 0) *k_out = __popcll(am);
+}
+
+__global__ __launch_bounds__(256) void gather_centroids_kernel(
+    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ cent_slot, const int32_t* __restrict__ k_ptr, int32_t N,
+    float* __restrict__ C, int32_t kcap) {
+  const int k = *k_ptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * kcap) return;
+  const int64_t n = i / kcap;
+  const int c = (int)(i % kcap);
+  float v = 0.f;
+  if (c < k) {
+    const int64_t slot = cent_slot[c];
+    const int64_t col = map ? (int64_t)map[slot] : slot;
+    v = Y[n * ld + col];
+  }
+  C[i] = v;
+}
+
+// Nearest centroid (src/SDGC/SNICIT/kernel.hpp:8-69): m(i) = argmin_c #{n : Y_i[n] != Y_c[n]},
+// first minimum in centroid order; centroids map to themselves.  flag = non-centroid with a
+// nonzero distance (its delta row is non-empty).  Centroid values are uniform across the
+// wavefront (scalar loads); eight running counts per pass over the row.
+__global__ __launch_bounds__(256) void assign_exact_kernel(
+    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap,
+    const int32_t* __restrict__ k_ptr, const int32_t* __restrict__ cent_slot, int32_t N,
+    int32_t* __restrict__ flags, int32_t* __restrict__ mcar, const int32_t* __restrict__ orig,
+    int32_t* __restrict__ rec) {
+  const int64_t count = *count_ptr;
+  const int k = *k_ptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int64_t)blockIdx.x * blockDim.x >= count) return;
+  const bool valid = i < count;
+  const int64_t col = valid ? (map ? (int64_t)map[i] : i) : 0;
+  int best = 0x7fffffff, bestc = 0;
+  for (int cb = 0; cb < k; cb += 8) {
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (valid) {
+      for (int n = 0; n < N; ++n) {
+        const float y = Y[(int64_t)n * ld + col];
+        const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cnt[q] += (y != cr[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (cb + q < k && cnt[q] < best) {
+        best = cnt[q];
+        bestc = cb + q;
+      }
+  }
+  if (!valid) return;
+  bool is_cent = false;
+  for (int c = 0; c < k; ++c)
+    if (cent_slot[c] == i) {
+      is_cent = true;
+      bestc = c;
+    }
+  mcar[i] = bestc;
+  rec[orig[i]] = bestc;
+  if (!is_cent && best > 0) flags[i] = 1;
+}
+
+__global__ __launch_bounds__(256) void make_delta_kernel(
+    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
+    const float* __restrict__ C, int32_t kcap, const int32_t* __restrict__ m,
+    const int32_t* __restrict__ count_ptr, int32_t N, float* __restrict__ D, int64_t ldD) {
+  const int64_t count = *count_ptr;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int64_t col = map[j];
+  const int c = m[j];
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+  for (int n = n0; n < n1; ++n)
+    D[(int64_t)n * ldD + j] = Y[(int64_t)n * ld + col] - C[(int64_t)n * kcap + c];
+}
+
+__global__ __launch_bounds__(256) void centroid_cats_kernel(const float* __restrict__ C,
+                                                             int32_t kcap,
+                                                             const int32_t* __restrict__ k_ptr,
+                                                             int32_t N,
+                                                             int32_t* __restrict__ catc) {
+  const int c = blockIdx.x;
+  if (c >= *k_ptr) return;
+  int any = 0;
+  for (int n = threadIdx.x; n < N; n += blockDim.x) any |= C[(int64_t)n * kcap + c] > 0.f;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) catc[c] = any ? 1 : 0;
+}
+
+__global__ void recover_base_kernel(const int32_t* __restrict__ rec,
+                                    const int32_t* __restrict__ B_ptr,
+                                    const int32_t* __restrict__ catc, int32_t* __restrict__ cat) {
+  const int64_t B = *B_ptr;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B;
+       b += (int64_t)gridDim.x * blockDim.x) {
+    const int r = rec[b];
+    cat[b] = r < 0 ? 0 : catc[r];
+  }
+}
+
+// active(i) = any(Yhat_i + Yhat_m(i) > 0) over a 256-neuron slice per block row.
+__global__ __launch_bounds__(256) void recover_rows_kernel(
+    const float* __restrict__ D, int64_t ldD, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ m, const int32_t* __restrict__ count_ptr,
+    const float* __restrict__ C, int32_t kcap, int32_t N, int32_t* __restrict__ act) {
+  const int64_t count = *count_ptr;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int64_t col = map[j];
+  const int c = m[j];
+  const int n0 = blockIdx.y * 256, n1 = min(N, n0 + 256);
+  bool any = false;
+  for (int n = n0; n < n1; ++n)
+    any |= (D[(int64_t)n * ldD + col] + C[(int64_t)n * kcap + c]) > 0.f;
+  if (any) act[j] = 1;
+}
+
+__global__ void recover_write_kernel(const int32_t* __restrict__ orig,
+                                     const int32_t* __restrict__ count_ptr,
+                                     int32_t* __restrict__ act, int32_t* __restrict__ cat) {
+  const int64_t count = *count_ptr;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < count;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    cat[orig[j]] = act[j];
+    act[j] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void expand_centroids_kernel(
+    const int32_t* __restrict__ rec, const int32_t* __restrict__ B_ptr,
+    const float* __restrict__ C, int32_t kcap, int32_t N, float* __restrict__ Y, int64_t ld) {
+  const int64_t B = *B_ptr;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const int r = rec[b];
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+  for (int n = n0; n < n1; ++n) Y[(int64_t)n * ld + b] = r < 0 ? 0.f : C[(int64_t)n * kcap + r];
+}
+
+__global__ __launch_bounds__(256) void scatter_delta_kernel(
+    const float* __restrict__ D, int64_t ldD, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ m, const int32_t* __restrict__ orig,
+    const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap, int32_t N,
+    float* __restrict__ Y, int64_t ld) {
+  const int64_t count = *count_ptr;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int64_t col = map[j], dst = orig[j];
+  const int c = m[j];
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+  for (int n = n0; n < n1; ++n)
+    Y[(int64_t)n * ld + dst] = D[(int64_t)n * ldD + col] + C[(int64_t)n * kcap + c];
+}
+
+inline unsigned grid1(int64_t n, int block, int64_t cap = 65535) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+inline void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    throw Error(SNICIT_E_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+}
+
+}  // namespace
+
+void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
+  const Net& net = *a.net;
+  const int l = a.layer;
+  const int64_t P = net.panels(l);
+  const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
+  // Enough workgroups to fill 256 CUs several times over; each loops over column tiles.
+  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(P, 1));
+  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
+  gy = std::min<int64_t>(gy, 65535);
+  SNICIT_REQUIRE(P <= 2147483647LL, "too many panels");
+  dim3 grid((unsigned)P, (unsigned)gy);
+  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
+  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
+  const int32_t* outs = net.d_outs + (size_t)l * net.N;
+  const float* w = net.d_w + (size_t)l * net.N * kSlots;
+  const float* bv = net.bias_vec(l);
+  const float bs = net.bias_scalar(l);
+#define SNICIT_SPMM(E)                                                                       \
+  hipLaunchKernelGGL(panel_spmm_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, bv, \
+                     bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags,    \
+                     a.zc, a.kcap, a.mcent, a.eps_post)
+  switch (a.epilogue) {
+    case EPI_PLAIN: SNICIT_SPMM(EPI_PLAIN); break;
+    case EPI_CENTROID: SNICIT_SPMM(EPI_CENTROID); break;
+    case EPI_DELTA: SNICIT_SPMM(EPI_DELTA); break;
+    case EPI_DELTA_LOSSY: SNICIT_SPMM(EPI_DELTA_LOSSY); break;
+    default: throw Error(SNICIT_E_INVALID, "bad epilogue");
+  }
+#undef SNICIT_SPMM
+  check_launch();
+}
+
+void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
+                    const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
+                    const int32_t* c1_in, int32_t* c1_out, hipStream_t s) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, s, flags, count_in, count_out,
+                     map_out, compose, c0_in, c0_out, c1_in, c1_out);
+  check_launch();
+}
+
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(grid1(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  check_launch();
+}
+void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(iota_i32_kernel, dim3(grid1(n, 256, 4096)), dim3(256), 0, s, p, n);
+  check_launch();
+}
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_f32_kernel, dim3(grid1(n, 256, 65535)), dim3(256), 0, s, p, v, n);
+  check_launch();
+}
+
+void launch_transpose_in(const float* src, int64_t B, int32_t N, float* dst, int64_t ld,
+                         hipStream_t s) {
+  dim3 grid((unsigned)((N + 63) / 64), (unsigned)((B + 63) / 64));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, src, B, (int64_t)N, (int64_t)N,
+                     dst, ld);
+  check_launch();
+}
+void launch_transpose_out(const float* src, int64_t ld, int64_t B, int32_t N, float* dst,
+                          hipStream_t s) {
+  dim3 grid((unsigned)((B + 63) / 64), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, src, (int64_t)N, B, ld, dst,
+                     (int64_t)N);
+  check_launch();
+}
+void launch_scatter_cols(const float* src, int64_t ldsrc, const int32_t* map,
+                         const int32_t* dstcol, const int32_t* count, int64_t max_cols,
+                         int32_t N, float* dst, int64_t lddst, hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(scatter_cols_kernel, grid, dim3(256), 0, s, src, ldsrc, map, dstcol, count,
+                     N, dst, lddst);
+  check_launch();
+}
+
+void launch_images(const ImageGeom& g, const int32_t* d_params, const int32_t* d_blobs,
+                   const int32_t* d_proto, int64_t B, int32_t N, float* Y, int64_t ld,
+                   hipStream_t s) {
+  dim3 grid(grid1(B, 256, 1 << 30), (unsigned)((N + 255) / 256));
+  hipLaunchKernelGGL(images_kernel, grid, dim3(256), 0, s, d_params, d_blobs, d_proto, g.W, g.H,
+                     B, N, Y, ld);
+  check_launch();
+}
+
+void launch_row_any(const float* Y, int64_t ld, const int32_t* count, int64_t max_cols,
+                    int32_t N, int32_t* flags, hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 255) / 256));
+  hipLaunchKernelGGL(row_any_kernel, grid, dim3(256), 0, s, Y, ld, count, N, flags);
+  check_launch();
+}
+void launch_cat_from_list(const int32_t* orig, const int32_t* count, int64_t max_cols,
+                          int32_t* cat, hipStream_t s) {
+  hipLaunchKernelGGL(cat_from_list_kernel, dim3(grid1(max_cols, 256, 4096)), dim3(256), 0, s,
+                     orig, count, cat);
+  check_launch();
+}
+void launch_cat_from_flags(int32_t* flags, const int32_t* count, int64_t max_cols, int32_t* cat,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(cat_from_flags_kernel, dim3(grid1(max_cols, 256, 4096)), dim3(256), 0, s,
+                     flags, count, cat);
+  check_launch();
+}
+
+void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
+                     int32_t pool, int32_t segments, int32_t N, float* F, hipStream_t s) {
+  hipLaunchKernelGGL(seg_sums_kernel, dim3(1), dim3(1024), 0, s, Y, ld, map, count, pool,
+                     segments, N, F);
+  check_launch();
+}
+void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int32_t segments,
+                         float eps, int32_t* cent_slot, int32_t* k_out, hipStream_t s) {
+  hipLaunchKernelGGL(select_exact_kernel, dim3(1), dim3(64), 0, s, F, count, pool, segments, eps,
+                     cent_slot, k_out);
+  check_launch();
+}
+void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
+                             const int32_t* cent_slot, const int32_t* k, int32_t N, float* C,
+                             int32_t kcap, hipStream_t s) {
+  hipLaunchKernelGGL(gather_centroids_kernel, dim3(grid1((int64_t)N * kcap, 256, 1 << 30)),
+                     dim3(256), 0, s, Y, ld, map, cent_slot, k, N, C, kcap);
+  check_launch();
+}
+void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
+                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
+                         const int32_t* cent_slot, int32_t N, int32_t* flags, int32_t* mcar,
+                         const int32_t* orig, int32_t* rec, hipStream_t s) {
+  hipLaunchKernelGGL(assign_exact_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
+                     Y, ld, map, count, C, kcap, k, cent_slot, N, flags, mcar, orig, rec);
+  check_launch();
+}
+void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
+                       int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
+                       int32_t N, float* D, int64_t ldD, hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(make_delta_kernel, grid, dim3(256), 0, s, Y, ld, map, C, kcap, m, count, N,
+                     D, ldD);
+  check_launch();
+}
+void launch_centroid_cats(const float* C, int32_t kcap, const int32_t* k, int32_t N,
+                          int32_t* catc, hipStream_t s) {
+  hipLaunchKernelGGL(centroid_cats_kernel, dim3((unsigned)kcap), dim3(256), 0, s, C, kcap, k, N,
+                     catc);
+  check_launch();
+}
+void launch_recover_base(const int32_t* rec, const int32_t* B, int64_t max_cols,
+                         const int32_t* catc, int32_t* cat, hipStream_t s) {
+  hipLaunchKernelGGL(recover_base_kernel, dim3(grid1(max_cols, 256, 4096)), dim3(256), 0, s, rec,
+                     B, catc, cat);
+  check_launch();
+}
+void launch_recover_rows(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
+                         const int32_t* orig, const int32_t* count, int64_t max_cols,
+                         const float* C, int32_t kcap, int32_t N, int32_t* act, int32_t* cat,
+                         hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 255) / 256));
+  hipLaunchKernelGGL(recover_rows_kernel, grid, dim3(256), 0, s, D, ldD, map, m, count, C, kcap,
+                     N, act);
+  check_launch();
+  hipLaunchKernelGGL(recover_write_kernel, dim3(grid1(max_cols, 256, 4096)), dim3(256), 0, s,
+                     orig, count, act, cat);
+  check_launch();
+}
+void launch_expand_centroids(const int32_t* rec, const int32_t* B, int64_t max_cols,
+                             const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
+                             hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(expand_centroids_kernel, grid, dim3(256), 0, s, rec, B, C, kcap, N, Y, ld);
+  check_launch();
+}
+void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
+                          const int32_t* orig, const int32_t* count, int64_t max_cols,
+                          const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
+                          hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(scatter_delta_kernel, grid, dim3(256), 0, s, D, ldD, map, m, orig, count, C,
+                     kcap, N, Y, ld);
+  check_launch();
+}
+
+}  // namespace snicit

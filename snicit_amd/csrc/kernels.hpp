@@ -1,0 +1,105 @@
+// Launch wrappers of the gfx950 kernels (kernels.hip).  Activations live feature-major:
+// Y[n * ld + pos] holds neuron n of the sample at column pos, so 64 consecutive samples
+// (one wavefront) read or write one 256-byte run of a neuron row.
+#pragma once
+
+#include "internal.hpp"
+
+namespace snicit {
+
+enum Epilogue : int {
+  EPI_PLAIN = 0,        // y = f(z + b); flag = any(y > 0)
+  EPI_CENTROID = 1,     // zc = z + b stored; y = f(zc)
+  EPI_DELTA = 2,        // y = f(zc[m] + z) - f(zc[m]); flag = any(y != 0)
+  EPI_DELTA_LOSSY = 3,  // as DELTA, then |y| <= eps -> 0
+};
+
+struct SpmmLaunch {
+  const Net* net;
+  int layer;
+  int epilogue;
+  const float* yin;
+  int64_t ldin;
+  float* yout;
+  int64_t ldout;
+  const int32_t* count;  // device: number of columns to produce
+  const int32_t* map;    // device: input column of each output column (null = identity)
+  int32_t* flags;        // device: per output column flag (null = none)
+  float* zc;             // centroid pre-activations [N][kcap]
+  int32_t kcap;
+  const int32_t* mcent;  // per output column centroid index (DELTA)
+  float eps_post;
+  int64_t max_cols;      // upper bound of *count (grid sizing)
+};
+void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
+
+void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
+                    const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
+                    const int32_t* c1_in, int32_t* c1_out, hipStream_t s);
+
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s);
+void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+
+// [B][N] row-major (device) -> Y[n*ld + b]
+void launch_transpose_in(const float* src, int64_t B, int32_t N, float* dst, int64_t ld,
+                         hipStream_t s);
+// Y[n*ld + b] -> [B][N] row-major (device), b < B
+void launch_transpose_out(const float* src, int64_t ld, int64_t B, int32_t N, float* dst,
+                          hipStream_t s);
+// dst[n*lddst + dstcol[j]] = src[n*ldsrc + (map ? map[j] : j)], j < *count
+void launch_scatter_cols(const float* src, int64_t ldsrc, const int32_t* map,
+                         const int32_t* dstcol, const int32_t* count, int64_t max_cols,
+                         int32_t N, float* dst, int64_t lddst, hipStream_t s);
+
+void launch_images(const ImageGeom& g, const int32_t* d_params /*B x 4*/,
+                   const int32_t* d_blobs /*nblobs x 4*/, const int32_t* d_proto /*20*/,
+                   int64_t B, int32_t N, float* Y, int64_t ld, hipStream_t s);
+
+// flags[b] = any(Y[:, b] != 0), b < *count
+void launch_row_any(const float* Y, int64_t ld, const int32_t* count, int64_t max_cols,
+                    int32_t N, int32_t* flags, hipStream_t s);
+// cat[orig[j]] = 1 for j < *count (cat pre-filled with 0)
+void launch_cat_from_list(const int32_t* orig, const int32_t* count, int64_t max_cols,
+                          int32_t* cat, hipStream_t s);
+// cat[j] = flags[j] != 0, flags reset, j < *count
+void launch_cat_from_flags(int32_t* flags, const int32_t* count, int64_t max_cols, int32_t* cat,
+                           hipStream_t s);
+
+// ---- exact clustering (src/SDGC/SNICIT/kernel.hpp:8-69,138-194) -------------------
+void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
+                     int32_t pool, int32_t segments, int32_t N, float* F, hipStream_t s);
+void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int32_t segments,
+                         float eps, int32_t* cent_slot, int32_t* k_out, hipStream_t s);
+void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
+                             const int32_t* cent_slot, const int32_t* k, int32_t N, float* C,
+                             int32_t kcap, hipStream_t s);
+void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
+                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
+                         const int32_t* cent_slot, int32_t N, int32_t* flags, int32_t* mcar,
+                         const int32_t* orig, int32_t* rec, hipStream_t s);
+// D[n*ldD + j] = Y[n*ld + map[j]] - C[n*kcap + m[j]]
+void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
+                       int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
+                       int32_t N, float* D, int64_t ldD, hipStream_t s);
+
+// ---- recovery (src/SDGC/SNICIT/kernel.hpp:197-218) ---------------------------------
+void launch_centroid_cats(const float* C, int32_t kcap, const int32_t* k, int32_t N,
+                          int32_t* catc, hipStream_t s);
+void launch_recover_base(const int32_t* rec, const int32_t* B, int64_t max_cols,
+                         const int32_t* catc, int32_t* cat, hipStream_t s);
+void launch_recover_rows(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
+                         const int32_t* orig, const int32_t* count, int64_t max_cols,
+                         const float* C, int32_t kcap, int32_t N, int32_t* act, int32_t* cat,
+                         hipStream_t s);
+// Yfull[n*ld + b] = rec[b] < 0 ? 0 : C[n*kcap + rec[b]]
+void launch_expand_centroids(const int32_t* rec, const int32_t* B, int64_t max_cols,
+                             const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
+                             hipStream_t s);
+// Y[n*ld + orig[j]] = D[n*ldD + map[j]] + C[n*kcap + m[j]]
+void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
+                          const int32_t* orig, const int32_t* count, int64_t max_cols,
+                          const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
+                          hipStream_t s);
+
+}  // namespace snicit
