@@ -58,18 +58,16 @@ def cpu_baseline(wl, gpu_cat, target_s):
     cores = min(16, os.cpu_count() or 1)
     cols = sa.gen_sdgc_cols(N, L, NET_SEED)
     vals = np.full(cols.shape, 0.0625, np.float32)
-    # calibrate on a small sample, then size the timed sample for ~target_s seconds
-    cal = 32
-    y0 = sa.gen_images(N, IMG_SEED, 0, cal)
-    t0 = time.perf_counter()
-    orc.plain_f32(cols, vals, wl["bias"], 32.0, y0, nthreads=cores, want_y=False)
-    per = (time.perf_counter() - t0) / cal
-    n = int(max(cal, min(wl["B"], target_s / max(per, 1e-6))))
-    n = max(cores, n // cores * cores)
-    y0 = sa.gen_images(N, IMG_SEED, 0, n)
-    t0 = time.perf_counter()
-    _, cat, _ = orc.plain_f32(cols, vals, wl["bias"], 32.0, y0, nthreads=cores, want_y=False)
-    dt = time.perf_counter() - t0
+    # grow the sample (first n samples of the batch) until the timed loop takes about
+    # target_s seconds; per-sample cost is not linear (dead rows are skipped)
+    n = 4 * cores
+    for _ in range(6):
+        y0 = sa.gen_images(N, IMG_SEED, 0, n)
+        _, cat, _, dt = orc.plain_f32(cols, vals, wl["bias"], 32.0, y0, nthreads=cores,
+                                      want_y=False, timing=True)
+        if dt >= 0.5 * target_s or n >= wl["B"]:
+            break
+        n = min(wl["B"], max(n + cores, int(n * 0.9 * target_s / max(dt, 1e-3))))
     edges = float(n) * N * 32 * L
     match = bool(np.array_equal(cat, gpu_cat[:n]))
     return {"value": edges / dt, "unit": "edges/s", "cores": cores, "kind": "port",

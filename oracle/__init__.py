@@ -29,6 +29,8 @@ def lib():
         L.oracle_plain_f32.argtypes = [ctypes.c_int] * 3 + [_I32P, _F32P, ctypes.c_int, _F32P,
                                                             ctypes.c_float, ctypes.c_int64, _F32P,
                                                             _F32P, _I32P, _I64P, ctypes.c_int]
+        L.oracle_plain_f32_ex.restype = ctypes.c_int
+        L.oracle_plain_f32_ex.argtypes = L.oracle_plain_f32.argtypes + [_F64P]
         L.oracle_plain_f64.restype = ctypes.c_int
         L.oracle_plain_f64.argtypes = [ctypes.c_int] * 3 + [_I32P, _F32P, ctypes.c_int, _F32P,
                                                             ctypes.c_float, ctypes.c_int64, _F32P,
@@ -57,7 +59,9 @@ def _bias(bias, L, N):
     return kind, b
 
 
-def plain_f32(cols, vals, bias, clip, y0, nthreads=0, want_y=True):
+def plain_f32(cols, vals, bias, clip, y0, nthreads=0, want_y=True, timing=False):
+    """Plain FP32 forward.  Returns (yL, cat, live_per_layer) and, with timing=True, also the
+    wall seconds of the layer loop alone."""
     cols = np.ascontiguousarray(cols, np.int32)
     vals = np.ascontiguousarray(vals, np.float32)
     y0 = np.ascontiguousarray(y0, np.float32)
@@ -67,10 +71,13 @@ def plain_f32(cols, vals, bias, clip, y0, nthreads=0, want_y=True):
     yL = np.empty((B, N), np.float32) if want_y else None
     cat = np.empty(B, np.int32)
     live = np.empty(L, np.int64)
-    rc = lib().oracle_plain_f32(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind, _p(b, _F32P),
-                                clip, B, _p(y0, _F32P), _p(yL, _F32P), _p(cat, _I32P),
-                                _p(live, _I64P), nthreads)
+    secs = np.zeros(1, np.float64)
+    rc = lib().oracle_plain_f32_ex(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind,
+                                   _p(b, _F32P), clip, B, _p(y0, _F32P), _p(yL, _F32P),
+                                   _p(cat, _I32P), _p(live, _I64P), nthreads, _p(secs, _F64P))
     assert rc == 0
+    if timing:
+        return yL, cat, live, float(secs[0])
     return yL, cat, live
 
 

@@ -119,9 +119,24 @@ static int row_nonzero(const float* y, int N) {
  * those rows are skipped (BF's rowsY, BF.hpp:222-229).  category(b) = any(Y_L[b] > 0)
  * (identify, src/SDGC/utility.hpp:194-212: sum > 0 with y >= 0).
  * live_per_layer[l] = rows with a nonzero after layer l (nullable). */
+int oracle_plain_f32_ex(int N, int L, int K, const int32_t* cols, const float* vals,
+                        int bias_kind, const float* bias, float clip, int64_t B, const float* y0,
+                        float* yL, int32_t* cat, int64_t* live_per_layer, int nthreads,
+                        double* forward_seconds);
+
 int oracle_plain_f32(int N, int L, int K, const int32_t* cols, const float* vals, int bias_kind,
                      const float* bias, float clip, int64_t B, const float* y0, float* yL,
                      int32_t* cat, int64_t* live_per_layer, int nthreads) {
+  return oracle_plain_f32_ex(N, L, K, cols, vals, bias_kind, bias, clip, B, y0, yL, cat,
+                             live_per_layer, nthreads, NULL);
+}
+
+/* As oracle_plain_f32; forward_seconds (nullable) gets the wall time of the layer loop
+ * alone (the one-off sort of the rows is excluded). */
+int oracle_plain_f32_ex(int N, int L, int K, const int32_t* cols, const float* vals,
+                        int bias_kind, const float* bias, float clip, int64_t B, const float* y0,
+                        float* yL, int32_t* cat, int64_t* live_per_layer, int nthreads,
+                        double* forward_seconds) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
   sorted_net s;
   if (sort_net(N, L, K, cols, vals, &s)) return -1;
@@ -132,6 +147,7 @@ int oracle_plain_f32(int N, int L, int K, const int32_t* cols, const float* vals
   if (!a || !b || !live) return -1;
   memcpy(a, y0, (size_t)B * N * sizeof(float));
   for (int64_t i = 0; i < B; ++i) live[i] = 1;
+  const double t0 = omp_get_wtime();
   for (int l = 0; l < L; ++l) {
     int64_t nlive = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : nlive)
@@ -156,6 +172,7 @@ int oracle_plain_f32(int N, int L, int K, const int32_t* cols, const float* vals
     a = b;
     b = t;
   }
+  if (forward_seconds) *forward_seconds = omp_get_wtime() - t0;
   for (int64_t i = 0; i < B; ++i) {
     const float* y = a + (size_t)i * N;
     int any = 0;
@@ -294,11 +311,19 @@ int oracle_snicit_exact(int N, int L, int K, const int32_t* cols, const float* v
   const int P = (int)(Bp < sample_size ? Bp : sample_size);
   const int seglen = N / segments;
   float* F = (float*)calloc((size_t)(P > 0 ? P : 1) * segments, sizeof(float));
+  /* F: segment g split into S2 = min(32, seglen) contiguous sub-chunks, each summed in
+   * ascending neuron order, partials added in ascending order (a fixed order for the
+   * reference's unordered shared-memory atomics, kernel.hpp:147-150). */
+  const int S2 = seglen < 32 ? seglen : 32;
   for (int q = 0; q < P; ++q) {
     const float* y = a + (size_t)lid[q] * N;
     for (int g = 0; g < segments; ++g) {
       float sum = 0.f;
-      for (int n = 0; n < seglen; ++n) sum += y[g * seglen + n];
+      for (int c = 0; c < S2; ++c) {
+        float part = 0.f;
+        for (int n = c * seglen / S2; n < (c + 1) * seglen / S2; ++n) part += y[g * seglen + n];
+        sum += part;
+      }
       F[q * segments + g] = sum;
     }
   }

@@ -146,6 +146,7 @@ struct snicit_plan {
   float* d_C[2] = {nullptr, nullptr};
   float* d_zc = nullptr;
   int32_t* d_rec = nullptr;
+  int32_t* d_dist = nullptr;  // [cap][32] distance accumulators (kept zero between runs)
   int32_t* d_catc = nullptr;
   int32_t* d_act = nullptr;
   int32_t* d_cat = nullptr;
@@ -298,7 +299,8 @@ void enqueue_cluster(snicit_plan* p) {
                       p->d_cent_slot, p->d_k, s);
   launch_gather_centroids(st.Yt, p->ld, map, p->d_cent_slot, p->d_k, N, p->d_C[0], p->kcap, s);
   launch_assign_exact(st.Yt, p->ld, map, count, p->cap, p->d_C[0], p->kcap, p->d_k,
-                      p->d_cent_slot, N, p->d_flags, p->d_mcar[0], p->d_orig[cur], p->d_rec, s);
+                      p->d_cent_slot, N, p->d_dist, p->d_flags, p->d_mcar[0], p->d_orig[cur],
+                      p->d_rec, s);
   // Non-empty delta rows: list of (column in Yt, centroid, original id).
   const int32_t* ident = nullptr;
   launch_compact(p->d_flags, count, p->d_nerow + t, p->d_mapD, map ? map : ident, p->d_orig[cur],
@@ -489,6 +491,7 @@ void plan_free(snicit_plan* p) {
                   p->d_mapD,   p->d_orig[0], p->d_orig[1],  p->d_mcar[0], p->d_mcar[1],
                   p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
                   p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
+                  p->d_dist,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
                   p->d_stage};
   for (void* q : ptrs) dev_free(q);
@@ -673,6 +676,8 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       p->d_cent_slot = dev_alloc<int32_t>(64, acct);
       p->d_zc = dev_alloc<float>((size_t)N * p->kcap, acct);
       p->d_rec = dev_alloc<int32_t>(p->ld, acct);
+      p->d_dist = dev_alloc<int32_t>((size_t)p->ld * kMaxCentroidsExact, acct);
+      SNICIT_HIP(hipMemset(p->d_dist, 0, (size_t)p->ld * kMaxCentroidsExact * 4));
       p->d_catc = dev_alloc<int32_t>(p->kcap, acct);
       p->d_act = dev_alloc<int32_t>(p->ld, acct);
       p->d_cat = dev_alloc<int32_t>(p->ld, acct);

@@ -16,7 +16,8 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ float act_clip(float z, float clip) {
   // f(z) = min(max(z, 0), clip): src/SDGC/XY/kernel.hpp:5-7, src/SDGC/BF/kernel.hpp:79-82.
-  return fminf(fmaxf(z, 0.f), clip);
+  // One v_med3_f32 (equal to fminf(fmaxf(z, 0), clip) for every non-NaN z when clip >= 0).
+  return __builtin_amdgcn_fmed3f(z, 0.f, clip);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -24,6 +25,31 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 __device__ __forceinline__ int lane_prefix(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+// Column counts up to this use the narrow path (one thread per output x column).
+constexpr int kNarrowCols = 64;
+
+// Epilogue of one output value (see panel_spmm_kernel).
+template <int EPI>
+__device__ __forceinline__ float epilogue(float acc, float b, float clip, bool valid, int o,
+                                          int64_t pos, int32_t mi, float* __restrict__ zc,
+                                          int32_t kcap, float eps_post, bool& flag) {
+  float y;
+  if (EPI == EPI_PLAIN) {
+    y = act_clip(acc + b, clip);
+    flag |= (y > 0.f);
+  } else if (EPI == EPI_CENTROID) {
+    const float z = acc + b;
+    if (valid) zc[(int64_t)o * kcap + pos] = z;
+    y = act_clip(z, clip);
+  } else {
+    const float z = valid ? zc[(int64_t)o * kcap + mi] : 0.f;
+    y = act_clip(z + acc, clip) - act_clip(z, clip);
+    if (EPI == EPI_DELTA_LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
+    flag |= (y != 0.f);
+  }
+  return y;
 }
 
 // ---- panel SpMM ---------------------------------------------------------------------
@@ -37,8 +63,11 @@ __device__ __forceinline__ int lane_prefix(uint64_t mask) {
 // Lossy delta (src/beyond/SNICIT/kernel.hpp:141-168): |y| <= eps -> 0.
 // A wavefront whose 64 x 32 inputs are all zero skips the multiply-adds (acc = +0 exactly,
 // what the chain would give), the zero-input skipping of src/SDGC/SNICIT/kernel.hpp:96.
+// With at most kNarrowCols columns the wavefront-per-64-columns layout would leave one
+// wavefront walking the panel's rows behind dependent scalar loads, so those launches run
+// one thread per (row, column) pair instead, with the same chain and epilogue.
 template <int EPI>
-__global__ __launch_bounds__(kBlock) void panel_spmm_kernel(
+__global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
     const int32_t* __restrict__ outs, const float* __restrict__ w,
     const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
@@ -47,10 +76,38 @@ __global__ __launch_bounds__(kBlock) void panel_spmm_kernel(
     int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
-  const int lane = lane_id();
-  const int wv = threadIdx.x / kWave;
   const int beg = pbeg[p], end = pbeg[p + 1];
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  if (count <= kNarrowCols) {
+    if (blockIdx.y != 0) return;
+    const int G = end - beg;
+    const int64_t pairs = (int64_t)G * count;
+    for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
+      const int j = beg + (int)(q % G);
+      const int64_t pos = q / G;
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+      const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
+      float acc = 0.f;
+#pragma unroll 2
+      for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+        const float4 wv = w4[r4];
+        acc = __builtin_fmaf(wv.x, yin[(int64_t)pc[4 * r4 + 0] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.y, yin[(int64_t)pc[4 * r4 + 1] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.z, yin[(int64_t)pc[4 * r4 + 2] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.w, yin[(int64_t)pc[4 * r4 + 3] * ldin + src], acc);
+      }
+      const int o = outs[j];
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+      const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
+      bool flag = false;
+      const float y = epilogue<EPI>(acc, b, clip, true, o, pos, mi, zc, kcap, eps_post, flag);
+      yout[(int64_t)o * ldout + pos] = y;
+      if (flags && flag) flags[pos] = 1;
+    }
+    return;
+  }
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
   for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
        base += (int64_t)gridDim.y * kBlock) {
     const int64_t pos = base + lane;
@@ -81,20 +138,7 @@ __global__ __launch_bounds__(kBlock) void panel_spmm_kernel(
       }
       const int o = outs[j];
       const float b = bias_vec ? bias_vec[o] : bias_s;
-      float y;
-      if (EPI == EPI_PLAIN) {
-        y = act_clip(acc + b, clip);
-        flag |= (y > 0.f);
-      } else if (EPI == EPI_CENTROID) {
-        const float z = acc + b;
-        if (valid) zc[(int64_t)o * kcap + pos] = z;
-        y = act_clip(z, clip);
-      } else {
-        const float z = valid ? zc[(int64_t)o * kcap + mi] : 0.f;
-        y = act_clip(z + acc, clip) - act_clip(z, clip);
-        if (EPI == EPI_DELTA_LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
-        flag |= (y != 0.f);
-      }
+      const float y = epilogue<EPI>(acc, b, clip, valid, o, pos, mi, zc, kcap, eps_post, flag);
       if (valid) yout[(int64_t)o * ldout + pos] = y;
     }
     if (flags && valid && flag) flags[pos] = 1;
@@ -102,51 +146,80 @@ __global__ __launch_bounds__(kBlock) void panel_spmm_kernel(
 }
 
 // ---- stable compaction ----------------------------------------------------------------
-// One workgroup walks the flags in rounds of 1024 columns; a wavefront ballot plus a
-// 16-entry prefix gives each set flag its output slot, so the new list keeps column order.
-// Flags are cleared as they are consumed.  Replaces the per-layer host loops over managed
-// memory (src/SDGC/BF/BF.hpp:222-229, src/SDGC/SNICIT/SNICIT.hpp:556-615).
+// One workgroup of 1024 threads walks the flags in rounds of 16 x 1024 columns: sixteen
+// coalesced loads per thread are in flight together, a ballot per 1024-column sub-round
+// gives per-wavefront counts, one wavefront scans the 256 counts, and every set flag gets
+// its output slot, so the new list keeps column order.  Flags are cleared as they are
+// consumed.  compose (optional) maps each flagged column to the value stored in map_out; it
+// must not alias map_out.  Replaces the per-layer host loops over managed memory
+// (src/SDGC/BF/BF.hpp:222-229, src/SDGC/SNICIT/SNICIT.hpp:556-615).
+constexpr int kCompactSub = 16;
 __global__ __launch_bounds__(1024) void compact_kernel(
     int32_t* __restrict__ flags, const int32_t* __restrict__ count_in,
     int32_t* __restrict__ count_out, int32_t* __restrict__ map_out,
     const int32_t* __restrict__ compose, const int32_t* __restrict__ c0_in,
     int32_t* __restrict__ c0_out, const int32_t* __restrict__ c1_in,
     int32_t* __restrict__ c1_out) {
-  __shared__ int32_t s_wave[16];
+  __shared__ int32_t s_cnt[kCompactSub * 16];
+  __shared__ int32_t s_off[kCompactSub * 16];
+  __shared__ int32_t s_tot;
   const int n = *count_in;
   const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
   int running = 0;
-  for (int base = 0; base < n; base += 1024) {
-    const int i = base + t;
-    bool f = false;
-    int32_t cm = 0, v0 = 0, v1 = 0;
-    if (i < n) {
-      f = flags[i] != 0;
-      if (f) {
-        cm = compose ? compose[i] : i;
-        if (c0_in) v0 = c0_in[i];
-        if (c1_in) v1 = c1_in[i];
-        flags[i] = 0;
+  for (int base = 0; base < n; base += 1024 * kCompactSub) {
+    // every load of the round is issued before the scan: flags and the carried values
+    bool f[kCompactSub];
+    int32_t cm[kCompactSub], v0[kCompactSub], v1[kCompactSub];
+#pragma unroll
+    for (int q = 0; q < kCompactSub; ++q) {
+      const int i = base + q * 1024 + t;
+      const bool in = i < n;
+      f[q] = in && flags[i] != 0;
+      cm[q] = (in && compose) ? compose[i] : i;
+      v0[q] = (in && c0_in) ? c0_in[i] : 0;
+      v1[q] = (in && c1_in) ? c1_in[i] : 0;
+    }
+    uint64_t m[kCompactSub];
+#pragma unroll
+    for (int q = 0; q < kCompactSub; ++q) {
+      m[q] = __ballot(f[q]);
+      if (lane == 0) s_cnt[q * 16 + wv] = __popcll(m[q]);
+    }
+    __syncthreads();
+    if (wv == 0) {
+      // entries (sub-round q, wavefront w) in order e = 16 q + w; lane L scans e = 4L .. 4L+3
+      int v[4], tot4 = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = s_cnt[4 * lane + e];
+        tot4 += v[e];
+      }
+      int incl = tot4;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int x = __shfl_up(incl, d, kWave);
+        if (lane >= d) incl += x;
+      }
+      int ex = incl - tot4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        s_off[4 * lane + e] = ex;
+        ex += v[e];
+      }
+      if (lane == kWave - 1) s_tot = incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kCompactSub; ++q) {
+      if (f[q]) {
+        const int o = running + s_off[q * 16 + wv] + lane_prefix(m[q]);
+        map_out[o] = cm[q];
+        if (c0_out) c0_out[o] = v0[q];
+        if (c1_out) c1_out[o] = v1[q];
+        flags[base + q * 1024 + t] = 0;
       }
     }
-    const uint64_t m = __ballot(f);
-    const int lp = lane_prefix(m);
-    if (lane == 0) s_wave[wv] = __popcll(m);
-    __syncthreads();
-    int woff = 0, tot = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = s_wave[q];
-      woff += (q < wv) ? c : 0;
-      tot += c;
-    }
-    if (f) {
-      const int o = running + woff + lp;
-      map_out[o] = cm;
-      if (c0_out) c0_out[o] = v0;
-      if (c1_out) c1_out[o] = v1;
-    }
-    running += tot;
+    running += s_tot;
     __syncthreads();
   }
   if (t == 0) *count_out = running;
@@ -266,25 +339,45 @@ __global__ void cat_from_flags_kernel(int32_t* __restrict__ flags,
 
 // ---- exact clustering ----------------------------------------------------------------
 // Pool row s (the s-th live row, s < min(pool, live)) gets `segments` sums over consecutive
-// blocks of N/segments neurons, each summed in ascending neuron order (the intended
-// segments of src/SDGC/SNICIT/kernel.hpp:138-153; see the oracle for the order).
+// blocks of N/segments neurons (the intended segments of src/SDGC/SNICIT/kernel.hpp:138-153,
+// which accumulates them with shared-memory atomics in no fixed order).  Fixed order here:
+// each segment splits into S2 = min(32, seglen) contiguous sub-chunks [c*seglen/S2,
+// (c+1)*seglen/S2), each summed in ascending neuron order, then the S2 partial sums are
+// added in ascending c.  One block per pool row, thread (g, c).
 __global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict__ Y, int64_t ld,
                                                          const int32_t* __restrict__ map,
                                                          const int32_t* __restrict__ count_ptr,
                                                          int32_t pool, int32_t segments,
                                                          int32_t N, float* __restrict__ F) {
+  __shared__ float part[32 * 32];
+  const int s = blockIdx.x;
   const int count = *count_ptr;
   const int P = min(pool, count);
   const int seglen = N / segments;
-  for (int it = threadIdx.x; it < pool * segments; it += blockDim.x) {
-    const int s = it / segments, g = it % segments;
+  const int S2 = min(32, seglen);
+  const int g = threadIdx.x / 32, c = threadIdx.x % 32;
+  if (s < P && g < segments && c < S2) {
+    const int64_t col = map ? (int64_t)map[s] : s;
+    const int n0 = g * seglen + c * seglen / S2, n1 = g * seglen + (c + 1) * seglen / S2;
+    const float* __restrict__ y = Y + col;
     float sum = 0.f;
-    if (s < P) {
-      const int64_t col = map ? (int64_t)map[s] : s;
-      const float* __restrict__ y = Y + (int64_t)g * seglen * ld + col;
-      for (int n = 0; n < seglen; ++n) sum += y[(int64_t)n * ld];
+    int n = n0;
+    for (; n + 16 <= n1; n += 16) {
+      float v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = y[(int64_t)(n + q) * ld];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) sum += v[q];
     }
-    F[it] = sum;
+    for (; n < n1; ++n) sum += y[(int64_t)n * ld];
+    part[g * 32 + c] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x < segments) {
+    float sum = 0.f;
+    if (s < P)
+      for (int q = 0; q < S2; ++q) sum += part[threadIdx.x * 32 + q];
+    F[s * segments + threadIdx.x] = sum;
   }
 }
 
@@ -336,39 +429,53 @@ __global__ __launch_bounds__(256) void gather_centroids_kernel(
 
 // Nearest centroid (src/SDGC/SNICIT/kernel.hpp:8-69): m(i) = argmin_c #{n : Y_i[n] != Y_c[n]},
 // first minimum in centroid order; centroids map to themselves.  flag = non-centroid with a
-// nonzero distance (its delta row is non-empty).  Centroid values are uniform across the
-// wavefront (scalar loads); eight running counts per pass over the row.
-__global__ __launch_bounds__(256) void assign_exact_kernel(
+// nonzero distance (its delta row is non-empty).  The distances are split over blocks of
+// neurons (grid y) and summed with integer atomics into dist[i][32]; centroid values are
+// uniform across the wavefront (scalar loads); eight running counts per pass.
+__global__ __launch_bounds__(256) void assign_dist_kernel(
     const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
     const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap,
-    const int32_t* __restrict__ k_ptr, const int32_t* __restrict__ cent_slot, int32_t N,
+    const int32_t* __restrict__ k_ptr, int32_t N, int32_t nchunk, int32_t* __restrict__ dist) {
+  const int64_t count = *count_ptr;
+  const int k = *k_ptr;
+  if ((int64_t)blockIdx.x * blockDim.x >= count) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t col = map ? (int64_t)map[i] : i;
+  const int n0 = (int)((int64_t)N * blockIdx.y / nchunk);
+  const int n1 = (int)((int64_t)N * (blockIdx.y + 1) / nchunk);
+  for (int cb = 0; cb < k; cb += 8) {
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n = n0; n < n1; ++n) {
+      const float y = Y[(int64_t)n * ld + col];
+      const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cnt[q] += (y != cr[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (cb + q < k && cnt[q]) atomicAdd(&dist[i * kMaxCentroidsExact + cb + q], cnt[q]);
+  }
+}
+
+__global__ __launch_bounds__(256) void assign_final_kernel(
+    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ k_ptr,
+    const int32_t* __restrict__ cent_slot, int32_t* __restrict__ dist,
     int32_t* __restrict__ flags, int32_t* __restrict__ mcar, const int32_t* __restrict__ orig,
     int32_t* __restrict__ rec) {
   const int64_t count = *count_ptr;
   const int k = *k_ptr;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if ((int64_t)blockIdx.x * blockDim.x >= count) return;
-  const bool valid = i < count;
-  const int64_t col = valid ? (map ? (int64_t)map[i] : i) : 0;
+  if (i >= count) return;
   int best = 0x7fffffff, bestc = 0;
-  for (int cb = 0; cb < k; cb += 8) {
-    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (valid) {
-      for (int n = 0; n < N; ++n) {
-        const float y = Y[(int64_t)n * ld + col];
-        const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) cnt[q] += (y != cr[q]);
-      }
+  for (int c = 0; c < k; ++c) {
+    const int d = dist[i * kMaxCentroidsExact + c];
+    dist[i * kMaxCentroidsExact + c] = 0;  // ready for the next run
+    if (d < best) {
+      best = d;
+      bestc = c;
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (cb + q < k && cnt[q] < best) {
-        best = cnt[q];
-        bestc = cb + q;
-      }
   }
-  if (!valid) return;
   bool is_cent = false;
   for (int c = 0; c < k; ++c)
     if (cent_slot[c] == i) {
@@ -390,6 +497,7 @@ __global__ __launch_bounds__(256) void make_delta_kernel(
   const int64_t col = map[j];
   const int c = m[j];
   const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+#pragma unroll 16
   for (int n = n0; n < n1; ++n)
     D[(int64_t)n * ldD + j] = Y[(int64_t)n * ld + col] - C[(int64_t)n * kcap + c];
 }
@@ -428,8 +536,9 @@ __global__ __launch_bounds__(256) void recover_rows_kernel(
   if (j >= count) return;
   const int64_t col = map[j];
   const int c = m[j];
-  const int n0 = blockIdx.y * 256, n1 = min(N, n0 + 256);
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
   bool any = false;
+#pragma unroll 16
   for (int n = n0; n < n1; ++n)
     any |= (D[(int64_t)n * ldD + col] + C[(int64_t)n * kcap + c]) > 0.f;
   if (any) act[j] = 1;
@@ -593,8 +702,9 @@ void launch_cat_from_flags(int32_t* flags, const int32_t* count, int64_t max_col
 
 void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
                      int32_t pool, int32_t segments, int32_t N, float* F, hipStream_t s) {
-  hipLaunchKernelGGL(seg_sums_kernel, dim3(1), dim3(1024), 0, s, Y, ld, map, count, pool,
-                     segments, N, F);
+  SNICIT_REQUIRE(pool <= 32 && segments <= 32, "pool and segments must be <= 32");
+  hipLaunchKernelGGL(seg_sums_kernel, dim3((unsigned)pool), dim3(1024), 0, s, Y, ld, map, count,
+                     pool, segments, N, F);
   check_launch();
 }
 void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int32_t segments,
@@ -612,10 +722,16 @@ void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
 }
 void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
                          int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_slot, int32_t N, int32_t* flags, int32_t* mcar,
-                         const int32_t* orig, int32_t* rec, hipStream_t s) {
-  hipLaunchKernelGGL(assign_exact_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
-                     Y, ld, map, count, C, kcap, k, cent_slot, N, flags, mcar, orig, rec);
+                         const int32_t* cent_slot, int32_t N, int32_t* dist, int32_t* flags,
+                         int32_t* mcar, const int32_t* orig, int32_t* rec, hipStream_t s) {
+  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "assign expects 32 centroid slots");
+  const int nchunk = std::max(1, std::min(64, N / 128));
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)nchunk);
+  hipLaunchKernelGGL(assign_dist_kernel, grid, dim3(256), 0, s, Y, ld, map, count, C, kcap, k, N,
+                     nchunk, dist);
+  check_launch();
+  hipLaunchKernelGGL(assign_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
+                     count, k, cent_slot, dist, flags, mcar, orig, rec);
   check_launch();
 }
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
@@ -642,7 +758,7 @@ void launch_recover_rows(const float* D, int64_t ldD, const int32_t* map, const 
                          const int32_t* orig, const int32_t* count, int64_t max_cols,
                          const float* C, int32_t kcap, int32_t N, int32_t* act, int32_t* cat,
                          hipStream_t s) {
-  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 255) / 256));
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(recover_rows_kernel, grid, dim3(256), 0, s, D, ldD, map, m, count, C, kcap,
                      N, act);
   check_launch();

@@ -74,10 +74,11 @@ void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int
 void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
                              const int32_t* cent_slot, const int32_t* k, int32_t N, float* C,
                              int32_t kcap, hipStream_t s);
+// dist: [max_cols][32] int32, all zero on entry and left zero on exit.
 void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
                          int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_slot, int32_t N, int32_t* flags, int32_t* mcar,
-                         const int32_t* orig, int32_t* rec, hipStream_t s);
+                         const int32_t* cent_slot, int32_t N, int32_t* dist, int32_t* flags,
+                         int32_t* mcar, const int32_t* orig, int32_t* rec, hipStream_t s);
 // D[n*ldD + j] = Y[n*ld + map[j]] - C[n*kcap + m[j]]
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
                        int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
