@@ -86,7 +86,21 @@ def main():
     ap.add_argument("--dense", action="store_true", help="also time the plain-dense form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--pmc-pass", default=None, choices=["BF", "SNICIT"],
+                    help="run exactly one forward of this mode (for counter collection) and "
+                         "write per-layer rows to gpurun_out/pmc_rows_<mode>.json")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="three 1 GiB device-to-device copies (torch), for scaling counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the layer loop "
               "shards by samples with no exchange inside the loop, so N GPUs would run N "
@@ -105,6 +119,21 @@ def main():
     log(f"net N={N} L={L}: {net.panels} panels, {net.device_bytes / 1e9:.2f} GB resident, "
         f"{time.perf_counter() - t0:.1f} s")
     P = net.panels // L
+
+    if args.pmc_pass:
+        plan = sa.Plan(net, sa.default_cfg(args.pmc_pass, threshold=t, use_graph=0), B)
+        plan.set_input_images(IMG_SEED, 0, B)
+        st = plan.run()
+        # rows of each plain-epilogue launch, in launch order (layers < t for SNICIT)
+        counts = plan.trace(sa.Plan.TRACE_LIVE)
+        rows = [B] + [int(c) for c in counts[:-1]]
+        if args.pmc_pass == "SNICIT":
+            rows = rows[:t]
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(os.path.join(ROOT, "gpurun_out", f"pmc_rows_{args.pmc_pass}.json"), "w") as f:
+            json.dump({"N": N, "P": P, "rows": rows, "ms_total": st.ms_total}, f)
+        log(f"pmc pass {args.pmc_pass}: {st.ms_total:.3f} ms, {len(rows)} plain launches")
+        return
 
     snic = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
     snic.set_input_images(IMG_SEED, 0, B)

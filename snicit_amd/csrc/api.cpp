@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -59,17 +60,58 @@ void dev_free(void* p) {
 }
 
 // Upload one layer's host panels into the net's concatenated arrays.
+// Layers whose panels average >= kMfmaMinAvgRows rows get slot-major weights (matrix-core
+// operands); the others keep row-major weights for the vector-ALU kernel.
+int32_t max_rows(const HostLayerPanels& hp) {
+  int32_t m = 0;
+  for (size_t p = 0; p + 1 < hp.pbeg.size(); ++p) m = std::max(m, hp.pbeg[p + 1] - hp.pbeg[p]);
+  return m;
+}
+
+// SNICIT_KERNEL=valu|mfma|auto selects the layer kernel family when a net is built.
+// auto = vector ALU: on the 32-nnz nets it runs at ~80% of device copy bandwidth with 8
+// wavefronts per SIMD, while the matrix-core form needs ~170 VGPRs (2 wavefronts per SIMD)
+// and measured slower; mfma stays available (bit-identical) for nets where it wins.
+int kernel_choice_from_env() {
+  const char* e = std::getenv("SNICIT_KERNEL");
+  if (!e || !*e || std::strcmp(e, "auto") == 0) return 1;
+  if (std::strcmp(e, "valu") == 0) return 1;
+  if (std::strcmp(e, "mfma") == 0) return 2;
+  throw Error(SNICIT_E_INVALID, std::string("SNICIT_KERNEL must be auto, valu or mfma, got ") + e);
+}
+
 struct NetBuilder {
   Net* net;
+  int kernel_choice = kernel_choice_from_env();
   std::vector<int32_t> pcols, pbeg, outs;
   std::vector<float> w;
   bool weights_given = true;
+  int64_t w_floats = 0;
   void add(const HostLayerPanels& hp) {
-    net->panel_off.push_back(net->panel_off.back() + (int64_t)hp.pbeg.size() - 1);
+    const int64_t P = (int64_t)hp.pbeg.size() - 1;
+    net->panel_off.push_back(net->panel_off.back() + P);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
     outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());
-    if (weights_given) w.insert(w.end(), hp.w.begin(), hp.w.end());
+    const bool mfma = kernel_choice != 1 && P > 0 &&
+                      (kernel_choice == 2 || (int64_t)net->N >= (int64_t)kMfmaMinAvgRows * P) &&
+                      max_rows(hp) <= 32;
+    net->mfma.push_back(mfma ? 1 : 0);
+    net->w_off.push_back(w_floats);
+    const int64_t nf = mfma ? P * kSlots * 32 : (int64_t)net->N * kSlots;
+    w_floats += nf;
+    if (!weights_given) return;
+    if (!mfma) {
+      w.insert(w.end(), hp.w.begin(), hp.w.end());
+      return;
+    }
+    const size_t at = w.size();
+    w.resize(at + (size_t)nf, 0.f);
+    for (int64_t p = 0; p < P; ++p)
+      for (int32_t g = 0; g < hp.pbeg[p + 1] - hp.pbeg[p]; ++g)
+        for (int r = 0; r < kSlots; ++r)
+          w[at + (size_t)p * kSlots * 32 + (size_t)r * 32 + g] =
+              hp.w[(size_t)(hp.pbeg[p] + g) * kSlots + r];
   }
 };
 
@@ -79,14 +121,15 @@ void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
   net->d_pcols = dev_alloc<int32_t>((size_t)Ptot * kSlots, &net->device_bytes);
   net->d_pbeg = dev_alloc<int32_t>((size_t)Ptot + net->L, &net->device_bytes);
   net->d_outs = dev_alloc<int32_t>((size_t)net->L * net->N, &net->device_bytes);
-  net->d_w = dev_alloc<float>((size_t)net->L * net->N * kSlots, &net->device_bytes);
+  net->d_w = dev_alloc<float>((size_t)nb.w_floats, &net->device_bytes);
   SNICIT_HIP(hipMemcpy(net->d_pcols, nb.pcols.data(), nb.pcols.size() * 4, hipMemcpyHostToDevice));
   SNICIT_HIP(hipMemcpy(net->d_pbeg, nb.pbeg.data(), nb.pbeg.size() * 4, hipMemcpyHostToDevice));
   SNICIT_HIP(hipMemcpy(net->d_outs, nb.outs.data(), nb.outs.size() * 4, hipMemcpyHostToDevice));
   if (nb.weights_given) {
     SNICIT_HIP(hipMemcpy(net->d_w, nb.w.data(), nb.w.size() * 4, hipMemcpyHostToDevice));
   } else {
-    launch_fill_f32(net->d_w, fill_weight, (int64_t)net->L * net->N * kSlots, s);
+    // generated nets: every panel has 32 rows, so both layouts hold only `fill_weight`
+    launch_fill_f32(net->d_w, fill_weight, nb.w_floats, s);
   }
   if (net->bias_kind == SNICIT_BIAS_PER_NEURON) {
     net->d_bias = dev_alloc<float>((size_t)net->L * net->N, &net->device_bytes);
@@ -317,33 +360,44 @@ void enqueue_post(snicit_plan* p) {
   SnicitState st = snicit_state(p);
   // clustering leaves the delta list's original ids in slot cur^1 and centroids in slot 1
   int orig = cluster_cur(p) ^ 1, mc = 1;
+  const bool lossy = p->cfg.cluster == SNICIT_CLUSTER_LOSSY;
   for (int l = t; l < L; ++l) {
     const int q = (l - t) & 1;
-    SpmmLaunch c = spmm_base(p, l, EPI_CENTROID);
-    c.yin = p->d_C[q];
-    c.ldin = p->kcap;
-    c.yout = p->d_C[q ^ 1];
-    c.ldout = p->kcap;
-    c.count = p->d_k;
-    c.zc = p->d_zc;
-    c.max_cols = p->kcap;
-    prof_mark(p, l, 2);
-    launch_panel_spmm(c, s);
-    prof_mark(p, l, 3);
-    SpmmLaunch d = spmm_base(p, l, p->cfg.cluster == SNICIT_CLUSTER_LOSSY ? EPI_DELTA_LOSSY
-                                                                          : EPI_DELTA);
-    d.yin = q == 0 ? st.Da : st.Db;
-    d.ldin = p->ld;
-    d.yout = q == 0 ? st.Db : st.Da;
-    d.ldout = p->ld;
-    d.map = l == t ? nullptr : p->d_mapD;
-    d.count = p->d_nerow + l;
-    d.flags = p->d_flags;
-    d.zc = p->d_zc;
-    d.mcent = p->d_mcar[mc];
-    prof_mark(p, l, 0);
-    launch_panel_spmm(d, s);
-    prof_mark(p, l, 1);
+    if (post_layer_fusable(*p->net, l)) {
+      prof_mark(p, l, 0);
+      launch_panel_post(*p->net, l, lossy, p->d_C[q], p->d_C[q ^ 1], p->kcap, p->d_k,
+                        q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
+                        p->d_nerow + l, l == t ? nullptr : p->d_mapD, p->d_mcar[mc], p->d_flags,
+                        p->cfg.lossy_eps_post, p->cap, s);
+      prof_mark(p, l, 1);
+      prof_mark(p, l, 2);
+      prof_mark(p, l, 3);
+    } else {
+      SpmmLaunch c = spmm_base(p, l, EPI_CENTROID);
+      c.yin = p->d_C[q];
+      c.ldin = p->kcap;
+      c.yout = p->d_C[q ^ 1];
+      c.ldout = p->kcap;
+      c.count = p->d_k;
+      c.zc = p->d_zc;
+      c.max_cols = p->kcap;
+      prof_mark(p, l, 2);
+      launch_panel_spmm(c, s);
+      prof_mark(p, l, 3);
+      SpmmLaunch d = spmm_base(p, l, lossy ? EPI_DELTA_LOSSY : EPI_DELTA);
+      d.yin = q == 0 ? st.Da : st.Db;
+      d.ldin = p->ld;
+      d.yout = q == 0 ? st.Db : st.Da;
+      d.ldout = p->ld;
+      d.map = l == t ? nullptr : p->d_mapD;
+      d.count = p->d_nerow + l;
+      d.flags = p->d_flags;
+      d.zc = p->d_zc;
+      d.mcent = p->d_mcar[mc];
+      prof_mark(p, l, 0);
+      launch_panel_spmm(d, s);
+      prof_mark(p, l, 1);
+    }
     launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
                    p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1], s);
     orig ^= 1;

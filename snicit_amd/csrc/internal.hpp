@@ -36,8 +36,11 @@ struct Error : std::runtime_error {
 // Panel slots: every panel lists exactly kSlots input neurons (rows with fewer nonzeros
 // are padded with value-0 slots).  32 = the fixed fan-in of the Graph Challenge nets.
 constexpr int kSlots = 32;
-// Maximum output rows per panel; larger sharing groups are split.
-constexpr int kMaxPanelRows = 64;
+// Maximum output rows per panel (one 32-row matrix-core tile); larger sharing groups are
+// split.
+constexpr int kMaxPanelRows = 32;
+// Layers whose panels average at least this many rows run on the matrix cores.
+constexpr int kMfmaMinAvgRows = 16;
 // Centroid capacity of the exact variant (pool <= 32 rows => k <= 32).
 constexpr int kMaxCentroidsExact = 32;
 // Centroid capacity of the lossy variant (pool of 128 strided rows).
@@ -52,6 +55,11 @@ struct HostLayerPanels {
   std::vector<float> w;        // [N][kSlots] weight of each slot, panel order (may be empty
                                //  when the net is generated: all weights equal)
 };
+
+// Weight layouts of a layer on the device:
+//  row-major  w[j][r]           (j = panel-order row), read by the vector-ALU kernel;
+//  slot-major w[p][r][g], g<32  (row g of panel p, zero for g >= rows), the matrix-core
+//             operand layout: one 32-float run per slot.
 
 struct Ctx {
   int device = 0;
@@ -71,7 +79,9 @@ struct Net {
   int32_t* d_pcols = nullptr;      // [Ptot][kSlots]
   int32_t* d_pbeg = nullptr;       // [Ptot + L] (P_l + 1 entries per layer)
   int32_t* d_outs = nullptr;       // [L][N]
-  float* d_w = nullptr;            // [L][N][kSlots]
+  float* d_w = nullptr;            // per layer: row-major [N][kSlots] or slot-major [P][kSlots][32]
+  std::vector<int64_t> w_off;      // [L] first float of each layer in d_w
+  std::vector<uint8_t> mfma;       // [L] layer runs on the matrix cores (slot-major weights)
   float* d_bias = nullptr;         // [L][N] when bias_kind == PER_NEURON
   int64_t device_bytes = 0;
 

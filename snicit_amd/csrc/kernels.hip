@@ -145,6 +145,204 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   }
 }
 
+// ---- fused post-threshold layer -----------------------------------------------------
+// One launch per post layer (src/SDGC/SNICIT/kernel.hpp:71-136, post_spMM + post_minus):
+// every block first computes its panel's centroid rows, zc = W yhat_c + b for the k
+// centroids, into LDS (block row 0 also writes yhat_c' = f(zc)), then produces the delta
+// rows of its column tiles with y = f(zc[m] + W yhat_i) - f(zc[m]) read from LDS, so the
+// residue update never waits on a global load.  Columns <= kNarrowCols use one thread per
+// (row, column) pair in block row 0.  Same fused-multiply-add chain as panel_spmm_kernel.
+template <bool LOSSY>
+__global__ __launch_bounds__(kBlock) void panel_post_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ cin,
+    float* __restrict__ cout, int32_t kcap, const int32_t* __restrict__ k_ptr,
+    const float* __restrict__ din, int64_t ldin, float* __restrict__ dout, int64_t ldout,
+    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post) {
+  __shared__ float zc_s[kMaxPanelRows * (kMaxCentroidsExact + 1)];
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int k = *k_ptr;
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  if (blockIdx.y != 0 && count <= kNarrowCols) return;
+  if ((int64_t)blockIdx.y * kBlock >= count && blockIdx.y != 0) return;
+  // centroid rows of this panel
+  for (int q = threadIdx.x; q < G * k; q += kBlock) {
+    const int jj = q % G, c = q / G, j = beg + jj;
+    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
+    float acc = 0.f;
+#pragma unroll 2
+    for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+      const float4 wv = w4[r4];
+      acc = __builtin_fmaf(wv.x, cin[(int64_t)pc[4 * r4 + 0] * kcap + c], acc);
+      acc = __builtin_fmaf(wv.y, cin[(int64_t)pc[4 * r4 + 1] * kcap + c], acc);
+      acc = __builtin_fmaf(wv.z, cin[(int64_t)pc[4 * r4 + 2] * kcap + c], acc);
+      acc = __builtin_fmaf(wv.w, cin[(int64_t)pc[4 * r4 + 3] * kcap + c], acc);
+    }
+    const int o = outs[j];
+    const float z = acc + (bias_vec ? bias_vec[o] : bias_s);
+    zc_s[jj * (kMaxCentroidsExact + 1) + c] = z;
+    if (blockIdx.y == 0) cout[(int64_t)o * kcap + c] = act_clip(z, clip);
+  }
+  __syncthreads();
+  auto residue = [&](float acc, float z) {
+    float y = act_clip(z + acc, clip) - act_clip(z, clip);
+    if (LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
+    return y;
+  };
+  if (count <= kNarrowCols) {
+    const int64_t pairs = (int64_t)G * count;
+    for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
+      const int jj = (int)(q % G), j = beg + jj;
+      const int64_t pos = q / G;
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+      const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
+      float acc = 0.f;
+#pragma unroll 2
+      for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+        const float4 wv = w4[r4];
+        acc = __builtin_fmaf(wv.x, din[(int64_t)pc[4 * r4 + 0] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.y, din[(int64_t)pc[4 * r4 + 1] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.z, din[(int64_t)pc[4 * r4 + 2] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.w, din[(int64_t)pc[4 * r4 + 3] * ldin + src], acc);
+      }
+      const float y = residue(acc, zc_s[jj * (kMaxCentroidsExact + 1) + mcent[pos]]);
+      dout[(int64_t)outs[j] * ldout + pos] = y;
+      if (y != 0.f) flags[pos] = 1;
+    }
+    return;
+  }
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
+       base += (int64_t)gridDim.y * kBlock) {
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    float x[kSlots];
+    bool nz = false;
+    int32_t mi = 0;
+    if (valid) {
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+      mi = mcent[pos];
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) {
+        x[r] = din[(int64_t)pc[r] * ldin + src];
+        nz |= (x[r] != 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) x[r] = 0.f;
+    }
+    const bool wave_nz = __ballot(nz) != 0ull;
+    bool flag = false;
+    for (int j = beg; j < beg + G; ++j) {
+      const float* __restrict__ wj = w + (size_t)j * kSlots;
+      float acc = 0.f;
+      if (wave_nz) {
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) acc = __builtin_fmaf(wj[r], x[r], acc);
+      }
+      const float y = residue(acc, zc_s[(j - beg) * (kMaxCentroidsExact + 1) + mi]);
+      flag |= (y != 0.f);
+      if (valid) dout[(int64_t)outs[j] * ldout + pos] = y;
+    }
+    if (valid && flag) flags[pos] = 1;
+  }
+}
+
+// ---- panel SpMM on the matrix cores ------------------------------------------------
+// Same contract as panel_spmm_kernel for layers stored slot-major (panels of <= 32 rows).
+// A wavefront owns 64 consecutive columns; each slot r is one v_mfma_f32_32x32x1_2b_f32:
+// A = the panel's 32 row weights of slot r (both 32-lane halves), B = the 64 columns' input
+// of slot r (block 0 = columns 0..31, block 1 = 32..63), D += A x B.  With K = 1 every
+// element is one fused multiply-add, so the 32 steps in slot order are exactly the chain
+// acc = fma(w[31], x[31], ... fma(w[0], x[0], 0)) of the vector kernel and the oracle.
+// D element (block b, row i, column j) sits in lane 32*(i/4 % 2) + j, register
+// 16 b + 4 (i / 8) + i % 4: each store writes two 128-byte runs (rows i, i+4 of 32 columns).
+typedef float floatx32 __attribute__((ext_vector_type(32)));
+
+template <int EPI>
+__global__ __launch_bounds__(kBlock, 2) void panel_mfma_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ wslot,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int64_t first = (int64_t)blockIdx.y * kBlock + wv * kWave;
+  if (first >= count) return;
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  const int half = lane >> 5, j = lane & 31;
+  float wa[kSlots];
+  {
+    const float* __restrict__ wp = wslot + (size_t)p * kSlots * 32 + j;
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) wa[r] = wp[r * 32];
+  }
+  // output neuron of each of this lane's 16 D rows (-1 past the panel's rows)
+  int orow[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = 8 * (q >> 2) + 4 * half + (q & 3);
+    orow[q] = i < G ? outs[beg + i] : -1;
+  }
+  for (int64_t base = first; base < count; base += (int64_t)gridDim.y * kBlock) {
+    // columns past the count read column 0 (in bounds: count > 0 here), zeroed after
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
+    float x[kSlots];
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) x[r] = yin[(int64_t)pc[r] * ldin + src];
+    bool nz = false;
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) {
+      x[r] = valid ? x[r] : 0.f;
+      nz |= x[r] != 0.f;
+    }
+    floatx32 acc;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.f;
+    if (__ballot(nz) != 0ull) {
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r)
+        acc = __builtin_amdgcn_mfma_f32_32x32x1f32(wa[r], x[r], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int64_t pb = base + 32 * b + j;
+      const bool vb = pb < count;
+      int32_t mi = 0;
+      if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = mcent[vb ? pb : 0];
+      bool flag = false;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int o = orow[q];
+        const bool ok = vb && o >= 0;
+        const int oc = o >= 0 ? o : 0;
+        const float bb = bias_vec ? bias_vec[oc] : bias_s;
+        bool f = false;
+        const float y = epilogue<EPI>(acc[16 * b + q], bb, clip, ok, oc, pb, mi, zc, kcap,
+                                      eps_post, f);
+        flag |= ok && f;
+        if (ok) yout[(int64_t)oc * ldout + pb] = y;
+      }
+      // the column's rows are split over lanes j and j + 32
+      const uint64_t m = __ballot(flag);
+      const bool any = ((m >> j) & 1ull) | ((m >> (j + 32)) & 1ull);
+      if (flags && half == 0 && vb && any) flags[pb] = 1;
+    }
+  }
+}
+
 // ---- stable compaction ----------------------------------------------------------------
 // One workgroup of 1024 threads walks the flags in rounds of 16 x 1024 columns: sixteen
 // coalesced loads per thread are in flight together, a ballot per 1024-column sub-round
@@ -610,13 +808,21 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
   const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
   const int32_t* outs = net.d_outs + (size_t)l * net.N;
-  const float* w = net.d_w + (size_t)l * net.N * kSlots;
+  const float* w = net.d_w + net.w_off[l];
   const float* bv = net.bias_vec(l);
   const float bs = net.bias_scalar(l);
-#define SNICIT_SPMM(E)                                                                       \
-  hipLaunchKernelGGL(panel_spmm_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, bv, \
-                     bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags,    \
-                     a.zc, a.kcap, a.mcent, a.eps_post)
+  const bool mfma = net.mfma[l] != 0;
+#define SNICIT_SPMM(E)                                                                         \
+  do {                                                                                         \
+    if (mfma)                                                                                  \
+      hipLaunchKernelGGL(panel_mfma_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, \
+                         bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,     \
+                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post);                          \
+    else                                                                                       \
+      hipLaunchKernelGGL(panel_spmm_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, \
+                         bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,     \
+                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post);                          \
+  } while (0)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_SPMM(EPI_PLAIN); break;
     case EPI_CENTROID: SNICIT_SPMM(EPI_CENTROID); break;
@@ -625,6 +831,36 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
     default: throw Error(SNICIT_E_INVALID, "bad epilogue");
   }
 #undef SNICIT_SPMM
+  check_launch();
+}
+
+bool post_layer_fusable(const Net& net, int l) { return net.mfma[l] == 0; }
+
+void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, float* cout,
+                       int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
+                       int64_t ldout, const int32_t* count, const int32_t* map,
+                       const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
+                       hipStream_t s) {
+  SNICIT_REQUIRE(net.mfma[l] == 0, "fused post layer needs row-major weights");
+  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "fused post layer expects 32 centroid slots");
+  const int64_t P = net.panels(l);
+  const int64_t tiles = (max_cols + kBlock - 1) / kBlock;
+  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(P, 1));
+  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
+  gy = std::min<int64_t>(gy, 65535);
+  dim3 grid((unsigned)P, (unsigned)gy);
+  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
+  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
+  const int32_t* outs = net.d_outs + (size_t)l * net.N;
+  const float* w = net.d_w + net.w_off[l];
+  if (lossy)
+    hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
+                       net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
+                       ldin, dout, ldout, count, map, mcent, flags, eps_post);
+  else
+    hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
+                       net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
+                       ldin, dout, ldout, count, map, mcent, flags, eps_post);
   check_launch();
 }
 
