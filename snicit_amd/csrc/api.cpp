@@ -363,7 +363,7 @@ void enqueue_post(snicit_plan* p) {
   const bool lossy = p->cfg.cluster == SNICIT_CLUSTER_LOSSY;
   for (int l = t; l < L; ++l) {
     const int q = (l - t) & 1;
-    if (post_layer_fusable(*p->net, l)) {
+    if (post_layer_fused(*p->net, l)) {
       prof_mark(p, l, 0);
       launch_panel_post(*p->net, l, lossy, p->d_C[q], p->d_C[q ^ 1], p->kcap, p->d_k,
                         q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,

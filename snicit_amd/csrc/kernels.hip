@@ -145,13 +145,58 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   }
 }
 
-// ---- fused post-threshold layer -----------------------------------------------------
-// One launch per post layer (src/SDGC/SNICIT/kernel.hpp:71-136, post_spMM + post_minus):
-// every block first computes its panel's centroid rows, zc = W yhat_c + b for the k
-// centroids, into LDS (block row 0 also writes yhat_c' = f(zc)), then produces the delta
-// rows of its column tiles with y = f(zc[m] + W yhat_i) - f(zc[m]) read from LDS, so the
-// residue update never waits on a global load.  Columns <= kNarrowCols use one thread per
-// (row, column) pair in block row 0.  Same fused-multiply-add chain as panel_spmm_kernel.
+// ---- post-threshold centroid rows --------------------------------------------------------
+// Centroid rows of one post layer: zc = W yhat_c + b and yhat_c' = f(zc) for the k centroid
+// columns (post_minus, src/SDGC/SNICIT/kernel.hpp:115-121).  One block per panel stages the
+// panel's weights and its 32 inputs of every centroid in LDS, then thread (row, centroid)
+// runs the fixed-order chain.
+__global__ __launch_bounds__(kBlock) void panel_centroid_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ cin,
+    float* __restrict__ cout, float* __restrict__ zc, int32_t kcap,
+    const int32_t* __restrict__ k_ptr) {
+  __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
+  __shared__ float x_s[kSlots * (kMaxCentroidsExact + 1)];
+  const int p = blockIdx.x;
+  const int k = *k_ptr;
+  if (k == 0) return;
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)beg * kSlots);
+  for (int q = threadIdx.x; q < G * kSlots / 4; q += kBlock) w_s[q] = w4[q];
+  for (int q = threadIdx.x; q < kSlots * k; q += kBlock) {
+    const int r = q / k, c = q % k;
+    x_s[r * (kMaxCentroidsExact + 1) + c] = cin[(int64_t)pc[r] * kcap + c];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < G * k; q += kBlock) {
+    const int jj = q / k, c = q % k;
+    float acc = 0.f;
+#pragma unroll
+    for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+      const float4 wv = w_s[jj * (kSlots / 4) + r4];
+      acc = __builtin_fmaf(wv.x, x_s[(4 * r4 + 0) * (kMaxCentroidsExact + 1) + c], acc);
+      acc = __builtin_fmaf(wv.y, x_s[(4 * r4 + 1) * (kMaxCentroidsExact + 1) + c], acc);
+      acc = __builtin_fmaf(wv.z, x_s[(4 * r4 + 2) * (kMaxCentroidsExact + 1) + c], acc);
+      acc = __builtin_fmaf(wv.w, x_s[(4 * r4 + 3) * (kMaxCentroidsExact + 1) + c], acc);
+    }
+    const int o = outs[beg + jj];
+    const float z = acc + (bias_vec ? bias_vec[o] : bias_s);
+    zc[(int64_t)o * kcap + c] = z;
+    cout[(int64_t)o * kcap + c] = act_clip(z, clip);
+  }
+}
+
+// ---- post-threshold layer -------------------------------------------------------------
+// One launch per post layer (src/SDGC/SNICIT/kernel.hpp:71-136, post_spMM + post_minus).
+// Each block stages its panel's weights (4 KB, one float4 per thread) and the panel's 32
+// inputs of every centroid in LDS, computes the centroid rows zc = W yhat_c + b of the panel
+// (block row 0 also writes yhat_c' = f(zc)), then produces the delta rows of its columns,
+// y = f(zc[m] + W yhat_i) - f(zc[m]), with weights and zc read from LDS.  Columns <=
+// kNarrowCols use one thread per (row, column) pair in block row 0; otherwise each
+// wavefront owns 64 columns and every row of the panel.  Same fused-multiply-add chain as
+// panel_spmm_kernel.
 template <bool LOSSY>
 __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
@@ -161,30 +206,41 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const float* __restrict__ din, int64_t ldin, float* __restrict__ dout, int64_t ldout,
     const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
     const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post) {
-  __shared__ float zc_s[kMaxPanelRows * (kMaxCentroidsExact + 1)];
+  constexpr int kC = kMaxCentroidsExact + 1;  // padded centroid stride in LDS
+  __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
+  __shared__ float x_s[kSlots * kC];
+  __shared__ float zc_s[kMaxPanelRows * kC];
+  __shared__ int32_t o_s[kMaxPanelRows];
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
+  if (count <= kNarrowCols ? blockIdx.y != 0 : (int64_t)blockIdx.y * kBlock >= count) return;
   const int k = *k_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
-  if (blockIdx.y != 0 && count <= kNarrowCols) return;
-  if ((int64_t)blockIdx.y * kBlock >= count && blockIdx.y != 0) return;
-  // centroid rows of this panel
-  for (int q = threadIdx.x; q < G * k; q += kBlock) {
-    const int jj = q % G, c = q / G, j = beg + jj;
-    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
-    float acc = 0.f;
-#pragma unroll 2
-    for (int r4 = 0; r4 < kSlots / 4; ++r4) {
-      const float4 wv = w4[r4];
-      acc = __builtin_fmaf(wv.x, cin[(int64_t)pc[4 * r4 + 0] * kcap + c], acc);
-      acc = __builtin_fmaf(wv.y, cin[(int64_t)pc[4 * r4 + 1] * kcap + c], acc);
-      acc = __builtin_fmaf(wv.z, cin[(int64_t)pc[4 * r4 + 2] * kcap + c], acc);
-      acc = __builtin_fmaf(wv.w, cin[(int64_t)pc[4 * r4 + 3] * kcap + c], acc);
+  {
+    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)beg * kSlots);
+    for (int q = threadIdx.x; q < G * kSlots / 4; q += kBlock) w_s[q] = w4[q];
+    if (threadIdx.x < G) o_s[threadIdx.x] = outs[beg + threadIdx.x];
+    for (int q = threadIdx.x; q < kSlots * k; q += kBlock) {
+      const int r = q / k, c = q % k;
+      x_s[r * kC + c] = cin[(int64_t)pc[r] * kcap + c];
     }
-    const int o = outs[j];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < G * k; q += kBlock) {
+    const int jj = q / k, c = q % k;
+    float acc = 0.f;
+#pragma unroll
+    for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+      const float4 wv = w_s[jj * (kSlots / 4) + r4];
+      acc = __builtin_fmaf(wv.x, x_s[(4 * r4 + 0) * kC + c], acc);
+      acc = __builtin_fmaf(wv.y, x_s[(4 * r4 + 1) * kC + c], acc);
+      acc = __builtin_fmaf(wv.z, x_s[(4 * r4 + 2) * kC + c], acc);
+      acc = __builtin_fmaf(wv.w, x_s[(4 * r4 + 3) * kC + c], acc);
+    }
+    const int o = o_s[jj];
     const float z = acc + (bias_vec ? bias_vec[o] : bias_s);
-    zc_s[jj * (kMaxCentroidsExact + 1) + c] = z;
+    zc_s[jj * kC + c] = z;
     if (blockIdx.y == 0) cout[(int64_t)o * kcap + c] = act_clip(z, clip);
   }
   __syncthreads();
@@ -196,21 +252,20 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   if (count <= kNarrowCols) {
     const int64_t pairs = (int64_t)G * count;
     for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
-      const int jj = (int)(q % G), j = beg + jj;
+      const int jj = (int)(q % G);
       const int64_t pos = q / G;
       const int64_t src = map ? (int64_t)map[pos] : pos;
-      const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
       float acc = 0.f;
 #pragma unroll 2
       for (int r4 = 0; r4 < kSlots / 4; ++r4) {
-        const float4 wv = w4[r4];
+        const float4 wv = w_s[jj * (kSlots / 4) + r4];
         acc = __builtin_fmaf(wv.x, din[(int64_t)pc[4 * r4 + 0] * ldin + src], acc);
         acc = __builtin_fmaf(wv.y, din[(int64_t)pc[4 * r4 + 1] * ldin + src], acc);
         acc = __builtin_fmaf(wv.z, din[(int64_t)pc[4 * r4 + 2] * ldin + src], acc);
         acc = __builtin_fmaf(wv.w, din[(int64_t)pc[4 * r4 + 3] * ldin + src], acc);
       }
-      const float y = residue(acc, zc_s[jj * (kMaxCentroidsExact + 1) + mcent[pos]]);
-      dout[(int64_t)outs[j] * ldout + pos] = y;
+      const float y = residue(acc, zc_s[jj * kC + mcent[pos]]);
+      dout[(int64_t)o_s[jj] * ldout + pos] = y;
       if (y != 0.f) flags[pos] = 1;
     }
     return;
@@ -238,16 +293,21 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     }
     const bool wave_nz = __ballot(nz) != 0ull;
     bool flag = false;
-    for (int j = beg; j < beg + G; ++j) {
-      const float* __restrict__ wj = w + (size_t)j * kSlots;
+    for (int jj = 0; jj < G; ++jj) {
       float acc = 0.f;
       if (wave_nz) {
 #pragma unroll
-        for (int r = 0; r < kSlots; ++r) acc = __builtin_fmaf(wj[r], x[r], acc);
+        for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+          const float4 wv4 = w_s[jj * (kSlots / 4) + r4];
+          acc = __builtin_fmaf(wv4.x, x[4 * r4 + 0], acc);
+          acc = __builtin_fmaf(wv4.y, x[4 * r4 + 1], acc);
+          acc = __builtin_fmaf(wv4.z, x[4 * r4 + 2], acc);
+          acc = __builtin_fmaf(wv4.w, x[4 * r4 + 3], acc);
+        }
       }
-      const float y = residue(acc, zc_s[(j - beg) * (kMaxCentroidsExact + 1) + mi]);
+      const float y = residue(acc, zc_s[jj * kC + mi]);
       flag |= (y != 0.f);
-      if (valid) dout[(int64_t)outs[j] * ldout + pos] = y;
+      if (valid) dout[(int64_t)o_s[jj] * ldout + pos] = y;
     }
     if (valid && flag) flags[pos] = 1;
   }
@@ -834,17 +894,31 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   check_launch();
 }
 
-bool post_layer_fusable(const Net& net, int l) { return net.mfma[l] == 0; }
+bool post_layer_fused(const Net& net, int l) { return net.mfma[l] == 0; }
+
+void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
+                           int32_t kcap, const int32_t* k, hipStream_t s) {
+  SNICIT_REQUIRE(net.mfma[l] == 0, "centroid kernel needs row-major weights");
+  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "centroid kernel expects 32 centroid slots");
+  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
+  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
+  const int32_t* outs = net.d_outs + (size_t)l * net.N;
+  const float* w = net.d_w + net.w_off[l];
+  hipLaunchKernelGGL(panel_centroid_kernel, dim3((unsigned)net.panels(l)), dim3(kBlock), 0, s,
+                     pcols, pbeg, outs, w, net.bias_vec(l), net.bias_scalar(l), net.clip, cin,
+                     cout, zc, kcap, k);
+  check_launch();
+}
 
 void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, float* cout,
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
                        hipStream_t s) {
-  SNICIT_REQUIRE(net.mfma[l] == 0, "fused post layer needs row-major weights");
-  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "fused post layer expects 32 centroid slots");
+  SNICIT_REQUIRE(net.mfma[l] == 0, "post kernel needs row-major weights");
+  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
-  const int64_t tiles = (max_cols + kBlock - 1) / kBlock;
+  const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
   int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(P, 1));
   gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
   gy = std::min<int64_t>(gy, 65535);

@@ -33,15 +33,18 @@ struct SpmmLaunch {
 };
 void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
 
-// Fused post-threshold layer: centroid rows (cin -> cout, k columns of ld kcap) and delta
-// rows (din -> dout through map, centroid index mcent) in one launch.  Needs row-major
-// weights (post_layer_fusable).
-bool post_layer_fusable(const Net& net, int l);
+// One post-threshold layer in one launch: centroid rows (cin -> cout, k columns of ld kcap)
+// and delta rows (din -> dout through map, centroid index mcent).  Needs row-major weights
+// (post_layer_fused); otherwise the centroid and delta epilogues of launch_panel_spmm run.
+bool post_layer_fused(const Net& net, int l);
 void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, float* cout,
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
                        hipStream_t s);
+// Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
+void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
+                           int32_t kcap, const int32_t* k, hipStream_t s);
 
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
