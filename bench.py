@@ -40,6 +40,21 @@ def spmm_layer_bytes(N, P, rows):
     return 8.0 * N * rows + weights
 
 
+def measured_traffic(wl):
+    """HBM bytes per pre-threshold launch from the committed counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by a known-size copy), or None."""
+    path = os.path.join(ROOT, "profiles", "panel_spmm_c2_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    w = rec.get("workload", {})
+    if (w.get("N"), w.get("L"), w.get("batch"), w.get("threshold")) != (
+            wl["N"], wl["L"], wl["B"], wl["t"]):
+        return None
+    return rec.get("pre_threshold_traffic_bytes_per_launch")
+
+
 def time_steps(plan, steps, warmup):
     for _ in range(warmup):
         plan.run()
@@ -127,11 +142,16 @@ def main():
         # rows of each plain-epilogue launch, in launch order (layers < t for SNICIT)
         counts = plan.trace(sa.Plan.TRACE_LIVE)
         rows = [B] + [int(c) for c in counts[:-1]]
+        extra = {}
         if args.pmc_pass == "SNICIT":
             rows = rows[:t]
+            ne = plan.trace(sa.Plan.TRACE_NEROW)
+            # delta rows entering post layer l: nerow after clustering, then after layer l-1
+            extra = {"post_rows": [int(st.nerow_first)] + [int(v) for v in ne[t:L - 1]],
+                     "k": int(st.centroids)}
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(os.path.join(ROOT, "gpurun_out", f"pmc_rows_{args.pmc_pass}.json"), "w") as f:
-            json.dump({"N": N, "P": P, "rows": rows, "ms_total": st.ms_total}, f)
+            json.dump({"N": N, "P": P, "rows": rows, "ms_total": st.ms_total, **extra}, f)
         log(f"pmc pass {args.pmc_pass}: {st.ms_total:.3f} ms, {len(rows)} plain launches")
         return
 
@@ -170,7 +190,7 @@ def main():
             "achieved": round(b_pre / t_pre / 1e9, 1) if t_pre > 0 else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b_pre / t_pre / 1e9 / HBM_PEAK_GBS, 4) if t_pre > 0 else None,
-            "traffic": None,
+            "traffic": measured_traffic(wl),
             "bytes_per_launch": b_pre / n_pre, "avg_launch_ms": 1000.0 * t_pre / n_pre}
     post = list(range(t, L))
     k = st_s.centroids
