@@ -73,7 +73,10 @@ typedef struct snicit_run_cfg {
   float lossy_eps_assign;  /* lossy: L0 tolerance and delta pruning at clustering (0.04)*/
   float lossy_eps_post;    /* lossy: delta pruning after each post layer (0.05)       */
   int32_t use_graph;       /* 1: replay the layer loop as captured hipGraphs          */
-  int32_t reserved[7];
+  int32_t lossy_positive_only; /* lossy: a delta row is non-empty iff some entry > 0 at
+                              * clustering (the reference's test, beyond/SNICIT/kernel.hpp:95);
+                              * 0: iff some entry != 0                                    */
+  int32_t reserved[6];
 } snicit_run_cfg;
 
 /* Per-run statistics.  Times are HIP-event milliseconds on the plan's stream with
@@ -121,6 +124,13 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K,
  * counterpart (the reference reads the Graph Challenge TSVs, src/SDGC/utility.hpp:65-157). */
 int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t seed,
                              float weight, float bias, float clip, snicit_net** out);
+/* Dense input layer (Nin -> N: y = min(max(W_in x + b_in, 0), in_clip), w_in[N][Nin]) and
+ * dense output layer (N -> Nout: logits = W_out y + b_out, w_out[Nout][N]) around the hidden
+ * layers, with category = argmax of the logits (first maximum).  Replaces the beyond nets'
+ * dense_input / dense_output / check_acc (src/beyond/general.hpp:5-33,74-128).  Either size
+ * may be 0.  Call before creating plans on the net. */
+int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const float* b_in,
+                      float in_clip, int32_t Nout, const float* w_out, const float* b_out);
 int snicit_net_destroy(snicit_net* net);
 int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,
                     int64_t* panels_total, int64_t* device_bytes);
@@ -153,7 +163,8 @@ int snicit_plan_activations(snicit_plan* plan, float* y);
  * 1 non-empty delta rows after each post layer (int64[L], 0 before t), 2 pool rows
  * as original sample ids (int32[pool]), 3 pool segment sums (float[pool*segments]),
  * 4 centroids as original sample ids (int32[k]), 5 per-sample assignment (int32[B]:
- * centroid index, or -1 for rows dead before t).  Returns the element count. */
+ * centroid index, or -1 for rows dead before t), 6 output-layer logits (float[B*Nout]).
+ * Returns the element count. */
 int64_t snicit_plan_trace(snicit_plan* plan, int32_t what, void* out, int64_t cap);
 
 /* Per-launch timing: replays the plan's layer loop eagerly with a hipEvent pair around
@@ -174,6 +185,13 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
 int snicit_gen_sdgc_ell(int32_t N, int32_t L, uint64_t seed, int32_t* cols);
 /* B x N binary blob images for samples first_sample.. (row-major pixels). */
 int snicit_gen_images(int32_t N, uint64_t seed, int64_t first_sample, int64_t B, float* out);
+/* Unstructured MLP: L hidden layers of N rows with K distinct random inputs each, weights
+ * N(0, 2/K) (cols/vals [L][N][K], ascending per row); optional dense w_in [N][Nin] with
+ * N(0, 2/Nin) and w_out [Nout][N] with N(0, 2/N). */
+int snicit_gen_unstructured(int32_t Nin, int32_t N, int32_t L, int32_t K, int32_t Nout,
+                            uint64_t seed, int32_t* cols, float* vals, float* w_in, float* w_out);
+/* B x Nin features uniform in [0, 1) for samples first_sample.. */
+int snicit_gen_features(int32_t Nin, uint64_t seed, int64_t first_sample, int64_t B, float* out);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,15 @@ def lib():
             _I32P, _F32P, ctypes.c_int, _F32P, ctypes.c_float, ctypes.c_int64, _F32P,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _I32P, _F32P, _I32P, _F32P,
             _I32P, _I32P, _I64P, _I64P, _I64P, ctypes.c_int]
+        model = [ctypes.c_int, _F32P, _F32P, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                 ctypes.c_int, _I32P, _F32P, ctypes.c_int, _F32P, ctypes.c_float, ctypes.c_int,
+                 _F32P, _F32P, ctypes.c_int64, _F32P]
+        L.oracle_model_plain.restype = ctypes.c_int
+        L.oracle_model_plain.argtypes = model + [_F32P, _F32P, _I32P, ctypes.c_int]
+        L.oracle_model_snicit_lossy.restype = ctypes.c_int
+        L.oracle_model_snicit_lossy.argtypes = model + [
+            ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+            ctypes.c_int, _I32P, _F32P, _F32P, _I32P, _I32P, _I32P, _I64P, _I64P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -139,3 +148,65 @@ def snicit_exact(cols, vals, bias, clip, y0, t, sample_size=32, segments=16, eps
     return {"cat": cat, "yL": yL, "pool": pool[:P], "F": F[:P * segments].reshape(P, segments),
             "centroids": cent[:k], "assign": assign, "live": live[:t], "nerow": nerow[t:],
             "live_at_t": Bp, "k": k, "nerow_at_t": ne0}
+
+
+def _model_args(cols, vals, bias, clip, w_in, b_in, in_clip, w_out, b_out):
+    cols = np.ascontiguousarray(cols, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    L, N, K = cols.shape
+    kind, b = _bias(bias, L, N)
+    Nin = 0 if w_in is None else w_in.shape[1]
+    Nout = 0 if w_out is None else w_out.shape[0]
+    wi = np.ascontiguousarray(w_in if Nin else np.zeros((1, 1)), np.float32)
+    bi = np.ascontiguousarray(b_in if Nin else np.zeros(1), np.float32)
+    wo = np.ascontiguousarray(w_out if Nout else np.zeros((1, 1)), np.float32)
+    bo = np.ascontiguousarray(b_out if Nout else np.zeros(1), np.float32)
+    keep = (cols, vals, b, wi, bi, wo, bo)
+    args = [Nin, _p(wi, _F32P), _p(bi, _F32P), in_clip, N, L, K, _p(cols, _I32P),
+            _p(vals, _F32P), kind, _p(b, _F32P), clip, Nout, _p(wo, _F32P), _p(bo, _F32P)]
+    return args, keep, N, Nout
+
+
+def model_plain(cols, vals, bias, clip, x0, w_in=None, b_in=None, in_clip=1.0, w_out=None,
+                b_out=None, nthreads=0):
+    """Plain forward of a model with optional dense input/output layers.
+    Returns (yL (B,N), logits (B,Nout) or None, cat)."""
+    args, keep, N, Nout = _model_args(cols, vals, bias, clip, w_in, b_in, in_clip, w_out, b_out)
+    x0 = np.ascontiguousarray(x0, np.float32)
+    B = x0.shape[0]
+    yL = np.empty((B, N), np.float32)
+    lg = np.empty((B, max(Nout, 1)), np.float32)
+    cat = np.empty(B, np.int32)
+    rc = lib().oracle_model_plain(*args, B, _p(x0, _F32P), _p(yL, _F32P), _p(lg, _F32P),
+                                  _p(cat, _I32P), nthreads)
+    assert rc == 0, rc
+    return yL, (lg if Nout else None), cat
+
+
+def model_snicit_lossy(cols, vals, bias, clip, x0, t, w_in=None, b_in=None, in_clip=1.0,
+                       w_out=None, b_out=None, pool=128, eps_pool=0.03, eps_assign=0.04,
+                       eps_post=0.05, positive_only=False, nthreads=0):
+    """Lossy SNICIT restatement.  Returns a dict with cat, yL, logits, pool, centroids,
+    assign, nerow (per post layer), k, nerow_at_t."""
+    args, keep, N, Nout = _model_args(cols, vals, bias, clip, w_in, b_in, in_clip, w_out, b_out)
+    x0 = np.ascontiguousarray(x0, np.float32)
+    B = x0.shape[0]
+    L = keep[0].shape[0]
+    cat = np.empty(B, np.int32)
+    yL = np.empty((B, N), np.float32)
+    lg = np.empty((B, max(Nout, 1)), np.float32)
+    pc = np.full(pool, -1, np.int32)
+    cc = np.full(pool, -1, np.int32)
+    assign = np.empty(B, np.int32)
+    nerow = np.zeros(L, np.int64)
+    info = np.zeros(4, np.int64)
+    rc = lib().oracle_model_snicit_lossy(*args, B, _p(x0, _F32P), t, pool, eps_pool, eps_assign,
+                                         eps_post, int(positive_only), _p(cat, _I32P),
+                                         _p(yL, _F32P), _p(lg, _F32P), _p(pc, _I32P),
+                                         _p(cc, _I32P), _p(assign, _I32P), _p(nerow, _I64P),
+                                         _p(info, _I64P), nthreads)
+    assert rc == 0, rc
+    _, P, k, ne0 = (int(x) for x in info)
+    return {"cat": cat, "yL": yL, "logits": lg if Nout else None, "pool": pc[:P],
+            "centroids": cc[:k], "assign": assign, "nerow": nerow[t:], "k": k,
+            "nerow_at_t": ne0}

@@ -461,3 +461,271 @@ int oracle_snicit_exact(int N, int L, int K, const int32_t* cols, const float* v
   free_net(&s);
   return 0;
 }
+
+/* ---- beyond-style models: dense input layer, hidden layers, dense output + argmax -------
+ * Dense input (src/beyond/general.hpp:5-33): h = min(max(W_in x + b_in, 0), in_clip), rows of
+ * W_in are [N][Nin], chain over inputs 0..Nin-1.  Dense output (general.hpp:74-102): logits =
+ * W_out y + b_out ([Nout][N]), no activation.  Category = first argmax of the logits starting
+ * from -10000 (check_acc, general.hpp:112-119).  Nin == 0 / Nout == 0 drop the layer (the
+ * category is then any(y > 0)). */
+static void dense_forward(int rows, int fan_in, const float* w, const float* b, int clip_mode,
+                          float clip, const float* in, float* out) {
+  for (int o = 0; o < rows; ++o) {
+    float acc = 0.f;
+    for (int i = 0; i < fan_in; ++i) acc = fmaf(w[(size_t)o * fan_in + i], in[i], acc);
+    const float z = acc + b[o];
+    out[o] = clip_mode ? act(z, clip) : z;
+  }
+}
+
+static int argmax_logits(const float* lg, int n) {
+  float best = -10000.f;
+  int arg = 0;
+  for (int o = 0; o < n; ++o)
+    if (lg[o] > best) {
+      best = lg[o];
+      arg = o;
+    }
+  return arg;
+}
+
+/* Plain model forward.  x0 is B x Nin (or B x N without an input layer); yL gets the last
+ * hidden activations (B x N), logits B x Nout; all outputs nullable except cat. */
+int oracle_model_plain(int Nin, const float* w_in, const float* b_in, float in_clip, int N, int L,
+                       int K, const int32_t* cols, const float* vals, int bias_kind,
+                       const float* bias, float clip, int Nout, const float* w_out,
+                       const float* b_out, int64_t B, const float* x0, float* yL, float* logits,
+                       int32_t* cat, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  sorted_net s;
+  if (sort_net(N, L, K, cols, vals, &s)) return -1;
+  const int width = Nin > 0 ? Nin : N;
+  int rc = 0;
+#pragma omp parallel
+  {
+    float* a = (float*)malloc((size_t)N * sizeof(float));
+    float* b = (float*)malloc((size_t)N * sizeof(float));
+    float* lg = (float*)malloc((size_t)(Nout > 0 ? Nout : 1) * sizeof(float));
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < B; ++i) {
+      const float* x = x0 + (size_t)i * width;
+      if (Nin > 0)
+        dense_forward(N, Nin, w_in, b_in, 1, in_clip, x, a);
+      else
+        memcpy(a, x, (size_t)N * sizeof(float));
+      for (int l = 0; l < L; ++l) {
+        for (int o = 0; o < N; ++o)
+          b[o] = act(row_dot(&s, l, o, a) + bias_of(bias_kind, bias, N, l, o), clip);
+        float* t = a;
+        a = b;
+        b = t;
+      }
+      if (yL) memcpy(yL + (size_t)i * N, a, (size_t)N * sizeof(float));
+      if (Nout > 0) {
+        dense_forward(Nout, N, w_out, b_out, 0, 0.f, a, lg);
+        if (logits) memcpy(logits + (size_t)i * Nout, lg, (size_t)Nout * sizeof(float));
+        cat[i] = argmax_logits(lg, Nout);
+      } else {
+        int any = 0;
+        for (int n = 0; n < N && !any; ++n) any = a[n] > 0.f;
+        cat[i] = any;
+      }
+    }
+    free(a);
+    free(b);
+    free(lg);
+  }
+  free_net(&s);
+  return rc;
+}
+
+/* ---- lossy SNICIT (src/beyond/SNICIT/SNICIT.hpp:494-641, kernel.hpp:5-183) --------------
+ * Every row is live.  At t, on the hidden activations Y_t:
+ *   pool  = rows y*B/pool for y < pool (y_star_gen, kernel.hpp:12)
+ *   prune in pool order: a present base i removes each other present row y with
+ *         #{n : |P_y[n] - P_i[n]| > eps_pool} < N*eps_pool (kernel.hpp:19-41)
+ *   m(i)  = first argmin_c #{n : |C_c[n] - Y_i[n]| > eps_assign}; centroids map to
+ *           themselves (coarse_cluster, kernel.hpp:56-87)
+ *   delta = |Y_i - C_m| > eps_assign ? Y_i - C_m : 0; non-empty iff any(delta != 0)
+ *           (any(delta > 0) with positive_only, the reference's test, kernel.hpp:95)
+ * Post layers (sparse_hidden_post + update_post, kernel.hpp:103-168):
+ *   zc = W C + b, C' = f(zc);  y = f(zc[m] + W delta) - f(zc[m]), |y| <= eps_post -> 0,
+ *   non-empty iff any(y != 0).
+ * Recovery (kernel.hpp:170-183): Y_i = delta_i + C_m (C_m for centroids and empty rows), then
+ * the output layer and argmax.  Outputs (nullable except cat): yL (recovered, B x N), logits,
+ * pool_cols[pool], cent_cols[pool], assign[B], nerow_per_layer[L] (after each post layer),
+ * info[4] = {B, pool, k, non-empty rows at t}. */
+int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, float in_clip, int N,
+                              int L, int K, const int32_t* cols, const float* vals, int bias_kind,
+                              const float* bias, float clip, int Nout, const float* w_out,
+                              const float* b_out, int64_t B, const float* x0, int t, int pool,
+                              float eps_pool, float eps_assign, float eps_post, int positive_only,
+                              int32_t* cat, float* yL, float* logits, int32_t* pool_cols,
+                              int32_t* cent_cols, int32_t* assign, int64_t* nerow_per_layer,
+                              int64_t* info, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  if (t < 0 || t >= L || pool < 1) return -2;
+  sorted_net s;
+  if (sort_net(N, L, K, cols, vals, &s)) return -1;
+  const int width = Nin > 0 ? Nin : N;
+  float* Y = (float*)malloc((size_t)(B > 0 ? B : 1) * N * sizeof(float));
+  /* input layer and pre-threshold layers, per sample */
+#pragma omp parallel
+  {
+    float* b = (float*)malloc((size_t)N * sizeof(float));
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t i = 0; i < B; ++i) {
+      float* a = Y + (size_t)i * N;
+      const float* x = x0 + (size_t)i * width;
+      if (Nin > 0)
+        dense_forward(N, Nin, w_in, b_in, 1, in_clip, x, a);
+      else
+        memcpy(a, x, (size_t)N * sizeof(float));
+      for (int l = 0; l < t; ++l) {
+        for (int o = 0; o < N; ++o)
+          b[o] = act(row_dot(&s, l, o, a) + bias_of(bias_kind, bias, N, l, o), clip);
+        memcpy(a, b, (size_t)N * sizeof(float));
+      }
+    }
+    free(b);
+  }
+  const int P = B > 0 ? pool : 0;
+  int32_t* pcol = (int32_t*)malloc((size_t)pool * sizeof(int32_t));
+  int* alive = (int*)malloc((size_t)pool * sizeof(int));
+  for (int y = 0; y < P; ++y) {
+    pcol[y] = (int32_t)((int64_t)y * B / pool);
+    alive[y] = 1;
+  }
+  const double limit = (double)N * (double)eps_pool;
+  for (int i = 0; i < P; ++i) {
+    if (!alive[i]) continue;
+    const float* pi = Y + (size_t)pcol[i] * N;
+    for (int y = 0; y < P; ++y) {
+      if (y == i || !alive[y]) continue;
+      const float* py = Y + (size_t)pcol[y] * N;
+      int cnt = 0;
+      for (int n = 0; n < N; ++n) cnt += fabsf(py[n] - pi[n]) > eps_pool;
+      if ((double)cnt < limit) alive[y] = 0;
+    }
+  }
+  int k = 0;
+  int32_t* cent = (int32_t*)malloc((size_t)(pool > 0 ? pool : 1) * sizeof(int32_t));
+  for (int y = 0; y < P; ++y)
+    if (alive[y]) cent[k++] = pcol[y];
+  float* C = (float*)malloc((size_t)(k > 0 ? k : 1) * N * sizeof(float));
+  for (int c = 0; c < k; ++c) memcpy(C + (size_t)c * N, Y + (size_t)cent[c] * N, (size_t)N * sizeof(float));
+  int32_t* m = (int32_t*)malloc((size_t)(B > 0 ? B : 1) * sizeof(int32_t));
+  unsigned char* ne = (unsigned char*)calloc((size_t)(B > 0 ? B : 1), 1);
+  int64_t nerow = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : nerow)
+  for (int64_t i = 0; i < B; ++i) {
+    float* y = Y + (size_t)i * N;
+    int best = 0x7fffffff, bestc = 0;
+    for (int c = 0; c < k; ++c) {
+      const float* yc = C + (size_t)c * N;
+      int d = 0;
+      for (int n = 0; n < N; ++n) d += fabsf(yc[n] - y[n]) > eps_assign;
+      if (d < best) {
+        best = d;
+        bestc = c;
+      }
+    }
+    int is_cent = 0;
+    for (int c = 0; c < k && !is_cent; ++c)
+      if (cent[c] == i) {
+        is_cent = 1;
+        bestc = c;
+      }
+    m[i] = bestc;
+    if (is_cent || k == 0) continue;
+    const float* yc = C + (size_t)bestc * N;
+    int any = 0;
+    for (int n = 0; n < N; ++n) {
+      const float d = y[n] - yc[n];
+      const float v = fabsf(d) > eps_assign ? d : 0.f;
+      y[n] = v;
+      any |= positive_only ? (v > 0.f) : (v != 0.f);
+    }
+    ne[i] = (unsigned char)any;
+    nerow += any;
+  }
+  if (info) {
+    info[0] = B;
+    info[1] = P;
+    info[2] = k;
+    info[3] = nerow;
+  }
+  if (pool_cols)
+    for (int y = 0; y < P; ++y) pool_cols[y] = pcol[y];
+  if (cent_cols)
+    for (int c = 0; c < k; ++c) cent_cols[c] = cent[c];
+  if (assign)
+    for (int64_t i = 0; i < B; ++i) assign[i] = k > 0 ? m[i] : -1;
+  float* Zc = (float*)malloc((size_t)(k > 0 ? k : 1) * N * sizeof(float));
+  float* Cn = (float*)malloc((size_t)(k > 0 ? k : 1) * N * sizeof(float));
+  for (int l = t; l < L; ++l) {
+    for (int c = 0; c < k; ++c)
+      for (int o = 0; o < N; ++o) {
+        const float z = row_dot(&s, l, o, C + (size_t)c * N) + bias_of(bias_kind, bias, N, l, o);
+        Zc[(size_t)c * N + o] = z;
+        Cn[(size_t)c * N + o] = act(z, clip);
+      }
+    int64_t cnt = 0;
+#pragma omp parallel reduction(+ : cnt)
+    {
+      float* outrow = (float*)malloc((size_t)N * sizeof(float));
+#pragma omp for schedule(dynamic, 4)
+      for (int64_t i = 0; i < B; ++i) {
+        if (!ne[i]) continue;
+        float* y = Y + (size_t)i * N;
+        const float* zc = Zc + (size_t)m[i] * N;
+        int any = 0;
+        for (int o = 0; o < N; ++o) {
+          const float z = row_dot(&s, l, o, y);
+          float v = act(zc[o] + z, clip) - act(zc[o], clip);
+          v = fabsf(v) > eps_post ? v : 0.f;
+          outrow[o] = v;
+          any |= v != 0.f;
+        }
+        memcpy(y, outrow, (size_t)N * sizeof(float));
+        ne[i] = (unsigned char)any;
+        cnt += any;
+      }
+      free(outrow);
+    }
+    if (nerow_per_layer) nerow_per_layer[l] = cnt;
+    float* sw = C;
+    C = Cn;
+    Cn = sw;
+  }
+  float* lg = (float*)malloc((size_t)(Nout > 0 ? Nout : 1) * sizeof(float));
+  float* rec = (float*)malloc((size_t)N * sizeof(float));
+  for (int64_t i = 0; i < B; ++i) {
+    const float* yc = C + (size_t)(k > 0 ? m[i] : 0) * N;
+    const float* y = Y + (size_t)i * N;
+    for (int n = 0; n < N; ++n) rec[n] = k == 0 ? 0.f : (ne[i] ? y[n] + yc[n] : yc[n]);
+    if (yL) memcpy(yL + (size_t)i * N, rec, (size_t)N * sizeof(float));
+    if (Nout > 0) {
+      dense_forward(Nout, N, w_out, b_out, 0, 0.f, rec, lg);
+      if (logits) memcpy(logits + (size_t)i * Nout, lg, (size_t)Nout * sizeof(float));
+      cat[i] = argmax_logits(lg, Nout);
+    } else {
+      int any = 0;
+      for (int n = 0; n < N && !any; ++n) any = rec[n] > 0.f;
+      cat[i] = any;
+    }
+  }
+  free(lg);
+  free(rec);
+  free(Zc);
+  free(Cn);
+  free(C);
+  free(m);
+  free(ne);
+  free(cent);
+  free(alive);
+  free(pcol);
+  free(Y);
+  free_net(&s);
+  return 0;
+}

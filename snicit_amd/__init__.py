@@ -43,7 +43,8 @@ class RunCfg(ctypes.Structure):
         ("sample_size", ctypes.c_int32), ("segments", ctypes.c_int32), ("eps", ctypes.c_float),
         ("lossy_pool", ctypes.c_int32), ("lossy_eps_pool", ctypes.c_float),
         ("lossy_eps_assign", ctypes.c_float), ("lossy_eps_post", ctypes.c_float),
-        ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7),
+        ("use_graph", ctypes.c_int32), ("lossy_positive_only", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 
@@ -77,6 +78,8 @@ SIGNATURES = {
     "snicit_net_generate_sdgc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
                                                 ctypes.c_float, ctypes.POINTER(_P)]),
+    "snicit_net_set_io": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, _F32P, ctypes.c_float,
+                                         ctypes.c_int32, _F32P, _F32P]),
     "snicit_net_destroy": (ctypes.c_int, [_P]),
     "snicit_net_info": (ctypes.c_int, [_P, _I32P, _I32P, _I32P,
                                        ctypes.POINTER(ctypes.c_int64),
@@ -100,6 +103,11 @@ SIGNATURES = {
     "snicit_gen_sdgc_ell": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, _I32P]),
     "snicit_gen_images": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
                                          ctypes.c_int64, _F32P]),
+    "snicit_gen_unstructured": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                               _I32P, _F32P, _F32P, _F32P]),
+    "snicit_gen_features": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                           ctypes.c_int64, _F32P]),
 }
 
 _lib = None
@@ -159,6 +167,24 @@ def gen_images(N, seed, first_sample, B):
     return out
 
 
+def gen_unstructured(Nin, N, L, K, Nout, seed):
+    """Seeded unstructured MLP: (cols (L,N,K), vals (L,N,K), w_in (N,Nin) or None,
+    w_out (Nout,N) or None)."""
+    cols = np.empty((L, N, K), np.int32)
+    vals = np.empty((L, N, K), np.float32)
+    w_in = np.empty((N, max(Nin, 1)), np.float32)
+    w_out = np.empty((max(Nout, 1), N), np.float32)
+    _check(lib().snicit_gen_unstructured(Nin, N, L, K, Nout, seed, _i32(cols), _f32(vals),
+                                         _f32(w_in), _f32(w_out)))
+    return cols, vals, (w_in if Nin else None), (w_out if Nout else None)
+
+
+def gen_features(Nin, seed, first_sample, B):
+    out = np.empty((B, Nin), np.float32)
+    _check(lib().snicit_gen_features(Nin, seed, first_sample, B, _f32(out)))
+    return out
+
+
 # ---- device objects --------------------------------------------------------------------
 class Context:
     """One device and one HIP stream (replaces GpuEnv, src/SDGC/gpu_lib/gpu_env.h:11-89)."""
@@ -194,6 +220,7 @@ class Net:
                                      ctypes.byref(p), ctypes.byref(b)))
         self.N, self.L, self.K = n.value, l.value, k.value
         self.panels, self.device_bytes = p.value, b.value
+        self.Nin, self.Nout = 0, 0
 
     @classmethod
     def from_ell(cls, ctx, cols, vals, bias, clip=32.0):
@@ -224,6 +251,21 @@ class Net:
                                               ctypes.byref(h)))
         return cls(ctx, h)
 
+    def set_io(self, w_in=None, b_in=None, in_clip=1.0, w_out=None, b_out=None):
+        """Dense input layer (N x Nin, clip [0, in_clip]) and dense output layer (Nout x N,
+        linear; category = argmax).  Call before creating plans."""
+        Nin = 0 if w_in is None else w_in.shape[1]
+        Nout = 0 if w_out is None else w_out.shape[0]
+        wi = np.ascontiguousarray(w_in if w_in is not None else np.zeros((1, 1)), np.float32)
+        bi = np.ascontiguousarray(b_in if b_in is not None else np.zeros(max(self.N, 1)),
+                                  np.float32)
+        wo = np.ascontiguousarray(w_out if w_out is not None else np.zeros((1, 1)), np.float32)
+        bo = np.ascontiguousarray(b_out if b_out is not None else np.zeros(max(Nout, 1)),
+                                  np.float32)
+        _check(lib().snicit_net_set_io(self.h, Nin, _f32(wi), _f32(bi), in_clip, Nout, _f32(wo),
+                                       _f32(bo)))
+        self.Nin, self.Nout = Nin, Nout
+
     def close(self):
         if self.h:
             lib().snicit_net_destroy(self.h)
@@ -239,7 +281,8 @@ class Net:
 class Plan:
     """Activations of up to ``max_batch`` samples for one mode on one net."""
 
-    TRACE_LIVE, TRACE_NEROW, TRACE_POOL, TRACE_F, TRACE_CENTROIDS, TRACE_ASSIGN = range(6)
+    (TRACE_LIVE, TRACE_NEROW, TRACE_POOL, TRACE_F, TRACE_CENTROIDS, TRACE_ASSIGN,
+     TRACE_LOGITS) = range(7)
 
     def __init__(self, net, cfg, max_batch):
         self.net = net
@@ -251,7 +294,7 @@ class Plan:
 
     def set_input(self, y0):
         y0 = np.ascontiguousarray(y0, dtype=np.float32)
-        assert y0.ndim == 2 and y0.shape[1] == self.net.N
+        assert y0.ndim == 2 and y0.shape[1] == (self.net.Nin or self.net.N)
         _check(lib().snicit_plan_set_input(self.h, _f32(y0), y0.shape[0]))
         self.B = y0.shape[0]
 
@@ -282,7 +325,7 @@ class Plan:
         if n < 0:
             _check(int(n))
         dtype = {0: np.int64, 1: np.int64, 2: np.int32, 3: np.float32, 4: np.int32,
-                 5: np.int32}[what]
+                 5: np.int32, 6: np.float32}[what]
         out = np.empty(max(int(n), 1), dtype=dtype)
         m = lib().snicit_plan_trace(self.h, what, out.ctypes.data_as(_P), out.size)
         if m < 0:
@@ -310,6 +353,17 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Model:
+    """A beyond-style MLP on one context: optional dense input layer, hidden layers (ELL),
+    optional dense output layer with argmax categories (src/beyond/SNICIT/SNICIT.hpp:59-74)."""
+
+    def __init__(self, ctx, cols, vals, bias, clip=1.0, w_in=None, b_in=None, in_clip=1.0,
+                 w_out=None, b_out=None):
+        self.net = Net.from_ell(ctx, cols, vals, bias, clip=clip)
+        if w_in is not None or w_out is not None:
+            self.net.set_io(w_in, b_in, in_clip, w_out, b_out)
 
 
 class _Method:

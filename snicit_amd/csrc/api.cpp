@@ -87,8 +87,41 @@ struct NetBuilder {
   std::vector<float> w;
   bool weights_given = true;
   int64_t w_floats = 0;
+  // wide-ELL layers
+  std::vector<int32_t> ecols;
+  std::vector<float> evals;
+  void add_ell(const int32_t* cols, const float* vals, int32_t K) {
+    const int32_t Kp = (K + 15) / 16 * 16;
+    const int32_t N = net->N;
+    net->panel_off.push_back(net->panel_off.back());
+    pbeg.push_back(0);
+    net->mfma.push_back(0);
+    net->w_off.push_back(w_floats);
+    net->ell_K.push_back(Kp);
+    net->ell_off.push_back((int64_t)ecols.size());
+    for (int32_t o = 0; o < N; ++o) outs.push_back(o);  // keeps the [L][N] row-id layout
+    std::vector<std::pair<int32_t, float>> tmp(K);
+    for (int32_t o = 0; o < N; ++o) {
+      for (int32_t r = 0; r < K; ++r) {
+        const int32_t c = cols[(size_t)o * K + r];
+        SNICIT_REQUIRE(c >= 0 && c < N, "ELL column out of range at row " + std::to_string(o));
+        tmp[r] = {c, vals[(size_t)o * K + r]};
+      }
+      // ascending input neuron, stable for ties (the oracle's order)
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::pair<int32_t, float>& x, const std::pair<int32_t, float>& y) {
+                         return x.first < y.first;
+                       });
+      for (int32_t r = 0; r < Kp; ++r) {
+        ecols.push_back(r < K ? tmp[r].first : tmp[K - 1].first);
+        evals.push_back(r < K ? tmp[r].second : 0.f);
+      }
+    }
+  }
   void add(const HostLayerPanels& hp) {
     const int64_t P = (int64_t)hp.pbeg.size() - 1;
+    net->ell_K.push_back(0);
+    net->ell_off.push_back((int64_t)ecols.size());
     net->panel_off.push_back(net->panel_off.back() + P);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
@@ -131,6 +164,12 @@ void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
     // generated nets: every panel has 32 rows, so both layouts hold only `fill_weight`
     launch_fill_f32(net->d_w, fill_weight, nb.w_floats, s);
   }
+  if (!nb.ecols.empty()) {
+    net->d_ecols = dev_alloc<int32_t>(nb.ecols.size(), &net->device_bytes);
+    net->d_evals = dev_alloc<float>(nb.evals.size(), &net->device_bytes);
+    SNICIT_HIP(hipMemcpy(net->d_ecols, nb.ecols.data(), nb.ecols.size() * 4, hipMemcpyHostToDevice));
+    SNICIT_HIP(hipMemcpy(net->d_evals, nb.evals.data(), nb.evals.size() * 4, hipMemcpyHostToDevice));
+  }
   if (net->bias_kind == SNICIT_BIAS_PER_NEURON) {
     net->d_bias = dev_alloc<float>((size_t)net->L * net->N, &net->device_bytes);
     SNICIT_HIP(hipMemcpy(net->d_bias, net->bias_host.data(), net->bias_host.size() * 4,
@@ -154,6 +193,12 @@ void net_set_bias(Net* net, int32_t bias_kind, const float* bias) {
 }
 
 void destroy_net(Net* net) {
+  dev_free(net->d_ecols);
+  dev_free(net->d_evals);
+  dev_free(net->d_win);
+  dev_free(net->d_bin);
+  dev_free(net->d_wout);
+  dev_free(net->d_bout);
   dev_free(net->d_pcols);
   dev_free(net->d_pbeg);
   dev_free(net->d_outs);
@@ -197,7 +242,16 @@ struct snicit_plan {
   int64_t img_params_cap = 0;
   int32_t* d_img_blobs = nullptr;
   int32_t* d_img_proto = nullptr;
-  float* d_stage = nullptr;  // [cap][N] staging for host transfers
+  float* d_stage = nullptr;  // [cap][max(N, Nin, Nout)] staging for host transfers
+  // nets with dense input / output layers
+  float* d_h0 = nullptr;      // [N][ld] input-layer output (hidden loop input), later the
+                              //   recovered hidden activations
+  float* d_logits = nullptr;  // [Nout][ld]
+  int64_t in_rows = 0;        // rows of d_in: N, or Nin padded to 16
+  // lossy clustering
+  float* d_poolrows = nullptr;  // [pool][N]
+  int32_t* d_poolcol = nullptr;
+  int32_t* d_centmark = nullptr;
   // where the last run left its final state
   int last_orig = 0, last_mcar = 0;
   const float* last_D = nullptr;
@@ -218,7 +272,85 @@ namespace {
 
 bool plain_mode(int32_t mode) { return mode != SNICIT_MODE_SNICIT; }
 bool compacting(const snicit_plan* p) {
-  return p->net->skip_dead && p->cfg.mode != SNICIT_MODE_PLAIN_DENSE;
+  return p->net->skip_dead && p->cfg.mode != SNICIT_MODE_PLAIN_DENSE && !p->net->has_io();
+}
+
+// Input of hidden layer 0: the input-layer output when the net has one.
+const float* hidden_in(const snicit_plan* p) { return p->net->Nin > 0 ? p->d_h0 : p->d_in; }
+
+// One hidden layer with the plan's conventions; panel or wide-ELL kernel by layer kind.
+void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
+  const Net& net = *p->net;
+  if (!net.is_ell(a.layer)) {
+    launch_panel_spmm(a, s);
+    return;
+  }
+  EllLaunch e{};
+  e.cols = net.d_ecols + net.ell_off[a.layer];
+  e.vals = net.d_evals + net.ell_off[a.layer];
+  e.K = net.ell_K[a.layer];
+  e.rows = net.N;
+  e.bias_vec = net.bias_vec(a.layer);
+  e.bias_s = net.bias_scalar(a.layer);
+  e.clip = net.clip;
+  e.epilogue = a.epilogue;
+  e.yin = a.yin;
+  e.ldin = a.ldin;
+  e.yout = a.yout;
+  e.ldout = a.ldout;
+  e.count = a.count;
+  e.map = a.map;
+  e.flags = a.flags;
+  e.zc = a.zc;
+  e.kcap = a.kcap;
+  e.mcent = a.mcent;
+  e.eps_post = a.eps_post;
+  e.max_cols = a.max_cols;
+  launch_ell_spmm(e, s);
+}
+
+// Dense input layer: d_in (Nin rows) -> d_h0, every column (src/beyond/general.hpp:5-33).
+void enqueue_input_layer(snicit_plan* p) {
+  const Net& net = *p->net;
+  if (net.Nin == 0) return;
+  EllLaunch e{};
+  e.cols = nullptr;
+  e.vals = net.d_win;
+  e.K = net.Kin;
+  e.rows = net.N;
+  e.bias_vec = net.d_bin;
+  e.clip = net.in_clip;
+  e.epilogue = EPI_PLAIN;
+  e.yin = p->d_in;
+  e.ldin = p->ld;
+  e.yout = p->d_h0;
+  e.ldout = p->ld;
+  e.count = p->d_counts;
+  e.max_cols = p->cap;
+  launch_ell_spmm(e, net.ctx->stream);
+}
+
+// Dense output layer on the hidden activations (all columns, sample order) and argmax
+// categories (src/beyond/general.hpp:74-128).
+void enqueue_output_layer(snicit_plan* p, const float* y) {
+  const Net& net = *p->net;
+  hipStream_t s = net.ctx->stream;
+  EllLaunch e{};
+  e.cols = nullptr;
+  e.vals = net.d_wout;
+  e.K = net.Kout;
+  e.rows = net.Nout;
+  e.bias_vec = net.d_bout;
+  e.clip = 0.f;
+  e.epilogue = EPI_LINEAR;
+  e.yin = y;
+  e.ldin = p->ld;
+  e.yout = p->d_logits;
+  e.ldout = p->ld;
+  e.count = p->d_counts;
+  e.max_cols = p->cap;
+  launch_ell_spmm(e, s);
+  launch_argmax(p->d_logits, p->ld, net.Nout, p->d_counts, p->cap, p->d_cat, s);
 }
 
 void prof_mark(snicit_plan* p, int l, int which) {
@@ -245,10 +377,11 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
   hipStream_t s = p->net->ctx->stream;
   const bool cmp = compacting(p);
   launch_iota_i32(p->d_orig[0], p->cap, s);
+  enqueue_input_layer(p);
   int cur = 0;
   for (int l = 0; l < l_end; ++l) {
     SpmmLaunch a = spmm_base(p, l, EPI_PLAIN);
-    a.yin = l == 0 ? p->d_in : p->d_y[(l - 1) & 1];
+    a.yin = l == 0 ? hidden_in(p) : p->d_y[(l - 1) & 1];
     a.ldin = p->ld;
     a.yout = p->d_y[l & 1];
     a.ldout = p->ld;
@@ -262,7 +395,7 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
       a.flags = (final_flags && l == l_end - 1) ? p->d_flags : nullptr;
     }
     prof_mark(p, l, 0);
-    launch_panel_spmm(a, s);
+    launch_layer(p, a, s);
     prof_mark(p, l, 1);
     if (cmp) {
       launch_compact(p->d_flags, p->d_counts + l, p->d_counts + l + 1, p->d_map, nullptr,
@@ -275,6 +408,10 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
 
 void enqueue_plain_categories(snicit_plan* p, int l_end, int cur) {
   hipStream_t s = p->net->ctx->stream;
+  if (p->net->Nout > 0) {
+    enqueue_output_layer(p, l_end == 0 ? hidden_in(p) : p->d_y[(l_end - 1) & 1]);
+    return;
+  }
   launch_fill_i32(p->d_cat, 0, p->cap, s);
   if (compacting(p)) {
     launch_cat_from_list(p->d_orig[cur], p->d_counts + l_end, p->cap, p->d_cat, s);
@@ -295,7 +432,7 @@ struct SnicitState {
 SnicitState snicit_state(snicit_plan* p) {
   SnicitState st;
   const int t = p->cfg.threshold;
-  st.Yt = t == 0 ? p->d_in : p->d_y[(t - 1) & 1];
+  st.Yt = t == 0 ? hidden_in(p) : p->d_y[(t - 1) & 1];
   st.Da = t == 0 ? p->d_y[0] : p->d_y[t & 1];
   st.Db = t == 0 ? p->d_y[1] : p->d_y[(t + 1) & 1];
   return st;
@@ -311,7 +448,35 @@ int cluster_cur(const snicit_plan* p) {
   return t == 0 ? 1 : (t & 1);
 }
 
+// Stage "cluster", lossy variant (src/beyond/SNICIT/SNICIT.hpp:539-566, kernel.hpp:5-101):
+// pool of strided rows, pruning on whole rows, tolerance assignment and pruned deltas.
+// Every row is live (no dead-row skipping in this variant), so positions are sample ids.
+void enqueue_cluster_lossy(snicit_plan* p) {
+  hipStream_t s = p->net->ctx->stream;
+  const int t = p->cfg.threshold;
+  const int N = p->net->N;
+  SnicitState st = snicit_state(p);
+  const snicit_run_cfg& c = p->cfg;
+  launch_lossy_pool(st.Yt, p->ld, p->d_counts, c.lossy_pool, N, p->d_poolrows, p->d_poolcol, s);
+  launch_lossy_select(p->d_poolrows, p->d_poolcol, p->d_counts, c.lossy_pool, N, c.lossy_eps_pool,
+                      p->d_cent_slot, p->d_k, s);
+  launch_gather_centroids(st.Yt, p->ld, nullptr, p->d_cent_slot, p->d_k, N, p->d_C[0], p->kcap, s);
+  launch_lossy_assign(st.Yt, p->ld, p->d_counts, p->cap, p->d_C[0], p->kcap, p->d_k,
+                      p->d_cent_slot, N, c.lossy_eps_assign, p->d_dist, p->d_mcar[0],
+                      p->d_centmark, p->d_rec, s);
+  launch_lossy_delta(st.Yt, p->ld, p->d_C[0], p->kcap, p->d_mcar[0], p->d_centmark, p->d_counts,
+                     p->cap, N, c.lossy_eps_assign, c.lossy_positive_only != 0, st.Da, p->ld,
+                     p->d_flags, s);
+  // non-empty delta rows: positions in Da, with original id and centroid carried
+  launch_compact(p->d_flags, p->d_counts, p->d_nerow + t, p->d_mapD, nullptr, p->d_orig[0],
+                 p->d_orig[1], p->d_mcar[0], p->d_mcar[1], s);
+}
+
 void enqueue_cluster(snicit_plan* p) {
+  if (p->cfg.cluster == SNICIT_CLUSTER_LOSSY) {
+    enqueue_cluster_lossy(p);
+    return;
+  }
   hipStream_t s = p->net->ctx->stream;
   const int t = p->cfg.threshold;
   const int N = p->net->N;
@@ -363,11 +528,12 @@ void enqueue_post(snicit_plan* p) {
   const bool lossy = p->cfg.cluster == SNICIT_CLUSTER_LOSSY;
   for (int l = t; l < L; ++l) {
     const int q = (l - t) & 1;
-    if (post_layer_fused(*p->net, l)) {
+    const int32_t* map_in = (l == t && !lossy) ? nullptr : p->d_mapD;
+    if (post_layer_fused(*p->net, l) && !p->net->is_ell(l) && p->kcap == kMaxCentroidsExact) {
       prof_mark(p, l, 0);
       launch_panel_post(*p->net, l, lossy, p->d_C[q], p->d_C[q ^ 1], p->kcap, p->d_k,
                         q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
-                        p->d_nerow + l, l == t ? nullptr : p->d_mapD, p->d_mcar[mc], p->d_flags,
+                        p->d_nerow + l, map_in, p->d_mcar[mc], p->d_flags,
                         p->cfg.lossy_eps_post, p->cap, s);
       prof_mark(p, l, 1);
       prof_mark(p, l, 2);
@@ -382,20 +548,20 @@ void enqueue_post(snicit_plan* p) {
       c.zc = p->d_zc;
       c.max_cols = p->kcap;
       prof_mark(p, l, 2);
-      launch_panel_spmm(c, s);
+      launch_layer(p, c, s);
       prof_mark(p, l, 3);
       SpmmLaunch d = spmm_base(p, l, lossy ? EPI_DELTA_LOSSY : EPI_DELTA);
       d.yin = q == 0 ? st.Da : st.Db;
       d.ldin = p->ld;
       d.yout = q == 0 ? st.Db : st.Da;
       d.ldout = p->ld;
-      d.map = l == t ? nullptr : p->d_mapD;
+      d.map = map_in;
       d.count = p->d_nerow + l;
       d.flags = p->d_flags;
       d.zc = p->d_zc;
       d.mcent = p->d_mcar[mc];
       prof_mark(p, l, 0);
-      launch_panel_spmm(d, s);
+      launch_layer(p, d, s);
       prof_mark(p, l, 1);
     }
     launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
@@ -408,6 +574,18 @@ void enqueue_post(snicit_plan* p) {
 void enqueue_recover(snicit_plan* p) {
   hipStream_t s = p->net->ctx->stream;
   const int L = p->net->L, N = p->net->N;
+  if (p->net->Nout > 0) {
+    // recovered hidden activations yhat_i + yhat_m(i) for every sample (the reference adds
+    // the centroid back in place, src/beyond/SNICIT/kernel.hpp:170-183), then the output
+    // layer and argmax
+    launch_expand_centroids(p->d_rec, p->d_counts, p->cap, p->last_C, p->kcap, N, p->d_h0, p->ld,
+                            s);
+    launch_scatter_delta(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar],
+                         p->d_orig[p->last_orig], p->d_nerow + L, p->cap, p->last_C, p->kcap, N,
+                         p->d_h0, p->ld, s);
+    enqueue_output_layer(p, p->d_h0);
+    return;
+  }
   launch_centroid_cats(p->last_C, p->kcap, p->d_k, N, p->d_catc, s);
   launch_recover_base(p->d_rec, p->d_counts, p->cap, p->d_catc, p->d_cat, s);
   launch_recover_rows(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar], p->d_orig[p->last_orig],
@@ -508,7 +686,8 @@ void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
     stats->ms_total = tot;
     stats->batch = p->B;
     stats->layers = p->stop_layer;
-    stats->edges = (double)p->B * (double)net->N * (double)net->K * (double)p->stop_layer;
+    stats->edges = (double)p->B * ((double)net->nnz_hidden * (double)p->stop_layer +
+                                   (double)net->N * net->Nin + (double)net->Nout * net->N);
     std::vector<int32_t> counts(L + 4), nerow(L + 2);
     SNICIT_HIP(hipMemcpy(counts.data(), p->d_counts, counts.size() * 4, hipMemcpyDeviceToHost));
     SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
@@ -545,7 +724,8 @@ void plan_free(snicit_plan* p) {
                   p->d_mapD,   p->d_orig[0], p->d_orig[1],  p->d_mcar[0], p->d_mcar[1],
                   p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
                   p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
-                  p->d_dist,
+                  p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
+                  p->d_centmark,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
                   p->d_stage};
   for (void* q : ptrs) dev_free(q);
@@ -604,8 +784,7 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, cons
                           snicit_net** out) {
   return guarded([&] {
     SNICIT_REQUIRE(ctx && out && cols && vals, "null argument");
-    SNICIT_REQUIRE(N > 0 && L > 0 && K > 0 && K <= kSlots,
-                   "need N > 0, L > 0 and 0 < K <= 32");
+    SNICIT_REQUIRE(N > 0 && L > 0 && K > 0 && K <= N, "need N > 0, L > 0 and 0 < K <= N");
     SNICIT_REQUIRE(clip > 0.f, "clip must be positive");
     SNICIT_HIP(hipSetDevice(ctx->device));
     auto net = std::make_unique<snicit_net>();
@@ -616,9 +795,15 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, cons
     net->clip = clip;
     net_set_bias(net.get(), bias_kind, bias);
     net->panel_off.push_back(0);
+    net->nnz_hidden = (int64_t)N * K;
     NetBuilder nb{net.get()};
     HostLayerPanels hp;
     for (int32_t l = 0; l < L; ++l) {
+      if (K > kSlots) {
+        // rows wider than a panel: wide-ELL layer
+        nb.add_ell(cols + (size_t)l * N * K, vals + (size_t)l * N * K, K);
+        continue;
+      }
       pack_layer_ell(N, K, cols + (size_t)l * N * K, vals + (size_t)l * N * K, hp);
       nb.add(hp);
     }
@@ -647,6 +832,7 @@ int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t see
     net->clip = clip;
     net_set_bias(net.get(), SNICIT_BIAS_SCALAR, &bias);
     net->panel_off.push_back(0);
+    net->nnz_hidden = (int64_t)N * kSlots;
     NetBuilder nb{net.get()};
     nb.weights_given = false;
     HostLayerPanels hp;
@@ -661,6 +847,49 @@ int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t see
       throw;
     }
     *out = net.release();
+  });
+}
+
+int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const float* b_in,
+                      float in_clip, int32_t Nout, const float* w_out, const float* b_out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net != nullptr, "null net");
+    SNICIT_REQUIRE(Nin >= 0 && Nout >= 0, "sizes must be >= 0");
+    SNICIT_REQUIRE(Nin == 0 || (w_in && b_in && in_clip > 0.f), "input layer needs weights, bias, clip");
+    SNICIT_REQUIRE(Nout == 0 || (w_out && b_out), "output layer needs weights and bias");
+    SNICIT_REQUIRE(Nout == 0 || net->N % 16 == 0, "an output layer needs N divisible by 16");
+    SNICIT_REQUIRE(net->live_plans == 0,
+                   "set the input/output layers before creating plans on the net");
+    SNICIT_HIP(hipSetDevice(net->ctx->device));
+    const int32_t N = net->N;
+    dev_free(net->d_win);
+    dev_free(net->d_bin);
+    dev_free(net->d_wout);
+    dev_free(net->d_bout);
+    net->d_win = net->d_bin = net->d_wout = net->d_bout = nullptr;
+    net->Nin = Nin;
+    net->Nout = Nout;
+    net->Kin = (Nin + 15) / 16 * 16;
+    net->Kout = (N + 15) / 16 * 16;
+    net->in_clip = in_clip;
+    if (Nin > 0) {
+      std::vector<float> w((size_t)N * net->Kin, 0.f);
+      for (int32_t o = 0; o < N; ++o)
+        std::memcpy(&w[(size_t)o * net->Kin], w_in + (size_t)o * Nin, (size_t)Nin * 4);
+      net->d_win = dev_alloc<float>(w.size(), &net->device_bytes);
+      net->d_bin = dev_alloc<float>(N, &net->device_bytes);
+      SNICIT_HIP(hipMemcpy(net->d_win, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(net->d_bin, b_in, (size_t)N * 4, hipMemcpyHostToDevice));
+    }
+    if (Nout > 0) {
+      std::vector<float> w((size_t)Nout * net->Kout, 0.f);
+      for (int32_t o = 0; o < Nout; ++o)
+        std::memcpy(&w[(size_t)o * net->Kout], w_out + (size_t)o * N, (size_t)N * 4);
+      net->d_wout = dev_alloc<float>(w.size(), &net->device_bytes);
+      net->d_bout = dev_alloc<float>(Nout, &net->device_bytes);
+      SNICIT_HIP(hipMemcpy(net->d_wout, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(net->d_bout, b_out, (size_t)Nout * 4, hipMemcpyHostToDevice));
+    }
   });
 }
 
@@ -696,12 +925,20 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_REQUIRE(cfg->threshold >= 0 && cfg->threshold < net->L,
                      "threshold must satisfy 0 <= t < L (got " + std::to_string(cfg->threshold) +
                          ", L = " + std::to_string(net->L) + ")");
-      SNICIT_REQUIRE(cfg->cluster == SNICIT_CLUSTER_EXACT,
-                     "lossy clustering is not available for panel nets");
-      SNICIT_REQUIRE(cfg->sample_size >= 1 && cfg->sample_size <= kMaxCentroidsExact,
-                     "sample_size must be in [1, 32]");
-      SNICIT_REQUIRE(cfg->segments >= 1 && cfg->segments <= 32 && net->N % cfg->segments == 0,
-                     "segments must divide N and be <= 32");
+      SNICIT_REQUIRE(cfg->cluster == SNICIT_CLUSTER_EXACT || cfg->cluster == SNICIT_CLUSTER_LOSSY,
+                     "bad cluster variant");
+      if (cfg->cluster == SNICIT_CLUSTER_EXACT) {
+        SNICIT_REQUIRE(cfg->sample_size >= 1 && cfg->sample_size <= kMaxCentroidsExact,
+                       "sample_size must be in [1, 32]");
+        SNICIT_REQUIRE(cfg->segments >= 1 && cfg->segments <= 32 && net->N % cfg->segments == 0,
+                       "segments must divide N and be <= 32");
+      } else {
+        SNICIT_REQUIRE(cfg->lossy_pool >= 1 && cfg->lossy_pool <= kMaxCentroidsLossy,
+                       "lossy_pool must be in [1, 128]");
+        SNICIT_REQUIRE(!net->skip_dead || net->has_io(),
+                       "lossy clustering runs on nets without dead-row skipping (a bias > 0 or "
+                       "dense input/output layers)");
+      }
     }
     SNICIT_HIP(hipSetDevice(net->ctx->device));
     auto p = std::make_unique<snicit_plan>();
@@ -709,10 +946,22 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
     p->cfg = *cfg;
     p->cap = max_batch;
     p->ld = (max_batch + 63) / 64 * 64;
+    const bool lossy = cfg->mode == SNICIT_MODE_SNICIT && cfg->cluster == SNICIT_CLUSTER_LOSSY;
+    p->kcap = lossy ? kMaxCentroidsLossy : kMaxCentroidsExact;
+    p->in_rows = net->Nin > 0 ? net->Kin : net->N;
     const int N = net->N, L = net->L;
     try {
       int64_t* acct = &p->device_bytes;
-      p->d_in = dev_alloc<float>((size_t)N * p->ld, acct);
+      p->d_in = dev_alloc<float>((size_t)p->in_rows * p->ld, acct);
+      if (net->Nin > 0)  // rows Nin..Kin-1 meet zero weights and stay zero
+        SNICIT_HIP(hipMemset(p->d_in, 0, (size_t)p->in_rows * p->ld * 4));
+      if (net->has_io()) p->d_h0 = dev_alloc<float>((size_t)N * p->ld, acct);
+      if (net->Nout > 0) p->d_logits = dev_alloc<float>((size_t)net->Nout * p->ld, acct);
+      if (lossy) {
+        p->d_poolrows = dev_alloc<float>((size_t)kMaxCentroidsLossy * N, acct);
+        p->d_poolcol = dev_alloc<int32_t>(kMaxCentroidsLossy, acct);
+        p->d_centmark = dev_alloc<int32_t>(p->ld, acct);
+      }
       p->d_y[0] = dev_alloc<float>((size_t)N * p->ld, acct);
       p->d_y[1] = dev_alloc<float>((size_t)N * p->ld, acct);
       p->d_flags = dev_alloc<int32_t>(p->ld, acct);
@@ -727,11 +976,11 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       p->d_nerow = dev_alloc<int32_t>(L + 2, acct);
       p->d_k = dev_alloc<int32_t>(1, acct);
       p->d_F = dev_alloc<float>(32 * 32, acct);
-      p->d_cent_slot = dev_alloc<int32_t>(64, acct);
+      p->d_cent_slot = dev_alloc<int32_t>(kMaxCentroidsLossy, acct);
       p->d_zc = dev_alloc<float>((size_t)N * p->kcap, acct);
       p->d_rec = dev_alloc<int32_t>(p->ld, acct);
-      p->d_dist = dev_alloc<int32_t>((size_t)p->ld * kMaxCentroidsExact, acct);
-      SNICIT_HIP(hipMemset(p->d_dist, 0, (size_t)p->ld * kMaxCentroidsExact * 4));
+      p->d_dist = dev_alloc<int32_t>((size_t)p->ld * p->kcap, acct);
+      SNICIT_HIP(hipMemset(p->d_dist, 0, (size_t)p->ld * p->kcap * 4));
       p->d_catc = dev_alloc<int32_t>(p->kcap, acct);
       p->d_act = dev_alloc<int32_t>(p->ld, acct);
       p->d_cat = dev_alloc<int32_t>(p->ld, acct);
@@ -748,6 +997,7 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       plan_free(p.get());
       throw;
     }
+    ++net->live_plans;
     *out = p.release();
   });
 }
@@ -756,6 +1006,7 @@ int snicit_plan_destroy(snicit_plan* plan) {
   return guarded([&] {
     if (!plan) return;
     (void)hipStreamSynchronize(plan->net->ctx->stream);
+    --plan->net->live_plans;
     plan_free(plan);
     delete plan;
   });
@@ -763,7 +1014,9 @@ int snicit_plan_destroy(snicit_plan* plan) {
 
 namespace {
 void ensure_stage(snicit_plan* p) {
-  if (!p->d_stage) p->d_stage = dev_alloc<float>((size_t)p->cap * p->net->N, &p->device_bytes);
+  const Net& n = *p->net;
+  const int64_t w = std::max<int64_t>(n.N, std::max<int64_t>(n.Nin, n.Nout));
+  if (!p->d_stage) p->d_stage = dev_alloc<float>((size_t)p->cap * w, &p->device_bytes);
 }
 void set_count(snicit_plan* p, int64_t B) {
   int32_t b32 = (int32_t)B;
@@ -780,7 +1033,8 @@ int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
   return guarded([&] {
     SNICIT_REQUIRE(p && (y0 || B == 0), "null argument");
     SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
-    const int N = p->net->N;
+    // input rows: Nin features with an input layer, else N neurons
+    const int N = p->net->Nin > 0 ? p->net->Nin : p->net->N;
     hipStream_t s = p->net->ctx->stream;
     if (B > 0) {
       ensure_stage(p);
@@ -794,6 +1048,7 @@ int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
 int snicit_plan_set_input_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B) {
   return guarded([&] {
     SNICIT_REQUIRE(p != nullptr, "null plan");
+    SNICIT_REQUIRE(p->net->Nin == 0, "image inputs feed nets without an input layer");
     SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
     const int N = p->net->N;
     hipStream_t s = p->net->ctx->stream;
@@ -946,6 +1201,13 @@ int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) 
       case 2:
       case 4: {
         if (!snic) throw Error(SNICIT_E_STATE, "trace needs a SNICIT run");
+        if (what == 2 && p->cfg.cluster == SNICIT_CLUSTER_LOSSY) {
+          n = p->B > 0 ? p->cfg.lossy_pool : 0;
+          std::vector<int32_t> pc(std::max<int64_t>(n, 1));
+          if (n) SNICIT_HIP(hipMemcpy(pc.data(), p->d_poolcol, n * 4, hipMemcpyDeviceToHost));
+          if (out) std::memcpy(out, pc.data(), std::min<int64_t>(n, cap) * 4);
+          break;
+        }
         // pool rows / centroids as original ids: the orig list at t
         std::vector<int32_t> slots;
         if (what == 2) {
@@ -971,7 +1233,7 @@ int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) 
       }
       case 3: {
         if (!snic) throw Error(SNICIT_E_STATE, "trace needs a SNICIT run");
-        n = (int64_t)pool * p->cfg.segments;
+        n = p->cfg.cluster == SNICIT_CLUSTER_LOSSY ? 0 : (int64_t)pool * p->cfg.segments;
         std::vector<float> F(32 * 32);
         SNICIT_HIP(hipMemcpy(F.data(), p->d_F, F.size() * 4, hipMemcpyDeviceToHost));
         if (out) std::memcpy(out, F.data(), std::min<int64_t>(n, cap) * 4);
@@ -984,6 +1246,21 @@ int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) 
           std::vector<int32_t> rec(p->B);
           SNICIT_HIP(hipMemcpy(rec.data(), p->d_rec, p->B * 4, hipMemcpyDeviceToHost));
           std::memcpy(out, rec.data(), std::min<int64_t>(n, cap) * 4);
+        }
+        break;
+      }
+      case 6: {  // output-layer logits, B x Nout, sample-major
+        const Net& net = *p->net;
+        if (net.Nout == 0) throw Error(SNICIT_E_STATE, "net has no output layer");
+        n = p->B * net.Nout;
+        if (out && p->B) {
+          ensure_stage(p);
+          hipStream_t s = net.ctx->stream;
+          launch_transpose_out(p->d_logits, p->ld, p->B, net.Nout, p->d_stage, s);
+          SNICIT_HIP(hipStreamSynchronize(s));
+          std::vector<float> lg((size_t)n);
+          SNICIT_HIP(hipMemcpy(lg.data(), p->d_stage, lg.size() * 4, hipMemcpyDeviceToHost));
+          std::memcpy(out, lg.data(), std::min<int64_t>(n, cap) * 4);
         }
         break;
       }
@@ -1075,6 +1352,26 @@ int snicit_gen_sdgc_ell(int32_t N, int32_t L, uint64_t seed, int32_t* cols) {
   return guarded([&] {
     SNICIT_REQUIRE(cols != nullptr && L > 0, "bad argument");
     for (int32_t l = 0; l < L; ++l) sdgc_layer_ell(N, l, seed, cols + (size_t)l * N * kSlots);
+  });
+}
+
+int snicit_gen_unstructured(int32_t Nin, int32_t N, int32_t L, int32_t K, int32_t Nout,
+                            uint64_t seed, int32_t* cols, float* vals, float* w_in, float* w_out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(cols && vals && N > 0 && L > 0 && K > 0 && K <= N, "bad argument");
+    SNICIT_REQUIRE(Nin == 0 || w_in, "w_in needed when Nin > 0");
+    SNICIT_REQUIRE(Nout == 0 || w_out, "w_out needed when Nout > 0");
+    for (int32_t l = 0; l < L; ++l)
+      unstructured_layer(N, K, l, seed, cols + (size_t)l * N * K, vals + (size_t)l * N * K);
+    if (Nin > 0) dense_layer(N, Nin, seed, 0x1D1D1D1D1ULL, w_in);
+    if (Nout > 0) dense_layer(Nout, N, seed, 0x0F0F0F0F0ULL, w_out);
+  });
+}
+
+int snicit_gen_features(int32_t Nin, uint64_t seed, int64_t first_sample, int64_t B, float* out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(Nin > 0 && (out || B == 0), "bad argument");
+    uniform_features(Nin, seed, first_sample, B, out);
   });
 }
 

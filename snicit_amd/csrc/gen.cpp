@@ -168,4 +168,51 @@ void image_render(const ImageGeom& g, const SampleParams& p, int32_t N, float* r
   }
 }
 
+// ---- unstructured nets (beyond-style MLPs) -----------------------------------------------
+// Hidden layer l: row o draws K distinct inputs uniformly (partial Fisher-Yates from
+// Rng{seed ^ mix(l*N + o + 1)}) with weights N(0, 2/K) by Box-Muller; the dense input layer
+// (N x Nin) uses N(0, 2/Nin) and the output layer (Nout x N) N(0, 2/N), each row from its own
+// stream.  Features are U[0,1).
+static double unit(Rng& r) { return (double)(r.next() >> 11) * (1.0 / 9007199254740992.0); }
+
+static float gauss(Rng& r, double sigma) {
+  const double u1 = unit(r), u2 = unit(r);
+  const double z = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(6.283185307179586 * u2);
+  return (float)(z * sigma);
+}
+
+void unstructured_layer(int32_t N, int32_t K, int32_t l, uint64_t seed, int32_t* cols,
+                        float* vals) {
+  std::vector<int32_t> idx(N);
+  const double sigma = std::sqrt(2.0 / K);
+  for (int32_t o = 0; o < N; ++o) {
+    Rng r{seed ^ sm_mix((uint64_t)l * (uint64_t)N + (uint64_t)o + 1)};
+    for (int32_t i = 0; i < N; ++i) idx[i] = i;
+    for (int32_t i = 0; i < K; ++i) {
+      const int32_t j = i + (int32_t)(r.next() % (uint64_t)(N - i));
+      std::swap(idx[i], idx[j]);
+    }
+    std::sort(idx.begin(), idx.begin() + K);
+    for (int32_t i = 0; i < K; ++i) {
+      cols[(size_t)o * K + i] = idx[i];
+      vals[(size_t)o * K + i] = gauss(r, sigma);
+    }
+  }
+}
+
+void dense_layer(int32_t rows, int32_t fan_in, uint64_t seed, uint64_t tag, float* w) {
+  const double sigma = std::sqrt(2.0 / fan_in);
+  for (int32_t o = 0; o < rows; ++o) {
+    Rng r{seed ^ sm_mix(tag ^ ((uint64_t)o + 1) * 0x9E3779B97F4A7C15ULL)};
+    for (int32_t i = 0; i < fan_in; ++i) w[(size_t)o * fan_in + i] = gauss(r, sigma);
+  }
+}
+
+void uniform_features(int32_t Nin, uint64_t seed, int64_t first, int64_t B, float* out) {
+  for (int64_t b = 0; b < B; ++b) {
+    Rng r{seed ^ sm_mix((uint64_t)(first + b) ^ 0x3C6EF372FE94F82AULL)};
+    for (int32_t i = 0; i < Nin; ++i) out[(size_t)b * Nin + i] = (float)unit(r);
+  }
+}
+
 }  // namespace snicit

@@ -83,7 +83,25 @@ struct Net {
   std::vector<int64_t> w_off;      // [L] first float of each layer in d_w
   std::vector<uint8_t> mfma;       // [L] layer runs on the matrix cores (slot-major weights)
   float* d_bias = nullptr;         // [L][N] when bias_kind == PER_NEURON
+  // wide-ELL layers (rows with more than 32 nonzeros): rows padded to ell_K (multiple of 16)
+  std::vector<int32_t> ell_K;      // [L] padded row length, 0 for panel layers
+  std::vector<int64_t> ell_off;    // [L] first entry of the layer in d_ecols / d_evals
+  int32_t* d_ecols = nullptr;
+  float* d_evals = nullptr;
+  // optional dense input (Nin -> N, clip in_clip) and output (N -> Nout, linear) layers,
+  // src/beyond/general.hpp:5-33,74-102; rows zero-padded to Kin / Kout (multiples of 16)
+  int32_t Nin = 0, Kin = 0, Nout = 0, Kout = 0;
+  float in_clip = 1.f;
+  float* d_win = nullptr;   // [N][Kin]
+  float* d_bin = nullptr;   // [N]
+  float* d_wout = nullptr;  // [Nout][Kout]
+  float* d_bout = nullptr;  // [Nout]
+  int64_t nnz_hidden = 0;   // nonzeros per hidden layer as given (edges per sample per layer)
   int64_t device_bytes = 0;
+  int32_t live_plans = 0;   // plans sized for this net's input/output layers
+
+  bool has_io() const { return Nin > 0 || Nout > 0; }
+  bool is_ell(int l) const { return ell_K[l] > 0; }
 
   int64_t panels(int l) const { return panel_off[l + 1] - panel_off[l]; }
   float bias_scalar(int l) const {
@@ -121,6 +139,11 @@ struct SampleParams {
 };
 SampleParams image_sample_params(const ImageGeom& g, uint64_t seed, int64_t sample);
 void image_render(const ImageGeom& g, const SampleParams& p, int32_t N, float* row);
+
+void unstructured_layer(int32_t N, int32_t K, int32_t l, uint64_t seed, int32_t* cols,
+                        float* vals);
+void dense_layer(int32_t rows, int32_t fan_in, uint64_t seed, uint64_t tag, float* w);
+void uniform_features(int32_t Nin, uint64_t seed, int64_t first, int64_t B, float* out);
 
 // ---- packing (net.cpp) -----------------------------------------------------------
 void pack_layer_ell(int32_t N, int32_t K, const int32_t* cols, const float* vals,

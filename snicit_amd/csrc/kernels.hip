@@ -43,6 +43,8 @@ __device__ __forceinline__ float epilogue(float acc, float b, float clip, bool v
     const float z = acc + b;
     if (valid) zc[(int64_t)o * kcap + pos] = z;
     y = act_clip(z, clip);
+  } else if (EPI == EPI_LINEAR) {
+    y = acc + b;
   } else {
     const float z = valid ? zc[(int64_t)o * kcap + mi] : 0.f;
     y = act_clip(z + acc, clip) - act_clip(z, clip);
@@ -401,6 +403,223 @@ __global__ __launch_bounds__(kBlock, 2) void panel_mfma_kernel(
       if (flags && half == 0 && vb && any) flags[pb] = 1;
     }
   }
+}
+
+// ---- wide ELL SpMM -------------------------------------------------------------------
+// Rows with more than 32 nonzeros (unstructured nets, src/beyond/general.hpp:36-72 and
+// src/beyond/SNICIT/kernel.hpp:103-139), and dense layers (cols == null: entry r reads input
+// r; src/beyond/general.hpp:5-33,74-102).  Block (32-row chunk, column tile); each
+// wavefront owns 64 columns and walks its rows with the fixed ascending-input chain, entries
+// fetched 16 at a time with scalar loads.  Padded entries carry value 0 (a zero product
+// leaves the chain unchanged).  Columns <= kNarrowCols: one thread per (row, column) pair.
+template <int EPI>
+__global__ __launch_bounds__(kBlock) void ell_spmm_kernel(
+    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+  const int r0 = blockIdx.x * 32, r1 = min(rows, r0 + 32);
+  const int64_t count = *count_ptr;
+  if (count <= kNarrowCols) {
+    if (blockIdx.y != 0) return;
+    const int G = r1 - r0;
+    const int64_t pairs = (int64_t)G * count;
+    for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
+      const int o = r0 + (int)(q % G);
+      const int64_t pos = q / G;
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+      const float* __restrict__ v = vals + (size_t)o * K;
+      const int32_t* __restrict__ c = cols ? cols + (size_t)o * K : nullptr;
+      float acc = 0.f;
+      for (int r = 0; r < K; ++r)
+        acc = __builtin_fmaf(v[r], yin[(int64_t)(c ? c[r] : r) * ldin + src], acc);
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+      const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
+      bool flag = false;
+      const float y = epilogue<EPI>(acc, b, clip, true, o, pos, mi, zc, kcap, eps_post, flag);
+      yout[(int64_t)o * ldout + pos] = y;
+      if (flags && flag) flags[pos] = 1;
+    }
+    return;
+  }
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
+       base += (int64_t)gridDim.y * kBlock) {
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
+    int32_t mi = 0;
+    if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = mcent[valid ? pos : 0];
+    bool flag = false;
+    for (int o = r0; o < r1; ++o) {
+      const float* __restrict__ v = vals + (size_t)o * K;
+      const int32_t* __restrict__ c = cols ? cols + (size_t)o * K : nullptr;
+      float acc = 0.f;
+      for (int r = 0; r < K; r += 16) {
+        float x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = yin[(int64_t)(c ? c[r + q] : r + q) * ldin + src];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc = __builtin_fmaf(v[r + q], valid ? x[q] : 0.f, acc);
+      }
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+      const float y = epilogue<EPI>(acc, b, clip, valid, o, pos, mi, zc, kcap, eps_post, flag);
+      if (valid) yout[(int64_t)o * ldout + pos] = y;
+    }
+    if (flags && valid && flag) flags[pos] = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void argmax_kernel(const float* __restrict__ logits, int64_t ld,
+                                                      int32_t rows,
+                                                      const int32_t* __restrict__ count_ptr,
+                                                      int32_t* __restrict__ cat) {
+  const int64_t count = *count_ptr;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  // check_acc, src/beyond/general.hpp:112-119: first maximum, starting from -10000
+  float best = -10000.f;
+  int arg = 0;
+  for (int o = 0; o < rows; ++o) {
+    const float v = logits[(int64_t)o * ld + j];
+    if (v > best) {
+      best = v;
+      arg = o;
+    }
+  }
+  cat[j] = arg;
+}
+
+// ---- lossy clustering ------------------------------------------------------------------
+// Pool (y_star_gen, src/beyond/SNICIT/kernel.hpp:12): pool row y is sample y*B/pool.
+__global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict__ Y, int64_t ld,
+                                                          const int32_t* __restrict__ B_ptr,
+                                                          int32_t pool, int32_t N,
+                                                          float* __restrict__ poolrows,
+                                                          int32_t* __restrict__ poolcol) {
+  const int64_t B = *B_ptr;
+  const int y = blockIdx.x;
+  const int64_t col = B > 0 ? (int64_t)y * B / pool : 0;
+  if (threadIdx.x == 0) poolcol[y] = (int32_t)col;
+  for (int n = threadIdx.x; n < N; n += blockDim.x)
+    poolrows[(int64_t)y * N + n] = B > 0 ? Y[(int64_t)n * ld + col] : 0.f;
+}
+
+// Pruning (y_star_gen, src/beyond/SNICIT/kernel.hpp:19-41): in pool order, every present
+// base i removes each other present row y with #{n : |P_y[n] - P_i[n]| > eps} < N*eps.
+// One workgroup; wavefront w tests rows w, w+16, ...; lanes stride the neurons.
+__global__ __launch_bounds__(1024) void lossy_select_kernel(
+    const float* __restrict__ poolrows, const int32_t* __restrict__ poolcol,
+    const int32_t* __restrict__ B_ptr, int32_t pool, int32_t N, float eps,
+    int32_t* __restrict__ cent_col, int32_t* __restrict__ k_out) {
+  __shared__ int32_t alive[kMaxCentroidsLossy];
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
+  const int64_t B = *B_ptr;
+  const int P = B > 0 ? pool : 0;
+  for (int y = t; y < kMaxCentroidsLossy; y += blockDim.x) alive[y] = y < P;
+  __syncthreads();
+  const double limit = (double)N * (double)eps;
+  for (int i = 0; i < P; ++i) {
+    if (!alive[i]) continue;  // uniform: read after the barrier below
+    const float* __restrict__ pi = poolrows + (int64_t)i * N;
+    for (int y = wv; y < P; y += 16) {
+      if (y == i || !alive[y]) continue;
+      const float* __restrict__ py = poolrows + (int64_t)y * N;
+      int cnt = 0;
+      for (int n0 = 0; n0 < N; n0 += kWave) {
+        const int n = n0 + lane;
+        const bool d = n < N && fabsf(py[n] - pi[n]) > eps;
+        cnt += __popcll(__ballot(d));
+      }
+      if ((double)cnt < limit && lane == 0) alive[y] = 0;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    int k = 0;
+    for (int y = 0; y < P; ++y)
+      if (alive[y]) cent_col[k++] = poolcol[y];
+    *k_out = k;
+  }
+}
+
+// Assignment (coarse_cluster, src/beyond/SNICIT/kernel.hpp:63-87): distance = #{n :
+// |C_c[n] - Y_i[n]| > eps}, first minimum; centroid columns map to their own index.
+__global__ __launch_bounds__(256) void lossy_dist_kernel(
+    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ B_ptr,
+    const float* __restrict__ C, int32_t kcap, const int32_t* __restrict__ k_ptr, int32_t N,
+    int32_t nchunk, float eps, int32_t* __restrict__ dist) {
+  const int64_t count = *B_ptr;
+  const int k = *k_ptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int n0 = (int)((int64_t)N * blockIdx.y / nchunk);
+  const int n1 = (int)((int64_t)N * (blockIdx.y + 1) / nchunk);
+  for (int cb = 0; cb < k; cb += 8) {
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n = n0; n < n1; ++n) {
+      const float y = Y[(int64_t)n * ld + i];
+      const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cnt[q] += fabsf(cr[q] - y) > eps;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (cb + q < k && cnt[q]) atomicAdd(&dist[i * kcap + cb + q], cnt[q]);
+  }
+}
+
+__global__ __launch_bounds__(256) void lossy_final_kernel(
+    const int32_t* __restrict__ B_ptr, const int32_t* __restrict__ k_ptr,
+    const int32_t* __restrict__ cent_col, int32_t* __restrict__ dist, int32_t kcap,
+    int32_t* __restrict__ mcar, int32_t* __restrict__ centmark, int32_t* __restrict__ rec) {
+  const int64_t count = *B_ptr;
+  const int k = *k_ptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int best = 0x7fffffff, bestc = 0;
+  for (int c = 0; c < k; ++c) {
+    const int d = dist[i * kcap + c];
+    dist[i * kcap + c] = 0;
+    if (d < best) {
+      best = d;
+      bestc = c;
+    }
+  }
+  int mark = 0;
+  for (int c = 0; c < k; ++c)
+    if (cent_col[c] == i && !mark) {
+      mark = 1;
+      bestc = c;
+    }
+  mcar[i] = bestc;
+  centmark[i] = mark;
+  rec[i] = k > 0 ? bestc : -1;
+}
+
+// Delta (coarse_cluster, src/beyond/SNICIT/kernel.hpp:89-99): v = |Y_i - C_m| > eps ?
+// Y_i - C_m : 0; non-empty iff any v != 0 (any v > 0 with positive_only, as the reference).
+__global__ __launch_bounds__(256) void lossy_delta_kernel(
+    const float* __restrict__ Y, int64_t ld, const float* __restrict__ C, int32_t kcap,
+    const int32_t* __restrict__ mcar, const int32_t* __restrict__ centmark,
+    const int32_t* __restrict__ B_ptr, int32_t N, float eps, int32_t positive_only,
+    float* __restrict__ D, int64_t ldD, int32_t* __restrict__ flags) {
+  const int64_t count = *B_ptr;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count || centmark[i]) return;
+  const int c = mcar[i];
+  const int n0 = blockIdx.y * 64, n1 = min(N, n0 + 64);
+  bool any = false;
+#pragma unroll 8
+  for (int n = n0; n < n1; ++n) {
+    const float d = Y[(int64_t)n * ld + i] - C[(int64_t)n * kcap + c];
+    const float v = fabsf(d) > eps ? d : 0.f;
+    D[(int64_t)n * ldD + i] = v;
+    any |= positive_only ? (v > 0.f) : (v != 0.f);
+  }
+  if (any) flags[i] = 1;
 }
 
 // ---- stable compaction ----------------------------------------------------------------
@@ -935,6 +1154,78 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, dout, ldout, count, map, mcent, flags, eps_post);
+  check_launch();
+}
+
+void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
+  SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
+  const int64_t chunks = (a.rows + 31) / 32;
+  const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
+  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(chunks, 1));
+  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
+  gy = std::min<int64_t>(gy, 65535);
+  dim3 grid((unsigned)chunks, (unsigned)gy);
+#define SNICIT_ELL(E)                                                                        \
+  hipLaunchKernelGGL(ell_spmm_kernel<E>, grid, dim3(kBlock), 0, s, a.cols, a.vals, a.K, a.rows, \
+                     a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout, a.ldout, a.count,   \
+                     a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post)
+  switch (a.epilogue) {
+    case EPI_PLAIN: SNICIT_ELL(EPI_PLAIN); break;
+    case EPI_CENTROID: SNICIT_ELL(EPI_CENTROID); break;
+    case EPI_DELTA: SNICIT_ELL(EPI_DELTA); break;
+    case EPI_DELTA_LOSSY: SNICIT_ELL(EPI_DELTA_LOSSY); break;
+    case EPI_LINEAR: SNICIT_ELL(EPI_LINEAR); break;
+    default: throw Error(SNICIT_E_INVALID, "bad epilogue");
+  }
+#undef SNICIT_ELL
+  check_launch();
+}
+
+void launch_argmax(const float* logits, int64_t ld, int32_t rows, const int32_t* count,
+                   int64_t max_cols, int32_t* cat, hipStream_t s) {
+  hipLaunchKernelGGL(argmax_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s, logits,
+                     ld, rows, count, cat);
+  check_launch();
+}
+
+void launch_lossy_pool(const float* Y, int64_t ld, const int32_t* B, int32_t pool, int32_t N,
+                       float* poolrows, int32_t* poolcol, hipStream_t s) {
+  SNICIT_REQUIRE(pool >= 1 && pool <= kMaxCentroidsLossy, "lossy pool must be in [1, 128]");
+  hipLaunchKernelGGL(lossy_pool_kernel, dim3((unsigned)pool), dim3(256), 0, s, Y, ld, B, pool, N,
+                     poolrows, poolcol);
+  check_launch();
+}
+
+void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const int32_t* B,
+                         int32_t pool, int32_t N, float eps, int32_t* cent_col, int32_t* k_out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(lossy_select_kernel, dim3(1), dim3(1024), 0, s, poolrows, poolcol, B, pool,
+                     N, eps, cent_col, k_out);
+  check_launch();
+}
+
+void launch_lossy_assign(const float* Y, int64_t ld, const int32_t* B, int64_t max_cols,
+                         const float* C, int32_t kcap, const int32_t* k,
+                         const int32_t* cent_col, int32_t N, float eps, int32_t* dist,
+                         int32_t* mcar, int32_t* centmark, int32_t* rec, hipStream_t s) {
+  SNICIT_REQUIRE(kcap % 8 == 0, "centroid capacity must be a multiple of 8");
+  const int nchunk = std::max(1, std::min(64, N / 64));
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)nchunk);
+  hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(256), 0, s, Y, ld, B, C, kcap, k, N, nchunk,
+                     eps, dist);
+  check_launch();
+  hipLaunchKernelGGL(lossy_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s, B,
+                     k, cent_col, dist, kcap, mcar, centmark, rec);
+  check_launch();
+}
+
+void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap,
+                        const int32_t* mcar, const int32_t* centmark, const int32_t* B,
+                        int64_t max_cols, int32_t N, float eps, bool positive_only, float* D,
+                        int64_t ldD, int32_t* flags, hipStream_t s) {
+  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
+  hipLaunchKernelGGL(lossy_delta_kernel, grid, dim3(256), 0, s, Y, ld, C, kcap, mcar, centmark, B,
+                     N, eps, positive_only ? 1 : 0, D, ldD, flags);
   check_launch();
 }
 

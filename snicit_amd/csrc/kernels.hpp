@@ -12,6 +12,7 @@ enum Epilogue : int {
   EPI_CENTROID = 1,     // zc = z + b stored; y = f(zc)
   EPI_DELTA = 2,        // y = f(zc[m] + z) - f(zc[m]); flag = any(y != 0)
   EPI_DELTA_LOSSY = 3,  // as DELTA, then |y| <= eps -> 0
+  EPI_LINEAR = 4,       // y = z + b (dense output layer, no activation)
 };
 
 struct SpmmLaunch {
@@ -45,6 +46,52 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
 // Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
 void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
                            int32_t kcap, const int32_t* k, hipStream_t s);
+
+// Wide ELL layer (rows with more than 32 nonzeros, or a dense layer when cols is null):
+// rows x K entries, ascending input per row, zero-padded; yin has any number of rows.
+struct EllLaunch {
+  const int32_t* cols;  // [rows][K] or null (column r = input r: dense layer)
+  const float* vals;    // [rows][K]
+  int32_t K, rows;
+  const float* bias_vec;
+  float bias_s, clip;
+  int epilogue;
+  const float* yin;
+  int64_t ldin;
+  float* yout;
+  int64_t ldout;
+  const int32_t* count;
+  const int32_t* map;
+  int32_t* flags;
+  float* zc;
+  int32_t kcap;
+  const int32_t* mcent;
+  float eps_post;
+  int64_t max_cols;
+};
+void launch_ell_spmm(const EllLaunch& a, hipStream_t s);
+// cat[j] = argmax_o logits[o][j] (first maximum, starting from -10000), j < *count
+void launch_argmax(const float* logits, int64_t ld, int32_t rows, const int32_t* count,
+                   int64_t max_cols, int32_t* cat, hipStream_t s);
+
+// ---- lossy clustering (src/beyond/SNICIT/kernel.hpp:5-101) ---------------------------
+// pool row y = column y*B/pool; rows gathered sample-major into poolrows[pool][N]
+void launch_lossy_pool(const float* Y, int64_t ld, const int32_t* B, int32_t pool, int32_t N,
+                       float* poolrows, int32_t* poolcol, hipStream_t s);
+void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const int32_t* B,
+                         int32_t pool, int32_t N, float eps, int32_t* cent_col, int32_t* k_out,
+                         hipStream_t s);
+// dist: [max_cols][kcap] int32, zero on entry and on exit; centmark[i] = 1 for centroids
+void launch_lossy_assign(const float* Y, int64_t ld, const int32_t* B, int64_t max_cols,
+                         const float* C, int32_t kcap, const int32_t* k,
+                         const int32_t* cent_col, int32_t N, float eps, int32_t* dist,
+                         int32_t* mcar, int32_t* centmark, int32_t* rec, hipStream_t s);
+// D[n*ldD + i] = |Y - C_m| > eps ? Y - C_m : 0 for non-centroid columns; flags = any(v != 0)
+// (or any(v > 0) with positive_only, the reference's test)
+void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap,
+                        const int32_t* mcar, const int32_t* centmark, const int32_t* B,
+                        int64_t max_cols, int32_t N, float eps, bool positive_only, float* D,
+                        int64_t ldD, int32_t* flags, hipStream_t s);
 
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,

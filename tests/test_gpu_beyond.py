@@ -1,0 +1,148 @@
+"""GPU parity of the unstructured-model path: wide rows (ELL layers), dense input and output
+layers with argmax categories, and lossy SNICIT, against the CPU oracle through the C-ABI.
+All FP32 results compare bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import snicit_amd as sa
+from tests import nets
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(net, cfg, x):
+    plan = sa.Plan(net, cfg, x.shape[0])
+    plan.set_input(x)
+    st = plan.run()
+    return plan, st
+
+
+def _model(Nin, N, L, K, Nout, seed):
+    cols, vals, w_in, w_out = sa.gen_unstructured(Nin, N, L, K, Nout, seed)
+    rng = np.random.default_rng(seed)
+    b_in = (rng.standard_normal(N) * 0.05).astype(np.float32)
+    b_out = (rng.standard_normal(Nout) * 0.1).astype(np.float32)
+    return dict(cols=cols, vals=vals, w_in=w_in, b_in=b_in, w_out=w_out, b_out=b_out)
+
+
+def _clustered(Nin, B, seed):
+    """Features drawn from 10 prototypes; half the samples get noise on ~5% of features, so
+    clustering finds tens of centroids and both empty and non-empty delta rows."""
+    r = np.random.default_rng(seed)
+    proto = sa.gen_features(Nin, 77, 0, 10)
+    x = proto[r.integers(0, 10, B)].copy()
+    m = (r.random((B, Nin)) < 0.05) & (r.random(B) < 0.5)[:, None]
+    x[m] += r.standard_normal(m.sum()).astype(np.float32) * 0.1
+    return np.clip(x, 0, 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("K,bias", [(64, -0.2), (100, -0.2), (48, "per_neuron")])
+def test_wide_rows_plain_bitexact(ctx, K, bias):
+    N, L, B = 1024, 6, 300
+    cols, vals = nets.random_ell(N, L, K=K, seed=K)
+    y0 = nets.random_inputs(N, B, seed=1, density=0.25)
+    clip = 32.0
+    if bias == "per_neuron":
+        bias = (np.random.default_rng(2).standard_normal((L, N)) * 0.05).astype(np.float32)
+        clip = 1.0
+    net = sa.Net.from_ell(ctx, cols, vals, bias, clip=clip)
+    yL, cat, _ = orc.plain_f32(cols, vals, bias, clip, y0)
+    assert 0 < cat.sum() < B or K == 48
+    for mode in ("BF", "PLAIN_DENSE"):
+        plan, _ = _run(net, sa.default_cfg(mode), y0)
+        np.testing.assert_array_equal(plan.activations(), yL)
+        np.testing.assert_array_equal(plan.categories(), cat)
+
+
+def test_wide_rows_exact_snicit(ctx):
+    N, L, B, t = 512, 10, 400, 3
+    cols, vals = nets.random_ell(N, L, K=80, seed=6)
+    y0 = nets.random_inputs(N, B, seed=4, density=0.2)
+    y0[50:90] = y0[1]
+    net = sa.Net.from_ell(ctx, cols, vals, -0.05)
+    plan, st = _run(net, sa.default_cfg("SNICIT", threshold=t), y0)
+    r = orc.snicit_exact(cols, vals, -0.05, 32.0, y0, t)
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+    np.testing.assert_array_equal(plan.activations(), r["yL"])
+    np.testing.assert_array_equal(plan.categories(), r["cat"])
+
+
+@pytest.mark.parametrize("mode", ["BF", "SNIG"])
+def test_model_io_plain_bitexact(ctx, mode):
+    """784 -> 1024 x 6 (103 nonzeros per row) -> 10 with argmax categories."""
+    Nin, N, L, K, Nout, B = 784, 1024, 6, 103, 10, 500
+    m = _model(Nin, N, L, K, Nout, seed=3)
+    x = sa.gen_features(Nin, 9, 0, B)
+    net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
+    net.set_io(m["w_in"], m["b_in"], 1.0, m["w_out"], m["b_out"])
+    plan, _ = _run(net, sa.default_cfg(mode), x)
+    yL, lg, cat = orc.model_plain(m["cols"], m["vals"], 0.0, 1.0, x, w_in=m["w_in"],
+                                  b_in=m["b_in"], in_clip=1.0, w_out=m["w_out"],
+                                  b_out=m["b_out"])
+    assert len(np.unique(cat)) > 3
+    np.testing.assert_array_equal(plan.activations(), yL)
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LOGITS).reshape(B, Nout), lg)
+    np.testing.assert_array_equal(plan.categories(), cat)
+
+
+@pytest.mark.parametrize("t,positive_only", [(0, 0), (2, 0), (2, 1), (5, 0)])
+def test_lossy_snicit_matches_oracle(ctx, t, positive_only):
+    Nin, N, L, K, Nout, B = 784, 1024, 6, 103, 10, 700
+    m = _model(Nin, N, L, K, Nout, seed=3)
+    x = _clustered(Nin, B, 1)
+    net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
+    net.set_io(m["w_in"], m["b_in"], 1.0, m["w_out"], m["b_out"])
+    cfg = sa.default_cfg("SNICIT", threshold=t, cluster=sa.CLUSTER_LOSSY,
+                         lossy_positive_only=positive_only)
+    plan, st = _run(net, cfg, x)
+    r = orc.model_snicit_lossy(m["cols"], m["vals"], 0.0, 1.0, x, t, w_in=m["w_in"],
+                               b_in=m["b_in"], in_clip=1.0, w_out=m["w_out"], b_out=m["b_out"],
+                               positive_only=bool(positive_only))
+    assert 10 < r["k"] < 128 and 0 < r["nerow_at_t"] < B
+    assert st.centroids == r["k"] and st.nerow_first == r["nerow_at_t"]
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_POOL), r["pool"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+    np.testing.assert_array_equal(plan.activations(), r["yL"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LOGITS).reshape(B, Nout),
+                                  r["logits"])
+    np.testing.assert_array_equal(plan.categories(), r["cat"])
+
+
+def test_lossy_without_io_layers(ctx):
+    """Lossy clustering on a hidden-only net whose per-neuron biases are partly positive."""
+    N, L, B, t = 512, 8, 400, 2
+    cols, vals = nets.random_ell(N, L, K=32, seed=12, share=4)
+    bias = (np.random.default_rng(5).standard_normal((L, N)) * 0.05).astype(np.float32)
+    y0 = nets.random_inputs(N, B, seed=3, density=0.2)
+    y0[200:260] = y0[0]
+    net = sa.Net.from_ell(ctx, cols, vals, bias, clip=1.0)
+    cfg = sa.default_cfg("SNICIT", threshold=t, cluster=sa.CLUSTER_LOSSY)
+    plan, st = _run(net, cfg, y0)
+    r = orc.model_snicit_lossy(cols, vals, bias, 1.0, y0, t)
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+    np.testing.assert_array_equal(plan.activations(), r["yL"])
+    np.testing.assert_array_equal(plan.categories(), r["cat"])
+
+
+def test_lossy_graph_replays_and_rerun(ctx):
+    Nin, N, L, K, Nout, B = 784, 1024, 6, 103, 10, 300
+    m = _model(Nin, N, L, K, Nout, seed=4)
+    net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
+    net.set_io(m["w_in"], m["b_in"], 1.0, m["w_out"], m["b_out"])
+    cfg = sa.default_cfg("SNICIT", threshold=2, cluster=sa.CLUSTER_LOSSY)
+    plan = sa.Plan(net, cfg, B)
+    for seed in (0, 1, 0):
+        x = _clustered(Nin, B, seed)
+        plan.set_input(x)
+        plan.run()
+        r = orc.model_snicit_lossy(m["cols"], m["vals"], 0.0, 1.0, x, 2, w_in=m["w_in"],
+                                   b_in=m["b_in"], w_out=m["w_out"], b_out=m["b_out"])
+        np.testing.assert_array_equal(plan.categories(), r["cat"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
