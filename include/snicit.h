@@ -180,6 +180,39 @@ int snicit_plan_profile(snicit_plan* plan, double* ms, int64_t* rows, double* ce
 int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
                  int64_t batch, int32_t* cat, snicit_stats* stats);
 
+/* ---- batches: many uneven batches over several nets in one graph launch ------------
+ * The reference runs one net over equal rounds of `batch` samples, one round after another
+ * (SNICIT.hpp:87-203, main/SDGC.cu:69-110).  A snicit_batch holds any number of batch
+ * segments of different sizes over several resident nets.  Segments of one net share that
+ * net's layer launches (their columns are concatenated, and the live rows and the non-empty
+ * delta rows are compacted across segments), while the threshold-layer clustering runs per
+ * segment: pool, centroids and nearest-centroid assignment only see the segment's own rows,
+ * so every segment's results equal a run of that batch alone.  Each net's layer loop runs on
+ * its own stream; the whole batch is captured once as one hipGraph and replayed per run. */
+typedef struct snicit_batch snicit_batch;
+typedef struct snicit_segment {
+  int32_t net;     /* index into the nets array of snicit_batch_create */
+  int32_t pad;
+  int64_t samples; /* samples in this segment (> 0) */
+} snicit_segment;
+
+/* cfgs[i] is the run configuration of nets[i] (mode, threshold, clustering). */
+int snicit_batch_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
+                        int32_t n_nets, const snicit_segment* segs, int32_t n_segs,
+                        snicit_batch** out);
+int snicit_batch_destroy(snicit_batch* batch);
+/* Host input of one segment: samples x (Nin or N) floats, sample-major. */
+int snicit_batch_set_input(snicit_batch* batch, int32_t seg, const float* y0);
+/* Device-rendered blob images of samples first_sample.. for one segment. */
+int snicit_batch_set_input_images(snicit_batch* batch, int32_t seg, uint64_t seed,
+                                  int64_t first_sample);
+/* Runs every segment (all inputs must be set).  ms = wall time of the graph launch on the
+ * device; per_net (nullable, n_nets entries) gets each net's counters (times zero). */
+int snicit_batch_run(snicit_batch* batch, double* ms, snicit_stats* per_net);
+int snicit_batch_categories(snicit_batch* batch, int32_t seg, int32_t* cat);
+/* Final hidden activations of one segment, samples x N. */
+int snicit_batch_activations(snicit_batch* batch, int32_t seg, float* y);
+
 /* ---- host-side seeded generators (splitmix64 streams) ---------------------------- */
 /* cols[(l*N+o)*32 + r] of the SDGC-like net (ascending per row); all values = weight. */
 int snicit_gen_sdgc_ell(int32_t N, int32_t L, uint64_t seed, int32_t* cols);

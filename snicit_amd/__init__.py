@@ -48,6 +48,10 @@ class RunCfg(ctypes.Structure):
     ]
 
 
+class Segment(ctypes.Structure):
+    _fields_ = [("net", ctypes.c_int32), ("pad", ctypes.c_int32), ("samples", ctypes.c_int64)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("ms_total", ctypes.c_double), ("ms_pre", ctypes.c_double),
@@ -100,6 +104,17 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_int64]),
     "snicit_infer": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
                                     ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
+    "snicit_batch_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
+                                           ctypes.c_int32, ctypes.POINTER(Segment), ctypes.c_int32,
+                                           ctypes.POINTER(_P)]),
+    "snicit_batch_destroy": (ctypes.c_int, [_P]),
+    "snicit_batch_set_input": (ctypes.c_int, [_P, ctypes.c_int32, _F32P]),
+    "snicit_batch_set_input_images": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64,
+                                                     ctypes.c_int64]),
+    "snicit_batch_run": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(Stats)]),
+    "snicit_batch_categories": (ctypes.c_int, [_P, ctypes.c_int32, _I32P]),
+    "snicit_batch_activations": (ctypes.c_int, [_P, ctypes.c_int32, _F32P]),
     "snicit_gen_sdgc_ell": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, _I32P]),
     "snicit_gen_images": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
                                          ctypes.c_int64, _F32P]),
@@ -364,6 +379,62 @@ class Model:
         self.net = Net.from_ell(ctx, cols, vals, bias, clip=clip)
         if w_in is not None or w_out is not None:
             self.net.set_io(w_in, b_in, in_clip, w_out, b_out)
+
+
+class Batch:
+    """Uneven batch segments over several nets, run as one captured graph.
+
+    ``nets`` and ``cfgs`` pair up; ``segments`` is a list of (net index, samples)."""
+
+    def __init__(self, ctx, nets, cfgs, segments):
+        self.nets = list(nets)
+        self.segments = [(int(n), int(b)) for n, b in segments]
+        hs = (_P * len(nets))(*[n.h for n in nets])
+        cf = (RunCfg * len(cfgs))(*cfgs)
+        sg = (Segment * len(segments))(*[Segment(n, 0, b) for n, b in self.segments])
+        h = _P()
+        _check(lib().snicit_batch_create(ctx.h, hs, cf, len(nets), sg, len(segments),
+                                         ctypes.byref(h)))
+        self.h = h
+
+    def set_input(self, seg, y0):
+        net, B = self.segments[seg]
+        y0 = np.ascontiguousarray(y0, dtype=np.float32)
+        n = self.nets[net]
+        assert y0.shape == (B, n.Nin or n.N)
+        _check(lib().snicit_batch_set_input(self.h, seg, _f32(y0)))
+
+    def set_input_images(self, seg, seed, first_sample):
+        _check(lib().snicit_batch_set_input_images(self.h, seg, seed, first_sample))
+
+    def run(self):
+        """Returns (ms, per-net Stats list)."""
+        ms = ctypes.c_double()
+        st = (Stats * len(self.nets))()
+        _check(lib().snicit_batch_run(self.h, ctypes.byref(ms), st))
+        return ms.value, list(st)
+
+    def categories(self, seg):
+        out = np.empty(self.segments[seg][1], np.int32)
+        _check(lib().snicit_batch_categories(self.h, seg, _i32(out)))
+        return out
+
+    def activations(self, seg):
+        net, B = self.segments[seg]
+        out = np.empty((B, self.nets[net].N), np.float32)
+        _check(lib().snicit_batch_activations(self.h, seg, _f32(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().snicit_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class _Method:

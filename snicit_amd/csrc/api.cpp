@@ -212,10 +212,18 @@ void destroy_net(Net* net) {
 struct snicit_plan {
   Net* net = nullptr;
   snicit_run_cfg cfg{};
+  hipStream_t stream = nullptr;  // the context's stream, or a batch's per-net stream
   int64_t cap = 0, ld = 0;
   int64_t B = 0;
   bool have_input = false, have_run = false;
-  int32_t kcap = kMaxCentroidsExact;
+  // batch segments: columns [segstart[s], segstart[s+1]) are clustered on their own; one
+  // segment covering every column for a plain plan
+  int32_t nseg = 1;
+  int32_t stride = kMaxCentroidsExact;  // centroid columns per segment
+  int32_t* d_segstart = nullptr;        // [nseg + 1]
+  int32_t* d_seglo = nullptr;           // [nseg + 1] segment bounds in the list at t
+  int32_t* d_kseg = nullptr;            // [nseg] centroids per segment
+  int32_t kcap = kMaxCentroidsExact;    // stride * nseg
   int32_t stop_layer = -1;  // layers actually run by the last run
   // activations
   float* d_in = nullptr;
@@ -228,13 +236,13 @@ struct snicit_plan {
   int32_t* d_mcar[2] = {nullptr, nullptr};
   int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
   int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
-  int32_t* d_k = nullptr;
+  int32_t* d_k = nullptr;  // centroid columns in use (k with one segment)
   float* d_F = nullptr;
   int32_t* d_cent_slot = nullptr;
   float* d_C[2] = {nullptr, nullptr};
   float* d_zc = nullptr;
   int32_t* d_rec = nullptr;
-  int32_t* d_dist = nullptr;  // [cap][32] distance accumulators (kept zero between runs)
+  int32_t* d_dist = nullptr;  // [cap][stride] distance accumulators (kept zero between runs)
   int32_t* d_catc = nullptr;
   int32_t* d_act = nullptr;
   int32_t* d_cat = nullptr;
@@ -249,7 +257,7 @@ struct snicit_plan {
   float* d_logits = nullptr;  // [Nout][ld]
   int64_t in_rows = 0;        // rows of d_in: N, or Nin padded to 16
   // lossy clustering
-  float* d_poolrows = nullptr;  // [pool][N]
+  float* d_poolrows = nullptr;  // [nseg * pool][N]
   int32_t* d_poolcol = nullptr;
   int32_t* d_centmark = nullptr;
   // where the last run left its final state
@@ -277,6 +285,10 @@ bool compacting(const snicit_plan* p) {
 
 // Input of hidden layer 0: the input-layer output when the net has one.
 const float* hidden_in(const snicit_plan* p) { return p->net->Nin > 0 ? p->d_h0 : p->d_in; }
+
+Segs plan_segs(snicit_plan* p) {
+  return Segs{p->nseg, p->stride, p->d_seglo, p->d_kseg, p->d_k};
+}
 
 // One hidden layer with the plan's conventions; panel or wide-ELL kernel by layer kind.
 void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
@@ -327,14 +339,14 @@ void enqueue_input_layer(snicit_plan* p) {
   e.ldout = p->ld;
   e.count = p->d_counts;
   e.max_cols = p->cap;
-  launch_ell_spmm(e, net.ctx->stream);
+  launch_ell_spmm(e, p->stream);
 }
 
 // Dense output layer on the hidden activations (all columns, sample order) and argmax
 // categories (src/beyond/general.hpp:74-128).
 void enqueue_output_layer(snicit_plan* p, const float* y) {
   const Net& net = *p->net;
-  hipStream_t s = net.ctx->stream;
+  hipStream_t s = p->stream;
   EllLaunch e{};
   e.cols = nullptr;
   e.vals = net.d_wout;
@@ -355,7 +367,7 @@ void enqueue_output_layer(snicit_plan* p, const float* y) {
 
 void prof_mark(snicit_plan* p, int l, int which) {
   if (!p->profiling) return;
-  SNICIT_HIP(hipEventRecord(p->prof_ev[(size_t)4 * l + which], p->net->ctx->stream));
+  SNICIT_HIP(hipEventRecord(p->prof_ev[(size_t)4 * l + which], p->stream));
 }
 
 SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
@@ -374,7 +386,7 @@ SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
 // list (BF's rowsY, src/SDGC/BF/BF.hpp:183-200).  Leaves the live list for layer l_end in
 // d_map / d_counts[l_end] / d_orig[return value].
 int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   const bool cmp = compacting(p);
   launch_iota_i32(p->d_orig[0], p->cap, s);
   enqueue_input_layer(p);
@@ -407,7 +419,7 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
 }
 
 void enqueue_plain_categories(snicit_plan* p, int l_end, int cur) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   if (p->net->Nout > 0) {
     enqueue_output_layer(p, l_end == 0 ? hidden_in(p) : p->d_y[(l_end - 1) & 1]);
     return;
@@ -452,18 +464,19 @@ int cluster_cur(const snicit_plan* p) {
 // pool of strided rows, pruning on whole rows, tolerance assignment and pruned deltas.
 // Every row is live (no dead-row skipping in this variant), so positions are sample ids.
 void enqueue_cluster_lossy(snicit_plan* p) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   const int t = p->cfg.threshold;
   const int N = p->net->N;
   SnicitState st = snicit_state(p);
   const snicit_run_cfg& c = p->cfg;
-  launch_lossy_pool(st.Yt, p->ld, p->d_counts, c.lossy_pool, N, p->d_poolrows, p->d_poolcol, s);
-  launch_lossy_select(p->d_poolrows, p->d_poolcol, p->d_counts, c.lossy_pool, N, c.lossy_eps_pool,
-                      p->d_cent_slot, p->d_k, s);
-  launch_gather_centroids(st.Yt, p->ld, nullptr, p->d_cent_slot, p->d_k, N, p->d_C[0], p->kcap, s);
-  launch_lossy_assign(st.Yt, p->ld, p->d_counts, p->cap, p->d_C[0], p->kcap, p->d_k,
-                      p->d_cent_slot, N, c.lossy_eps_assign, p->d_dist, p->d_mcar[0],
-                      p->d_centmark, p->d_rec, s);
+  const Segs sg = plan_segs(p);
+  launch_segment_bounds(p->d_orig[0], p->d_counts, p->d_segstart, sg, s);
+  launch_lossy_pool(st.Yt, p->ld, sg, c.lossy_pool, N, p->d_poolrows, p->d_poolcol, s);
+  launch_lossy_select(p->d_poolrows, p->d_poolcol, sg, c.lossy_pool, N, c.lossy_eps_pool,
+                      p->d_cent_slot, s);
+  launch_gather_centroids(st.Yt, p->ld, nullptr, p->d_cent_slot, sg, N, p->d_C[0], p->kcap, s);
+  launch_lossy_assign(st.Yt, p->ld, sg, p->cap, p->d_C[0], p->kcap, p->d_cent_slot, N,
+                      c.lossy_eps_assign, p->d_dist, p->d_mcar[0], p->d_centmark, p->d_rec, s);
   launch_lossy_delta(st.Yt, p->ld, p->d_C[0], p->kcap, p->d_mcar[0], p->d_centmark, p->d_counts,
                      p->cap, N, c.lossy_eps_assign, c.lossy_positive_only != 0, st.Da, p->ld,
                      p->d_flags, s);
@@ -477,7 +490,7 @@ void enqueue_cluster(snicit_plan* p) {
     enqueue_cluster_lossy(p);
     return;
   }
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   const int t = p->cfg.threshold;
   const int N = p->net->N;
   SnicitState st = snicit_state(p);
@@ -502,13 +515,14 @@ void enqueue_cluster(snicit_plan* p) {
     map = nullptr;
   }
   launch_fill_i32(p->d_rec, -1, p->cap, s);
-  launch_seg_sums(st.Yt, p->ld, map, count, p->cfg.sample_size, p->cfg.segments, N, p->d_F, s);
-  launch_select_exact(p->d_F, count, p->cfg.sample_size, p->cfg.segments, p->cfg.eps,
-                      p->d_cent_slot, p->d_k, s);
-  launch_gather_centroids(st.Yt, p->ld, map, p->d_cent_slot, p->d_k, N, p->d_C[0], p->kcap, s);
-  launch_assign_exact(st.Yt, p->ld, map, count, p->cap, p->d_C[0], p->kcap, p->d_k,
-                      p->d_cent_slot, N, p->d_dist, p->d_flags, p->d_mcar[0], p->d_orig[cur],
-                      p->d_rec, s);
+  const Segs sg = plan_segs(p);
+  launch_segment_bounds(p->d_orig[cur], count, p->d_segstart, sg, s);
+  launch_seg_sums(st.Yt, p->ld, map, sg, p->cfg.sample_size, p->cfg.segments, N, p->d_F, s);
+  launch_select_exact(p->d_F, sg, p->cfg.sample_size, p->cfg.segments, p->cfg.eps,
+                      p->d_cent_slot, s);
+  launch_gather_centroids(st.Yt, p->ld, map, p->d_cent_slot, sg, N, p->d_C[0], p->kcap, s);
+  launch_assign_exact(st.Yt, p->ld, map, sg, p->cap, p->d_C[0], p->kcap, p->d_cent_slot, N,
+                      p->d_dist, p->d_flags, p->d_mcar[0], p->d_orig[cur], p->d_rec, s);
   // Non-empty delta rows: list of (column in Yt, centroid, original id).
   const int32_t* ident = nullptr;
   launch_compact(p->d_flags, count, p->d_nerow + t, p->d_mapD, map ? map : ident, p->d_orig[cur],
@@ -520,7 +534,7 @@ void enqueue_cluster(snicit_plan* p) {
 // Stage "post" (src/SDGC/SNICIT/SNICIT.hpp:526-538,603-614): per layer the centroid rows,
 // then the delta rows with the fused residue epilogue, then the device-side recompaction.
 void enqueue_post(snicit_plan* p) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   const int t = p->cfg.threshold, L = p->net->L;
   SnicitState st = snicit_state(p);
   // clustering leaves the delta list's original ids in slot cur^1 and centroids in slot 1
@@ -572,8 +586,10 @@ void enqueue_post(snicit_plan* p) {
 }
 
 void enqueue_recover(snicit_plan* p) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   const int L = p->net->L, N = p->net->N;
+  SNICIT_REQUIRE(p->last_C != nullptr && p->last_D != nullptr,
+                 "internal: recovery enqueued before the final-state bookkeeping");
   if (p->net->Nout > 0) {
     // recovered hidden activations yhat_i + yhat_m(i) for every sample (the reference adds
     // the centroid back in place, src/beyond/SNICIT/kernel.hpp:170-183), then the output
@@ -594,7 +610,7 @@ void enqueue_recover(snicit_plan* p) {
 
 // Runs `body` directly or through a captured graph cached in slot `g`.
 void run_stage(snicit_plan* p, int g, const std::function<void()>& body) {
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   if (!p->cfg.use_graph) {
     body();
     return;
@@ -626,64 +642,98 @@ void drop_graphs(snicit_plan* p) {
     }
 }
 
+// Where a run leaves its final state; the stage bodies run at capture only, so this follows
+// from the configuration alone.  Must run before the stages are enqueued: enqueue_recover
+// and the activation readers take these pointers.
+void plan_bookkeeping(snicit_plan* p, int32_t stop) {
+  const int L = p->net->L;
+  if (plain_mode(p->cfg.mode)) {
+    const int l_end = stop < 0 ? L : stop;
+    p->last_orig = compacting(p) ? (l_end & 1) : 0;
+    p->last_Y = l_end == 0 ? p->d_in : p->d_y[(l_end - 1) & 1];
+    p->last_list = compacting(p) && l_end > 0;
+    p->stop_layer = l_end;
+  } else {
+    // each post layer flips every slot
+    const int t = p->cfg.threshold;
+    SnicitState st = snicit_state(p);
+    const int n_post = L - t;
+    p->last_orig = (cluster_cur(p) ^ 1) ^ (n_post & 1);
+    p->last_mcar = 1 ^ (n_post & 1);
+    p->last_C = p->d_C[n_post & 1];
+    p->last_D = (((n_post - 1) & 1) == 0) ? st.Db : st.Da;
+    p->stop_layer = L;
+  }
+}
+
+// Every stage of one run on the plan's stream, for capture into a caller's graph.
+void enqueue_all(snicit_plan* p) {
+  if (plain_mode(p->cfg.mode)) {
+    const int c = enqueue_plain_layers(p, p->net->L, true);
+    enqueue_plain_categories(p, p->net->L, c);
+  } else {
+    enqueue_plain_layers(p, p->cfg.threshold, false);
+    enqueue_cluster(p);
+    enqueue_post(p);
+    enqueue_recover(p);
+  }
+}
+
+void plan_stats(snicit_plan* p, snicit_stats* stats, const float* stage_ms);
+
 void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
   if (!p->have_input) throw Error(SNICIT_E_STATE, "plan has no input (call set_input first)");
   Net* net = p->net;
-  hipStream_t s = net->ctx->stream;
+  hipStream_t s = p->stream;
   const int L = net->L;
   if (p->graph_stop != stop) {
     drop_graphs(p);
     p->graph_stop = stop;
   }
-  const bool plain = plain_mode(p->cfg.mode);
+  // the recovery stage reads where the post stage leaves its state when it is enqueued
+  plan_bookkeeping(p, stop);
   SNICIT_HIP(hipEventRecord(p->ev[0], s));
-  if (plain) {
+  if (plain_mode(p->cfg.mode)) {
     const int l_end = stop < 0 ? L : stop;
     run_stage(p, 0, [&] {
       const int c = enqueue_plain_layers(p, l_end, true);
       enqueue_plain_categories(p, l_end, c);
     });
-    // the stage body runs at capture only; where the final state lives follows from l_end
-    const int cur = compacting(p) ? (l_end & 1) : 0;
-    p->last_orig = cur;
-    p->last_Y = l_end == 0 ? p->d_in : p->d_y[(l_end - 1) & 1];
-    p->last_list = compacting(p) && l_end > 0;
-    p->stop_layer = l_end;
     for (int i = 1; i < 5; ++i) SNICIT_HIP(hipEventRecord(p->ev[i], s));
   } else {
-    const int t = p->cfg.threshold;
-    run_stage(p, 0, [&] { enqueue_plain_layers(p, t, false); });
+    run_stage(p, 0, [&] { enqueue_plain_layers(p, p->cfg.threshold, false); });
     SNICIT_HIP(hipEventRecord(p->ev[1], s));
     run_stage(p, 1, [&] { enqueue_cluster(p); });
     SNICIT_HIP(hipEventRecord(p->ev[2], s));
     run_stage(p, 2, [&] { enqueue_post(p); });
-    {
-      // where the post stage leaves the final state (each post layer flips every slot)
-      SnicitState st = snicit_state(p);
-      const int n_post = L - t;
-      p->last_orig = (cluster_cur(p) ^ 1) ^ (n_post & 1);
-      p->last_mcar = 1 ^ (n_post & 1);
-      p->last_C = p->d_C[n_post & 1];
-      p->last_D = (((n_post - 1) & 1) == 0) ? st.Db : st.Da;
-    }
     SNICIT_HIP(hipEventRecord(p->ev[3], s));
     run_stage(p, 3, [&] { enqueue_recover(p); });
     SNICIT_HIP(hipEventRecord(p->ev[4], s));
-    p->stop_layer = L;
   }
   SNICIT_HIP(hipEventSynchronize(p->ev[4]));
   p->have_run = true;
   if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
     float ms[4];
     for (int i = 0; i < 4; ++i) SNICIT_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
-    stats->ms_pre = ms[0];
-    stats->ms_cluster = ms[1];
-    stats->ms_post = ms[2];
-    stats->ms_recover = ms[3];
-    float tot;
-    SNICIT_HIP(hipEventElapsedTime(&tot, p->ev[0], p->ev[4]));
-    stats->ms_total = tot;
+    plan_stats(p, stats, ms);
+  }
+}
+
+// Statistics of the last run from the device counters; stage_ms (nullable) = pre, cluster,
+// post, recover milliseconds.
+void plan_stats(snicit_plan* p, snicit_stats* stats, const float* stage_ms) {
+  Net* net = p->net;
+  const int L = net->L;
+  const bool plain = plain_mode(p->cfg.mode);
+  {
+    std::memset(stats, 0, sizeof(*stats));
+    if (stage_ms) {
+      stats->ms_pre = stage_ms[0];
+      stats->ms_cluster = stage_ms[1];
+      stats->ms_post = stage_ms[2];
+      stats->ms_recover = stage_ms[3];
+      stats->ms_total = stage_ms[0] + stage_ms[1] + stage_ms[2] + stage_ms[3];
+    }
     stats->batch = p->B;
     stats->layers = p->stop_layer;
     stats->edges = (double)p->B * ((double)net->nnz_hidden * (double)p->stop_layer +
@@ -697,9 +747,13 @@ void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
     } else {
       const int t = p->cfg.threshold;
       for (int l = 0; l < t; ++l) rows += compacting(p) ? counts[l] : counts[0];
-      int32_t k = 0;
+      int32_t k = 0;  // centroid columns the post layers process
       SNICIT_HIP(hipMemcpy(&k, p->d_k, 4, hipMemcpyDeviceToHost));
-      stats->centroids = k;
+      std::vector<int32_t> kseg(p->nseg);
+      SNICIT_HIP(hipMemcpy(kseg.data(), p->d_kseg, p->nseg * 4, hipMemcpyDeviceToHost));
+      int32_t ktot = 0;
+      for (int32_t v : kseg) ktot += v;
+      stats->centroids = ktot;
       stats->live_at_threshold = compacting(p) ? counts[t == 0 ? 1 : t] : counts[0];
       stats->nerow_first = nerow[t];
       stats->nerow_last = nerow[L];
@@ -725,7 +779,7 @@ void plan_free(snicit_plan* p) {
                   p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
                   p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
-                  p->d_centmark,
+                  p->d_centmark, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
                   p->d_stage};
   for (void* q : ptrs) dev_free(q);
@@ -913,10 +967,19 @@ int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,
   });
 }
 
-int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_batch,
-                       snicit_plan** out) {
-  return guarded([&] {
-    SNICIT_REQUIRE(net && cfg && out, "null argument");
+}  // extern "C"
+
+namespace snicit {
+// Plan of one net over max_batch columns split into batch segments at segstart (nseg + 1
+// entries, ascending; the last one bounds every column).  Runs on `stream`.
+snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_batch,
+                       const std::vector<int32_t>& segstart, hipStream_t stream) {
+  {
+    SNICIT_REQUIRE(net && cfg, "null argument");
+    const int32_t nseg = (int32_t)segstart.size() - 1;
+    SNICIT_REQUIRE(nseg >= 1, "a plan needs at least one batch segment");
+    for (int32_t i = 0; i < nseg; ++i)
+      SNICIT_REQUIRE(segstart[i] <= segstart[i + 1], "batch segments must be ascending");
     SNICIT_REQUIRE(max_batch > 0 && max_batch < (int64_t(1) << 31) - 1024,
                    "max_batch out of range");
     SNICIT_REQUIRE(cfg->mode >= SNICIT_MODE_SNICIT && cfg->mode <= SNICIT_MODE_PLAIN_DENSE,
@@ -944,10 +1007,13 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
     auto p = std::make_unique<snicit_plan>();
     p->net = net;
     p->cfg = *cfg;
+    p->stream = stream;
     p->cap = max_batch;
     p->ld = (max_batch + 63) / 64 * 64;
     const bool lossy = cfg->mode == SNICIT_MODE_SNICIT && cfg->cluster == SNICIT_CLUSTER_LOSSY;
-    p->kcap = lossy ? kMaxCentroidsLossy : kMaxCentroidsExact;
+    p->nseg = nseg;
+    p->stride = lossy ? kMaxCentroidsLossy : kMaxCentroidsExact;
+    p->kcap = p->stride * nseg;
     p->in_rows = net->Nin > 0 ? net->Kin : net->N;
     const int N = net->N, L = net->L;
     try {
@@ -958,10 +1024,16 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       if (net->has_io()) p->d_h0 = dev_alloc<float>((size_t)N * p->ld, acct);
       if (net->Nout > 0) p->d_logits = dev_alloc<float>((size_t)net->Nout * p->ld, acct);
       if (lossy) {
-        p->d_poolrows = dev_alloc<float>((size_t)kMaxCentroidsLossy * N, acct);
-        p->d_poolcol = dev_alloc<int32_t>(kMaxCentroidsLossy, acct);
+        p->d_poolrows = dev_alloc<float>((size_t)kMaxCentroidsLossy * nseg * N, acct);
+        p->d_poolcol = dev_alloc<int32_t>((size_t)kMaxCentroidsLossy * nseg, acct);
         p->d_centmark = dev_alloc<int32_t>(p->ld, acct);
       }
+      p->d_segstart = dev_alloc<int32_t>(nseg + 1, acct);
+      p->d_seglo = dev_alloc<int32_t>(nseg + 1, acct);
+      p->d_kseg = dev_alloc<int32_t>(nseg, acct);
+      SNICIT_HIP(hipMemcpy(p->d_segstart, segstart.data(), (nseg + 1) * 4,
+                           hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemset(p->d_kseg, 0, nseg * 4));
       p->d_y[0] = dev_alloc<float>((size_t)N * p->ld, acct);
       p->d_y[1] = dev_alloc<float>((size_t)N * p->ld, acct);
       p->d_flags = dev_alloc<int32_t>(p->ld, acct);
@@ -975,12 +1047,12 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       p->d_counts = dev_alloc<int32_t>(L + 4, acct);
       p->d_nerow = dev_alloc<int32_t>(L + 2, acct);
       p->d_k = dev_alloc<int32_t>(1, acct);
-      p->d_F = dev_alloc<float>(32 * 32, acct);
-      p->d_cent_slot = dev_alloc<int32_t>(kMaxCentroidsLossy, acct);
+      p->d_F = dev_alloc<float>((size_t)32 * 32 * nseg, acct);
+      p->d_cent_slot = dev_alloc<int32_t>(p->kcap, acct);
       p->d_zc = dev_alloc<float>((size_t)N * p->kcap, acct);
       p->d_rec = dev_alloc<int32_t>(p->ld, acct);
-      p->d_dist = dev_alloc<int32_t>((size_t)p->ld * p->kcap, acct);
-      SNICIT_HIP(hipMemset(p->d_dist, 0, (size_t)p->ld * p->kcap * 4));
+      p->d_dist = dev_alloc<int32_t>((size_t)p->ld * p->stride, acct);
+      SNICIT_HIP(hipMemset(p->d_dist, 0, (size_t)p->ld * p->stride * 4));
       p->d_catc = dev_alloc<int32_t>(p->kcap, acct);
       p->d_act = dev_alloc<int32_t>(p->ld, acct);
       p->d_cat = dev_alloc<int32_t>(p->ld, acct);
@@ -998,14 +1070,25 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       throw;
     }
     ++net->live_plans;
-    *out = p.release();
+    return p.release();
+  }
+}
+}  // namespace snicit
+
+extern "C" {
+
+int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_batch,
+                       snicit_plan** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net && cfg && out, "null argument");
+    *out = plan_make(net, cfg, max_batch, {0, INT32_MAX}, net->ctx->stream);
   });
 }
 
 int snicit_plan_destroy(snicit_plan* plan) {
   return guarded([&] {
     if (!plan) return;
-    (void)hipStreamSynchronize(plan->net->ctx->stream);
+    (void)hipStreamSynchronize(plan->stream);
     --plan->net->live_plans;
     plan_free(plan);
     delete plan;
@@ -1018,9 +1101,12 @@ void ensure_stage(snicit_plan* p) {
   const int64_t w = std::max<int64_t>(n.N, std::max<int64_t>(n.Nin, n.Nout));
   if (!p->d_stage) p->d_stage = dev_alloc<float>((size_t)p->cap * w, &p->device_bytes);
 }
+void load_host_input(snicit_plan* p, const float* y0, int64_t B, int64_t col_off);
+void load_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B,
+                 int64_t col_off);
 void set_count(snicit_plan* p, int64_t B) {
   int32_t b32 = (int32_t)B;
-  hipStream_t s = p->net->ctx->stream;
+  hipStream_t s = p->stream;
   SNICIT_HIP(hipMemcpyAsync(p->d_counts, &b32, 4, hipMemcpyHostToDevice, s));
   SNICIT_HIP(hipStreamSynchronize(s));
   p->B = B;
@@ -1033,59 +1119,74 @@ int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
   return guarded([&] {
     SNICIT_REQUIRE(p && (y0 || B == 0), "null argument");
     SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
-    // input rows: Nin features with an input layer, else N neurons
-    const int N = p->net->Nin > 0 ? p->net->Nin : p->net->N;
-    hipStream_t s = p->net->ctx->stream;
-    if (B > 0) {
-      ensure_stage(p);
-      SNICIT_HIP(hipMemcpyAsync(p->d_stage, y0, (size_t)B * N * 4, hipMemcpyHostToDevice, s));
-      launch_transpose_in(p->d_stage, B, N, p->d_in, p->ld, s);
-    }
+    SNICIT_REQUIRE(p->nseg == 1, "batch plans take their input per segment");
+    load_host_input(p, y0, B, 0);
     set_count(p, B);
   });
 }
 
+namespace {
+// Renders the blob images of samples first_sample .. first_sample+B-1 on the device into
+// columns col_off .. col_off+B-1 of the plan's input.
+void load_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B,
+                 int64_t col_off) {
+  SNICIT_REQUIRE(p->net->Nin == 0, "image inputs feed nets without an input layer");
+  if (B <= 0) return;
+  const int N = p->net->N;
+  hipStream_t s = p->stream;
+  ImageGeom g = image_geometry(N, seed);
+  std::vector<int32_t> params((size_t)B * 4);
+  for (int64_t b = 0; b < B; ++b) {
+    SampleParams sp = image_sample_params(g, seed, first_sample + b);
+    params[b * 4 + 0] = sp.cls;
+    params[b * 4 + 1] = sp.dx;
+    params[b * 4 + 2] = sp.dy;
+    params[b * 4 + 3] = sp.keep;
+  }
+  std::vector<int32_t> blobs(g.blobs.size() * 4), proto(20);
+  for (size_t i = 0; i < g.blobs.size(); ++i) {
+    blobs[i * 4 + 0] = g.blobs[i].cx;
+    blobs[i * 4 + 1] = g.blobs[i].cy;
+    blobs[i * 4 + 2] = g.blobs[i].r;
+    blobs[i * 4 + 3] = 0;
+  }
+  for (int c = 0; c < 10; ++c) {
+    proto[c] = g.first_blob[c];
+    proto[10 + c] = g.nproto_blobs[c];
+  }
+  SNICIT_REQUIRE(blobs.size() <= 64 * 4, "too many blobs");
+  // an earlier render may still read the parameter buffers
+  SNICIT_HIP(hipStreamSynchronize(s));
+  if (p->img_params_cap < B) {
+    dev_free(p->d_img_params);
+    p->d_img_params = dev_alloc<int32_t>((size_t)B * 4);
+    p->img_params_cap = B;
+  }
+  if (!p->d_img_blobs) p->d_img_blobs = dev_alloc<int32_t>(64 * 4);
+  if (!p->d_img_proto) p->d_img_proto = dev_alloc<int32_t>(20);
+  SNICIT_HIP(hipMemcpy(p->d_img_params, params.data(), params.size() * 4, hipMemcpyHostToDevice));
+  SNICIT_HIP(hipMemcpy(p->d_img_blobs, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice));
+  SNICIT_HIP(hipMemcpy(p->d_img_proto, proto.data(), proto.size() * 4, hipMemcpyHostToDevice));
+  launch_images(g, p->d_img_params, p->d_img_blobs, p->d_img_proto, B, N, p->d_in + col_off,
+                p->ld, s);
+}
+
+// Copies B host samples (B x width, sample-major) into columns col_off.. of the plan's input.
+void load_host_input(snicit_plan* p, const float* y0, int64_t B, int64_t col_off) {
+  if (B <= 0) return;
+  const int width = p->net->Nin > 0 ? p->net->Nin : p->net->N;
+  hipStream_t s = p->stream;
+  ensure_stage(p);
+  SNICIT_HIP(hipMemcpyAsync(p->d_stage, y0, (size_t)B * width * 4, hipMemcpyHostToDevice, s));
+  launch_transpose_in(p->d_stage, B, width, p->d_in + col_off, p->ld, s);
+}
+}  // namespace
+
 int snicit_plan_set_input_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B) {
   return guarded([&] {
     SNICIT_REQUIRE(p != nullptr, "null plan");
-    SNICIT_REQUIRE(p->net->Nin == 0, "image inputs feed nets without an input layer");
     SNICIT_REQUIRE(B >= 0 && B <= p->cap, "batch exceeds plan capacity");
-    const int N = p->net->N;
-    hipStream_t s = p->net->ctx->stream;
-    ImageGeom g = image_geometry(N, seed);
-    if (B > 0) {
-      std::vector<int32_t> params((size_t)B * 4);
-      for (int64_t b = 0; b < B; ++b) {
-        SampleParams sp = image_sample_params(g, seed, first_sample + b);
-        params[b * 4 + 0] = sp.cls;
-        params[b * 4 + 1] = sp.dx;
-        params[b * 4 + 2] = sp.dy;
-        params[b * 4 + 3] = sp.keep;
-      }
-      std::vector<int32_t> blobs(g.blobs.size() * 4), proto(20);
-      for (size_t i = 0; i < g.blobs.size(); ++i) {
-        blobs[i * 4 + 0] = g.blobs[i].cx;
-        blobs[i * 4 + 1] = g.blobs[i].cy;
-        blobs[i * 4 + 2] = g.blobs[i].r;
-        blobs[i * 4 + 3] = 0;
-      }
-      for (int c = 0; c < 10; ++c) {
-        proto[c] = g.first_blob[c];
-        proto[10 + c] = g.nproto_blobs[c];
-      }
-      if (p->img_params_cap < B) {
-        dev_free(p->d_img_params);
-        p->d_img_params = dev_alloc<int32_t>((size_t)B * 4);
-        p->img_params_cap = B;
-      }
-      if (!p->d_img_blobs) p->d_img_blobs = dev_alloc<int32_t>(64 * 4);
-      if (!p->d_img_proto) p->d_img_proto = dev_alloc<int32_t>(20);
-      SNICIT_REQUIRE(blobs.size() <= 64 * 4, "too many blobs");
-      SNICIT_HIP(hipMemcpy(p->d_img_params, params.data(), params.size() * 4, hipMemcpyHostToDevice));
-      SNICIT_HIP(hipMemcpy(p->d_img_blobs, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice));
-      SNICIT_HIP(hipMemcpy(p->d_img_proto, proto.data(), proto.size() * 4, hipMemcpyHostToDevice));
-      launch_images(g, p->d_img_params, p->d_img_blobs, p->d_img_proto, B, N, p->d_in, p->ld, s);
-    }
+    load_images(p, seed, first_sample, B, 0);
     set_count(p, B);
   });
 }
@@ -1124,7 +1225,7 @@ int snicit_plan_activations(snicit_plan* p, float* y) {
     ensure_stage(p);
     Net* net = p->net;
     const int N = net->N;
-    hipStream_t s = net->ctx->stream;
+    hipStream_t s = p->stream;
     // Assemble Y_final[n * ld + b] in original sample order inside d_stage's companion
     // scratch (the first plan buffer not holding final state), then transpose to [B][N].
     float* full = nullptr;
@@ -1255,7 +1356,7 @@ int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) 
         n = p->B * net.Nout;
         if (out && p->B) {
           ensure_stage(p);
-          hipStream_t s = net.ctx->stream;
+          hipStream_t s = p->stream;
           launch_transpose_out(p->d_logits, p->ld, p->B, net.Nout, p->d_stage, s);
           SNICIT_HIP(hipStreamSynchronize(s));
           std::vector<float> lg((size_t)n);
@@ -1345,6 +1446,209 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
       acc.nerow_last += st.nerow_last;
     }
     if (stats) *stats = acc;
+  });
+}
+
+// ---- batches ------------------------------------------------------------------------
+struct snicit_batch {
+  Ctx* ctx = nullptr;
+  std::vector<snicit_plan*> plans;   // one per net
+  std::vector<hipStream_t> streams;  // streams[0] is the context's (not owned)
+  struct Seg {
+    int32_t net;
+    int64_t off, samples;
+    bool loaded;
+  };
+  std::vector<Seg> segs;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev[2] = {};
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  bool have_run = false;
+};
+
+namespace {
+void batch_free(snicit_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (snicit_plan* p : b->plans) {
+    if (!p) continue;
+    (void)hipStreamSynchronize(p->stream);
+    --p->net->live_plans;
+    plan_free(p);
+    delete p;
+  }
+  for (size_t i = 1; i < b->streams.size(); ++i)
+    if (b->streams[i]) (void)hipStreamDestroy(b->streams[i]);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (hipEvent_t e : b->join)
+    if (e) (void)hipEventDestroy(e);
+}
+
+snicit_batch::Seg& batch_seg(snicit_batch* b, int32_t seg) {
+  SNICIT_REQUIRE(b != nullptr, "null batch");
+  SNICIT_REQUIRE(seg >= 0 && seg < (int32_t)b->segs.size(), "segment index out of range");
+  return b->segs[seg];
+}
+
+// One graph over every net: a fork from the context's stream to one stream per net, each
+// net's whole layer loop, and a join back.
+void batch_capture(snicit_batch* b) {
+  hipStream_t s0 = b->streams[0];
+  hipGraph_t graph = nullptr;
+  SNICIT_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  try {
+    SNICIT_HIP(hipEventRecord(b->fork, s0));
+    for (size_t i = 1; i < b->plans.size(); ++i)
+      SNICIT_HIP(hipStreamWaitEvent(b->streams[i], b->fork, 0));
+    for (snicit_plan* p : b->plans) enqueue_all(p);
+    for (size_t i = 1; i < b->plans.size(); ++i) {
+      SNICIT_HIP(hipEventRecord(b->join[i], b->streams[i]));
+      SNICIT_HIP(hipStreamWaitEvent(s0, b->join[i], 0));
+    }
+  } catch (...) {
+    hipGraph_t junk = nullptr;
+    (void)hipStreamEndCapture(s0, &junk);
+    if (junk) (void)hipGraphDestroy(junk);
+    throw;
+  }
+  SNICIT_HIP(hipStreamEndCapture(s0, &graph));
+  hipError_t e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  SNICIT_HIP(e);
+}
+}  // namespace
+
+int snicit_batch_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
+                        int32_t n_nets, const snicit_segment* segs, int32_t n_segs,
+                        snicit_batch** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(ctx && nets && cfgs && segs && out, "null argument");
+    SNICIT_REQUIRE(n_nets >= 1 && n_segs >= 1, "a batch needs nets and segments");
+    SNICIT_HIP(hipSetDevice(ctx->device));
+    auto b = std::make_unique<snicit_batch>();
+    b->ctx = ctx;
+    std::vector<std::vector<int32_t>> starts(n_nets);
+    std::vector<int64_t> total(n_nets, 0);
+    for (int32_t i = 0; i < n_segs; ++i) {
+      const int32_t n = segs[i].net;
+      SNICIT_REQUIRE(n >= 0 && n < n_nets, "segment " + std::to_string(i) + " names no net");
+      SNICIT_REQUIRE(segs[i].samples > 0, "segment " + std::to_string(i) + " is empty");
+      starts[n].push_back((int32_t)total[n]);
+      b->segs.push_back({n, total[n], segs[i].samples, false});
+      total[n] += segs[i].samples;
+      SNICIT_REQUIRE(total[n] < (int64_t(1) << 31) - 1024, "too many samples for one net");
+    }
+    try {
+      b->streams.push_back(ctx->stream);
+      b->join.assign(n_nets, nullptr);
+      for (int i = 0; i < 2; ++i) SNICIT_HIP(hipEventCreateWithFlags(&b->ev[i], 0));
+      SNICIT_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+      for (int32_t n = 0; n < n_nets; ++n) {
+        SNICIT_REQUIRE(nets[n] && nets[n]->ctx == ctx, "nets must belong to the batch's context");
+        SNICIT_REQUIRE(total[n] > 0, "net " + std::to_string(n) + " has no segment");
+        if (n > 0) {
+          hipStream_t st = nullptr;
+          SNICIT_HIP(hipStreamCreate(&st));
+          b->streams.push_back(st);
+          SNICIT_HIP(hipEventCreateWithFlags(&b->join[n], hipEventDisableTiming));
+        }
+        std::vector<int32_t> st = starts[n];
+        st.push_back((int32_t)total[n]);
+        b->plans.push_back(plan_make(nets[n], &cfgs[n], total[n], st, b->streams[n]));
+        set_count(b->plans.back(), total[n]);
+        b->plans.back()->have_input = false;
+      }
+    } catch (...) {
+      batch_free(b.get());
+      throw;
+    }
+    *out = b.release();
+  });
+}
+
+int snicit_batch_destroy(snicit_batch* batch) {
+  return guarded([&] {
+    if (!batch) return;
+    batch_free(batch);
+    delete batch;
+  });
+}
+
+int snicit_batch_set_input(snicit_batch* batch, int32_t seg, const float* y0) {
+  return guarded([&] {
+    snicit_batch::Seg& sg = batch_seg(batch, seg);
+    SNICIT_REQUIRE(y0 != nullptr, "null input");
+    SNICIT_HIP(hipSetDevice(batch->ctx->device));
+    snicit_plan* p = batch->plans[sg.net];
+    load_host_input(p, y0, sg.samples, sg.off);
+    SNICIT_HIP(hipStreamSynchronize(p->stream));
+    sg.loaded = true;
+  });
+}
+
+int snicit_batch_set_input_images(snicit_batch* batch, int32_t seg, uint64_t seed,
+                                  int64_t first_sample) {
+  return guarded([&] {
+    snicit_batch::Seg& sg = batch_seg(batch, seg);
+    SNICIT_HIP(hipSetDevice(batch->ctx->device));
+    snicit_plan* p = batch->plans[sg.net];
+    load_images(p, seed, first_sample, sg.samples, sg.off);
+    SNICIT_HIP(hipStreamSynchronize(p->stream));
+    sg.loaded = true;
+  });
+}
+
+int snicit_batch_run(snicit_batch* batch, double* ms, snicit_stats* per_net) {
+  return guarded([&] {
+    SNICIT_REQUIRE(batch != nullptr, "null batch");
+    for (const auto& sg : batch->segs)
+      if (!sg.loaded) throw Error(SNICIT_E_STATE, "every segment needs its input before a run");
+    SNICIT_HIP(hipSetDevice(batch->ctx->device));
+    hipStream_t s0 = batch->streams[0];
+    for (snicit_plan* p : batch->plans) SNICIT_HIP(hipStreamSynchronize(p->stream));
+    // before the capture: the recovery stage reads where the post stage leaves its state
+    for (snicit_plan* p : batch->plans) plan_bookkeeping(p, -1);
+    if (!batch->graph) batch_capture(batch);
+    SNICIT_HIP(hipEventRecord(batch->ev[0], s0));
+    SNICIT_HIP(hipGraphLaunch(batch->graph, s0));
+    SNICIT_HIP(hipEventRecord(batch->ev[1], s0));
+    SNICIT_HIP(hipEventSynchronize(batch->ev[1]));
+    float t = 0.f;
+    SNICIT_HIP(hipEventElapsedTime(&t, batch->ev[0], batch->ev[1]));
+    if (ms) *ms = t;
+    for (size_t i = 0; i < batch->plans.size(); ++i) {
+      snicit_plan* p = batch->plans[i];
+      p->have_input = p->have_run = true;
+      if (per_net) plan_stats(p, &per_net[i], nullptr);
+    }
+    batch->have_run = true;
+  });
+}
+
+int snicit_batch_categories(snicit_batch* batch, int32_t seg, int32_t* cat) {
+  return guarded([&] {
+    snicit_batch::Seg& sg = batch_seg(batch, seg);
+    SNICIT_REQUIRE(cat != nullptr, "null output");
+    if (!batch->have_run) throw Error(SNICIT_E_STATE, "no completed run");
+    SNICIT_HIP(hipSetDevice(batch->ctx->device));
+    const snicit_plan* p = batch->plans[sg.net];
+    SNICIT_HIP(hipMemcpy(cat, p->d_cat + sg.off, sg.samples * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int snicit_batch_activations(snicit_batch* batch, int32_t seg, float* y) {
+  return guarded([&] {
+    snicit_batch::Seg& sg = batch_seg(batch, seg);
+    SNICIT_REQUIRE(y != nullptr, "null output");
+    if (!batch->have_run) throw Error(SNICIT_E_STATE, "no completed run");
+    snicit_plan* p = batch->plans[sg.net];
+    const int64_t N = p->net->N;
+    std::vector<float> all((size_t)p->B * N);
+    const int rc = snicit_plan_activations(p, all.data());
+    if (rc != SNICIT_OK) throw Error(rc, snicit_last_error());
+    std::memcpy(y, all.data() + (size_t)sg.off * N, (size_t)sg.samples * N * 4);
   });
 }
 

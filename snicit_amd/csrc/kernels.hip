@@ -492,41 +492,86 @@ __global__ __launch_bounds__(256) void argmax_kernel(const float* __restrict__ l
   cat[j] = arg;
 }
 
+// ---- batch segments ---------------------------------------------------------------------
+// A plan's columns can hold several independent batches ("segments"), each clustered on its
+// own.  segstart[s] is the first original column of segment s; the live list keeps original
+// columns ascending, so segment s occupies list positions [seglo[s], seglo[s+1]).  A plain
+// plan is the one-segment case (segstart = {0, INT_MAX}).
+__global__ void segment_bounds_kernel(const int32_t* __restrict__ orig,
+                                      const int32_t* __restrict__ count_ptr,
+                                      const int32_t* __restrict__ segstart, int32_t nseg,
+                                      int32_t* __restrict__ seglo) {
+  const int count = *count_ptr;
+  for (int s = threadIdx.x; s <= nseg; s += blockDim.x) {
+    if (s == nseg) {
+      seglo[s] = count;
+      continue;
+    }
+    const int32_t key = segstart[s];
+    int lo = 0, hi = count;  // first position with orig >= key
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (orig[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    seglo[s] = lo;
+  }
+}
+
+// Segment of list position i: the last s with seglo[s] <= i (empty segments skipped).
+__device__ __forceinline__ int segment_of(const int32_t* __restrict__ seglo, int nseg,
+                                          int64_t i) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seglo[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---- lossy clustering ------------------------------------------------------------------
-// Pool (y_star_gen, src/beyond/SNICIT/kernel.hpp:12): pool row y is sample y*B/pool.
+// Pool (y_star_gen, src/beyond/SNICIT/kernel.hpp:12): pool row y of a batch segment with
+// B_s rows is its row y*B_s/pool.  Every row is live in this variant, so list positions are
+// columns.  Block (pool row, batch segment).
 __global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict__ Y, int64_t ld,
-                                                          const int32_t* __restrict__ B_ptr,
+                                                          const int32_t* __restrict__ seglo,
                                                           int32_t pool, int32_t N,
                                                           float* __restrict__ poolrows,
                                                           int32_t* __restrict__ poolcol) {
-  const int64_t B = *B_ptr;
-  const int y = blockIdx.x;
-  const int64_t col = B > 0 ? (int64_t)y * B / pool : 0;
-  if (threadIdx.x == 0) poolcol[y] = (int32_t)col;
+  const int y = blockIdx.x, sg = blockIdx.y;
+  const int64_t lo = seglo[sg], Bs = seglo[sg + 1] - lo;
+  const int64_t col = Bs > 0 ? lo + (int64_t)y * Bs / pool : -1;
+  const int64_t row = (int64_t)sg * pool + y;
+  if (threadIdx.x == 0) poolcol[row] = (int32_t)col;
   for (int n = threadIdx.x; n < N; n += blockDim.x)
-    poolrows[(int64_t)y * N + n] = B > 0 ? Y[(int64_t)n * ld + col] : 0.f;
+    poolrows[row * N + n] = col >= 0 ? Y[(int64_t)n * ld + col] : 0.f;
 }
 
 // Pruning (y_star_gen, src/beyond/SNICIT/kernel.hpp:19-41): in pool order, every present
 // base i removes each other present row y with #{n : |P_y[n] - P_i[n]| > eps} < N*eps.
-// One workgroup; wavefront w tests rows w, w+16, ...; lanes stride the neurons.
+// One workgroup per batch segment; wavefront w tests rows w, w+16, ...; lanes stride the
+// neurons.  Centroids (columns) go to cent_col[s*stride ...]; the last segment's block also
+// writes the number of centroid columns in use.
 __global__ __launch_bounds__(1024) void lossy_select_kernel(
     const float* __restrict__ poolrows, const int32_t* __restrict__ poolcol,
-    const int32_t* __restrict__ B_ptr, int32_t pool, int32_t N, float eps,
-    int32_t* __restrict__ cent_col, int32_t* __restrict__ k_out) {
+    const int32_t* __restrict__ seglo, int32_t nseg, int32_t pool, int32_t N, float eps,
+    int32_t stride, int32_t* __restrict__ cent_col, int32_t* __restrict__ kseg,
+    int32_t* __restrict__ kcols) {
   __shared__ int32_t alive[kMaxCentroidsLossy];
-  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
-  const int64_t B = *B_ptr;
-  const int P = B > 0 ? pool : 0;
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave, sg = blockIdx.x;
+  const int64_t Bs = seglo[sg + 1] - seglo[sg];
+  const int P = Bs > 0 ? pool : 0;
+  const float* __restrict__ rows = poolrows + (int64_t)sg * pool * N;
   for (int y = t; y < kMaxCentroidsLossy; y += blockDim.x) alive[y] = y < P;
   __syncthreads();
   const double limit = (double)N * (double)eps;
   for (int i = 0; i < P; ++i) {
     if (!alive[i]) continue;  // uniform: read after the barrier below
-    const float* __restrict__ pi = poolrows + (int64_t)i * N;
+    const float* __restrict__ pi = rows + (int64_t)i * N;
     for (int y = wv; y < P; y += 16) {
       if (y == i || !alive[y]) continue;
-      const float* __restrict__ py = poolrows + (int64_t)y * N;
+      const float* __restrict__ py = rows + (int64_t)y * N;
       int cnt = 0;
       for (int n0 = 0; n0 < N; n0 += kWave) {
         const int n = n0 + lane;
@@ -540,49 +585,63 @@ __global__ __launch_bounds__(1024) void lossy_select_kernel(
   if (t == 0) {
     int k = 0;
     for (int y = 0; y < P; ++y)
-      if (alive[y]) cent_col[k++] = poolcol[y];
-    *k_out = k;
+      if (alive[y]) cent_col[sg * stride + k++] = poolcol[sg * pool + y];
+    kseg[sg] = k;
+    if (sg == nseg - 1) *kcols = sg * stride + k;
   }
 }
 
 // Assignment (coarse_cluster, src/beyond/SNICIT/kernel.hpp:63-87): distance = #{n :
-// |C_c[n] - Y_i[n]| > eps}, first minimum; centroid columns map to their own index.
+// |C_c[n] - Y_i[n]| > eps} over the centroids of i's batch segment, first minimum; centroid
+// columns map to their own index.  Same wavefront walk over segments as the exact variant.
 __global__ __launch_bounds__(256) void lossy_dist_kernel(
-    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ B_ptr,
-    const float* __restrict__ C, int32_t kcap, const int32_t* __restrict__ k_ptr, int32_t N,
-    int32_t nchunk, float eps, int32_t* __restrict__ dist) {
-  const int64_t count = *B_ptr;
-  const int k = *k_ptr;
+    const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ seglo, int32_t nseg,
+    const float* __restrict__ C, int32_t kcap, int32_t stride, const int32_t* __restrict__ kseg,
+    int32_t N, int32_t nchunk, float eps, int32_t* __restrict__ dist) {
+  const int64_t count = seglo[nseg];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
+  const int sg = nseg == 1 ? 0 : segment_of(seglo, nseg, i);
   const int n0 = (int)((int64_t)N * blockIdx.y / nchunk);
   const int n1 = (int)((int64_t)N * (blockIdx.y + 1) / nchunk);
-  for (int cb = 0; cb < k; cb += 8) {
-    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int n = n0; n < n1; ++n) {
-      const float y = Y[(int64_t)n * ld + i];
-      const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
+  uint64_t todo = __ballot(true);
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int s0 = __builtin_amdgcn_readfirstlane(__shfl(sg, lead));
+    const bool mine = sg == s0;
+    todo &= ~__ballot(mine);
+    const int k = kseg[s0];
+    const float* __restrict__ Cs = C + (int64_t)s0 * stride;
+    for (int cb = 0; cb < k; cb += 8) {
+      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int n = n0; n < n1; ++n) {
+        const float y = Y[(int64_t)n * ld + i];
+        const float* __restrict__ cr = Cs + (int64_t)n * kcap + cb;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) cnt[q] += fabsf(cr[q] - y) > eps;
+        for (int q = 0; q < 8; ++q) cnt[q] += fabsf(cr[q] - y) > eps;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (mine && cb + q < k && cnt[q])
+          atomicAdd(&dist[i * kMaxCentroidsLossy + cb + q], cnt[q]);
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (cb + q < k && cnt[q]) atomicAdd(&dist[i * kcap + cb + q], cnt[q]);
   }
 }
 
 __global__ __launch_bounds__(256) void lossy_final_kernel(
-    const int32_t* __restrict__ B_ptr, const int32_t* __restrict__ k_ptr,
-    const int32_t* __restrict__ cent_col, int32_t* __restrict__ dist, int32_t kcap,
+    const int32_t* __restrict__ seglo, int32_t nseg, const int32_t* __restrict__ kseg,
+    int32_t stride, const int32_t* __restrict__ cent_col, int32_t* __restrict__ dist,
     int32_t* __restrict__ mcar, int32_t* __restrict__ centmark, int32_t* __restrict__ rec) {
-  const int64_t count = *B_ptr;
-  const int k = *k_ptr;
+  const int64_t count = seglo[nseg];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
+  const int sg = segment_of(seglo, nseg, i);
+  const int k = kseg[sg];
+  const int32_t* __restrict__ cs = cent_col + sg * stride;
   int best = 0x7fffffff, bestc = 0;
   for (int c = 0; c < k; ++c) {
-    const int d = dist[i * kcap + c];
-    dist[i * kcap + c] = 0;
+    const int d = dist[i * kMaxCentroidsLossy + c];
+    dist[i * kMaxCentroidsLossy + c] = 0;
     if (d < best) {
       best = d;
       bestc = c;
@@ -590,13 +649,14 @@ __global__ __launch_bounds__(256) void lossy_final_kernel(
   }
   int mark = 0;
   for (int c = 0; c < k; ++c)
-    if (cent_col[c] == i && !mark) {
+    if (cs[c] == i && !mark) {
       mark = 1;
       bestc = c;
     }
-  mcar[i] = bestc;
+  const int32_t m = sg * stride + bestc;
+  mcar[i] = m;
   centmark[i] = mark;
-  rec[i] = k > 0 ? bestc : -1;
+  rec[i] = k > 0 ? m : -1;
 }
 
 // Delta (coarse_cluster, src/beyond/SNICIT/kernel.hpp:89-99): v = |Y_i - C_m| > eps ?
@@ -815,26 +875,27 @@ __global__ void cat_from_flags_kernel(int32_t* __restrict__ flags,
 }
 
 // ---- exact clustering ----------------------------------------------------------------
-// Pool row s (the s-th live row, s < min(pool, live)) gets `segments` sums over consecutive
-// blocks of N/segments neurons (the intended segments of src/SDGC/SNICIT/kernel.hpp:138-153,
-// which accumulates them with shared-memory atomics in no fixed order).  Fixed order here:
-// each segment splits into S2 = min(32, seglen) contiguous sub-chunks [c*seglen/S2,
-// (c+1)*seglen/S2), each summed in ascending neuron order, then the S2 partial sums are
-// added in ascending c.  One block per pool row, thread (g, c).
+// Pool row r of segment s (its r-th live row, r < min(pool, live in s)) gets `segments`
+// sums over consecutive blocks of N/segments neurons (the intended segments of
+// src/SDGC/SNICIT/kernel.hpp:138-153, which accumulates them with shared-memory atomics in no
+// fixed order).  Fixed order here: each segment splits into S2 = min(32, seglen) contiguous
+// sub-chunks [c*seglen/S2, (c+1)*seglen/S2), each summed in ascending neuron order, then the
+// S2 partial sums are added in ascending c.  Block (pool row, batch segment), thread (g, c).
 __global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict__ Y, int64_t ld,
                                                          const int32_t* __restrict__ map,
-                                                         const int32_t* __restrict__ count_ptr,
+                                                         const int32_t* __restrict__ seglo,
                                                          int32_t pool, int32_t segments,
                                                          int32_t N, float* __restrict__ F) {
   __shared__ float part[32 * 32];
-  const int s = blockIdx.x;
-  const int count = *count_ptr;
-  const int P = min(pool, count);
+  const int r = blockIdx.x, sg = blockIdx.y;
+  const int lo = seglo[sg];
+  const int P = min(pool, seglo[sg + 1] - lo);
   const int seglen = N / segments;
   const int S2 = min(32, seglen);
   const int g = threadIdx.x / 32, c = threadIdx.x % 32;
-  if (s < P && g < segments && c < S2) {
-    const int64_t col = map ? (int64_t)map[s] : s;
+  if (r < P && g < segments && c < S2) {
+    const int64_t pos = lo + r;
+    const int64_t col = map ? (int64_t)map[pos] : pos;
     const int n0 = g * seglen + c * seglen / S2, n1 = g * seglen + (c + 1) * seglen / S2;
     const float* __restrict__ y = Y + col;
     float sum = 0.f;
@@ -852,24 +913,30 @@ __global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict_
   __syncthreads();
   if (threadIdx.x < segments) {
     float sum = 0.f;
-    if (s < P)
+    if (r < P)
       for (int q = 0; q < S2; ++q) sum += part[threadIdx.x * 32 + q];
-    F[s * segments + threadIdx.x] = sum;
+    F[((int64_t)sg * pool + r) * segments + threadIdx.x] = sum;
   }
 }
 
 // Alg. 1 sample pruning (src/SDGC/SNICIT/kernel.hpp:156-194): in pool order, every still
 // present base c removes each other present row y whose segment sums all lie within eps
 // of c's (count of |F_y - F_c| > eps below segments*eps).  Survivors are the centroids,
-// listed in pool order.  One wavefront, lane = pool row.
+// listed in pool order as list positions in cent_slot[s*stride ...].  One wavefront per
+// batch segment, lane = pool row.  The last segment's block also writes the number of
+// centroid columns in use, (nseg-1)*stride + k_last.
 __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restrict__ F,
-                                                           const int32_t* __restrict__ count_ptr,
-                                                           int32_t pool, int32_t segments,
-                                                           float eps,
+                                                           const int32_t* __restrict__ seglo,
+                                                           int32_t nseg, int32_t pool,
+                                                           int32_t segments, float eps,
+                                                           int32_t stride,
                                                            int32_t* __restrict__ cent_slot,
-                                                           int32_t* __restrict__ k_out) {
-  const int lane = threadIdx.x;
-  const int P = min(pool, *count_ptr);
+                                                           int32_t* __restrict__ kseg,
+                                                           int32_t* __restrict__ kcols) {
+  const int lane = threadIdx.x, sg = blockIdx.x;
+  const int lo = seglo[sg];
+  const int P = min(pool, seglo[sg + 1] - lo);
+  const float* __restrict__ Fs = F + (int64_t)sg * pool * segments;
   bool alive = lane < P;
   for (int c = 0; c < P; ++c) {
     const uint64_t am = __ballot(alive);
@@ -877,26 +944,29 @@ __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restric
     if (alive && lane != c) {
       int cnt = 0;
       for (int g = 0; g < segments; ++g)
-        cnt += fabsf(F[lane * segments + g] - F[c * segments + g]) > eps;
+        cnt += fabsf(Fs[lane * segments + g] - Fs[c * segments + g]) > eps;
       if ((double)cnt < (double)segments * (double)eps) alive = false;
     }
   }
   const uint64_t am = __ballot(alive);
-  if (alive) cent_slot[lane_prefix(am)] = lane;
-  if (lane == 0) *k_out = __popcll(am);
+  if (alive) cent_slot[sg * stride + lane_prefix(am)] = lo + lane;
+  if (lane == 0) {
+    kseg[sg] = __popcll(am);
+    if (sg == nseg - 1) *kcols = sg * stride + __popcll(am);
+  }
 }
 
+// Centroid column c belongs to segment c / stride and is in use when c % stride < kseg.
 __global__ __launch_bounds__(256) void gather_centroids_kernel(
     const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ cent_slot, const int32_t* __restrict__ k_ptr, int32_t N,
-    float* __restrict__ C, int32_t kcap) {
-  const int k = *k_ptr;
+    const int32_t* __restrict__ cent_slot, const int32_t* __restrict__ kseg, int32_t stride,
+    int32_t N, float* __restrict__ C, int32_t kcap) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)N * kcap) return;
   const int64_t n = i / kcap;
   const int c = (int)(i % kcap);
   float v = 0.f;
-  if (c < k) {
+  if (c % stride < kseg[c / stride]) {
     const int64_t slot = cent_slot[c];
     const int64_t col = map ? (int64_t)map[slot] : slot;
     v = Y[n * ld + col];
@@ -904,46 +974,60 @@ __global__ __launch_bounds__(256) void gather_centroids_kernel(
   C[i] = v;
 }
 
-// Nearest centroid (src/SDGC/SNICIT/kernel.hpp:8-69): m(i) = argmin_c #{n : Y_i[n] != Y_c[n]},
-// first minimum in centroid order; centroids map to themselves.  flag = non-centroid with a
-// nonzero distance (its delta row is non-empty).  The distances are split over blocks of
-// neurons (grid y) and summed with integer atomics into dist[i][32]; centroid values are
-// uniform across the wavefront (scalar loads); eight running counts per pass.
+// Nearest centroid (src/SDGC/SNICIT/kernel.hpp:8-69): m(i) = argmin_c #{n : Y_i[n] != Y_c[n]}
+// over the centroids of i's batch segment, first minimum in centroid order; centroids map to
+// themselves.  flag = non-centroid with a nonzero distance (its delta row is non-empty).
+// The distances are split over blocks of neurons (grid y) and summed with integer atomics
+// into dist[i][local centroid].  A wavefront walks the segments its lanes cover one at a time
+// (almost always one), so the centroid values stay uniform (scalar loads); eight running
+// counts per pass.
 __global__ __launch_bounds__(256) void assign_dist_kernel(
     const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap,
-    const int32_t* __restrict__ k_ptr, int32_t N, int32_t nchunk, int32_t* __restrict__ dist) {
-  const int64_t count = *count_ptr;
-  const int k = *k_ptr;
+    const int32_t* __restrict__ seglo, int32_t nseg, const float* __restrict__ C, int32_t kcap,
+    int32_t stride, const int32_t* __restrict__ kseg, int32_t N, int32_t nchunk,
+    int32_t* __restrict__ dist) {
+  const int64_t count = seglo[nseg];
   if ((int64_t)blockIdx.x * blockDim.x >= count) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int64_t col = map ? (int64_t)map[i] : i;
+  const int sg = nseg == 1 ? 0 : segment_of(seglo, nseg, i);
   const int n0 = (int)((int64_t)N * blockIdx.y / nchunk);
   const int n1 = (int)((int64_t)N * (blockIdx.y + 1) / nchunk);
-  for (int cb = 0; cb < k; cb += 8) {
-    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int n = n0; n < n1; ++n) {
-      const float y = Y[(int64_t)n * ld + col];
-      const float* __restrict__ cr = C + (int64_t)n * kcap + cb;
+  uint64_t todo = __ballot(true);
+  while (todo) {
+    const int lead = __ffsll((unsigned long long)todo) - 1;
+    const int s0 = __builtin_amdgcn_readfirstlane(__shfl(sg, lead));
+    const bool mine = sg == s0;
+    todo &= ~__ballot(mine);
+    const int k = kseg[s0];
+    const float* __restrict__ Cs = C + (int64_t)s0 * stride;
+    for (int cb = 0; cb < k; cb += 8) {
+      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int n = n0; n < n1; ++n) {
+        const float y = Y[(int64_t)n * ld + col];
+        const float* __restrict__ cr = Cs + (int64_t)n * kcap + cb;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) cnt[q] += (y != cr[q]);
+        for (int q = 0; q < 8; ++q) cnt[q] += (y != cr[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (mine && cb + q < k && cnt[q]) atomicAdd(&dist[i * kMaxCentroidsExact + cb + q], cnt[q]);
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (cb + q < k && cnt[q]) atomicAdd(&dist[i * kMaxCentroidsExact + cb + q], cnt[q]);
   }
 }
 
 __global__ __launch_bounds__(256) void assign_final_kernel(
-    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ k_ptr,
-    const int32_t* __restrict__ cent_slot, int32_t* __restrict__ dist,
+    const int32_t* __restrict__ seglo, int32_t nseg, const int32_t* __restrict__ kseg,
+    int32_t stride, const int32_t* __restrict__ cent_slot, int32_t* __restrict__ dist,
     int32_t* __restrict__ flags, int32_t* __restrict__ mcar, const int32_t* __restrict__ orig,
     int32_t* __restrict__ rec) {
-  const int64_t count = *count_ptr;
-  const int k = *k_ptr;
+  const int64_t count = seglo[nseg];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
+  const int sg = segment_of(seglo, nseg, i);
+  const int k = kseg[sg];
+  const int32_t* __restrict__ cs = cent_slot + sg * stride;
   int best = 0x7fffffff, bestc = 0;
   for (int c = 0; c < k; ++c) {
     const int d = dist[i * kMaxCentroidsExact + c];
@@ -955,12 +1039,13 @@ __global__ __launch_bounds__(256) void assign_final_kernel(
   }
   bool is_cent = false;
   for (int c = 0; c < k; ++c)
-    if (cent_slot[c] == i) {
+    if (cs[c] == i) {
       is_cent = true;
       bestc = c;
     }
-  mcar[i] = bestc;
-  rec[orig[i]] = bestc;
+  const int32_t m = sg * stride + bestc;
+  mcar[i] = m;
+  rec[orig[i]] = m;
   if (!is_cent && best > 0) flags[i] = 1;
 }
 
@@ -1188,34 +1273,37 @@ void launch_argmax(const float* logits, int64_t ld, int32_t rows, const int32_t*
   check_launch();
 }
 
-void launch_lossy_pool(const float* Y, int64_t ld, const int32_t* B, int32_t pool, int32_t N,
+void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool, int32_t N,
                        float* poolrows, int32_t* poolcol, hipStream_t s) {
   SNICIT_REQUIRE(pool >= 1 && pool <= kMaxCentroidsLossy, "lossy pool must be in [1, 128]");
-  hipLaunchKernelGGL(lossy_pool_kernel, dim3((unsigned)pool), dim3(256), 0, s, Y, ld, B, pool, N,
-                     poolrows, poolcol);
+  hipLaunchKernelGGL(lossy_pool_kernel, dim3((unsigned)pool, (unsigned)sg.nseg), dim3(256), 0, s,
+                     Y, ld, sg.seglo, pool, N, poolrows, poolcol);
   check_launch();
 }
 
-void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const int32_t* B,
-                         int32_t pool, int32_t N, float eps, int32_t* cent_col, int32_t* k_out,
+void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
+                         int32_t pool, int32_t N, float eps, int32_t* cent_col, hipStream_t s) {
+  SNICIT_REQUIRE(pool <= sg.stride, "pool larger than the centroid stride");
+  hipLaunchKernelGGL(lossy_select_kernel, dim3((unsigned)sg.nseg), dim3(1024), 0, s, poolrows,
+                     poolcol, sg.seglo, sg.nseg, pool, N, eps, sg.stride, cent_col, sg.kseg,
+                     sg.kcols);
+  check_launch();
+}
+
+void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max_cols,
+                         const float* C, int32_t kcap, const int32_t* cent_col, int32_t N,
+                         float eps, int32_t* dist, int32_t* mcar, int32_t* centmark, int32_t* rec,
                          hipStream_t s) {
-  hipLaunchKernelGGL(lossy_select_kernel, dim3(1), dim3(1024), 0, s, poolrows, poolcol, B, pool,
-                     N, eps, cent_col, k_out);
-  check_launch();
-}
-
-void launch_lossy_assign(const float* Y, int64_t ld, const int32_t* B, int64_t max_cols,
-                         const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_col, int32_t N, float eps, int32_t* dist,
-                         int32_t* mcar, int32_t* centmark, int32_t* rec, hipStream_t s) {
-  SNICIT_REQUIRE(kcap % 8 == 0, "centroid capacity must be a multiple of 8");
+  SNICIT_REQUIRE(sg.stride % 8 == 0 && sg.stride <= kMaxCentroidsLossy &&
+                     kcap >= sg.stride * sg.nseg,
+                 "bad centroid layout");
   const int nchunk = std::max(1, std::min(64, N / 64));
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)nchunk);
-  hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(256), 0, s, Y, ld, B, C, kcap, k, N, nchunk,
-                     eps, dist);
+  hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(256), 0, s, Y, ld, sg.seglo, sg.nseg, C, kcap,
+                     sg.stride, sg.kseg, N, nchunk, eps, dist);
   check_launch();
-  hipLaunchKernelGGL(lossy_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s, B,
-                     k, cent_col, dist, kcap, mcar, centmark, rec);
+  hipLaunchKernelGGL(lossy_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
+                     sg.seglo, sg.nseg, sg.kseg, sg.stride, cent_col, dist, mcar, centmark, rec);
   check_launch();
 }
 
@@ -1301,38 +1389,48 @@ void launch_cat_from_flags(int32_t* flags, const int32_t* count, int64_t max_col
   check_launch();
 }
 
-void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
-                     int32_t pool, int32_t segments, int32_t N, float* F, hipStream_t s) {
-  SNICIT_REQUIRE(pool <= 32 && segments <= 32, "pool and segments must be <= 32");
-  hipLaunchKernelGGL(seg_sums_kernel, dim3((unsigned)pool), dim3(1024), 0, s, Y, ld, map, count,
-                     pool, segments, N, F);
+void launch_segment_bounds(const int32_t* orig, const int32_t* count, const int32_t* segstart,
+                           const Segs& sg, hipStream_t s) {
+  hipLaunchKernelGGL(segment_bounds_kernel, dim3(1), dim3(256), 0, s, orig, count, segstart,
+                     sg.nseg, sg.seglo);
   check_launch();
 }
-void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int32_t segments,
-                         float eps, int32_t* cent_slot, int32_t* k_out, hipStream_t s) {
-  hipLaunchKernelGGL(select_exact_kernel, dim3(1), dim3(64), 0, s, F, count, pool, segments, eps,
-                     cent_slot, k_out);
+void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const Segs& sg, int32_t pool,
+                     int32_t segments, int32_t N, float* F, hipStream_t s) {
+  SNICIT_REQUIRE(pool <= 32 && segments <= 32, "pool and segments must be <= 32");
+  hipLaunchKernelGGL(seg_sums_kernel, dim3((unsigned)pool, (unsigned)sg.nseg), dim3(1024), 0, s, Y,
+                     ld, map, sg.seglo, pool, segments, N, F);
+  check_launch();
+}
+void launch_select_exact(const float* F, const Segs& sg, int32_t pool, int32_t segments, float eps,
+                         int32_t* cent_slot, hipStream_t s) {
+  SNICIT_REQUIRE(pool <= sg.stride && sg.stride <= 64, "bad exact centroid layout");
+  hipLaunchKernelGGL(select_exact_kernel, dim3((unsigned)sg.nseg), dim3(64), 0, s, F, sg.seglo,
+                     sg.nseg, pool, segments, eps, sg.stride, cent_slot, sg.kseg, sg.kcols);
   check_launch();
 }
 void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
-                             const int32_t* cent_slot, const int32_t* k, int32_t N, float* C,
+                             const int32_t* cent_slot, const Segs& sg, int32_t N, float* C,
                              int32_t kcap, hipStream_t s) {
+  SNICIT_REQUIRE(kcap >= sg.stride * sg.nseg, "bad centroid layout");
   hipLaunchKernelGGL(gather_centroids_kernel, dim3(grid1((int64_t)N * kcap, 256, 1 << 30)),
-                     dim3(256), 0, s, Y, ld, map, cent_slot, k, N, C, kcap);
+                     dim3(256), 0, s, Y, ld, map, cent_slot, sg.kseg, sg.stride, N, C, kcap);
   check_launch();
 }
-void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
-                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_slot, int32_t N, int32_t* dist, int32_t* flags,
-                         int32_t* mcar, const int32_t* orig, int32_t* rec, hipStream_t s) {
-  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "assign expects 32 centroid slots");
+void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const Segs& sg,
+                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* cent_slot,
+                         int32_t N, int32_t* dist, int32_t* flags, int32_t* mcar,
+                         const int32_t* orig, int32_t* rec, hipStream_t s) {
+  SNICIT_REQUIRE(sg.stride == kMaxCentroidsExact && kcap >= sg.stride * sg.nseg,
+                 "assign expects 32 centroid slots per segment");
   const int nchunk = std::max(1, std::min(64, N / 128));
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)nchunk);
-  hipLaunchKernelGGL(assign_dist_kernel, grid, dim3(256), 0, s, Y, ld, map, count, C, kcap, k, N,
-                     nchunk, dist);
+  hipLaunchKernelGGL(assign_dist_kernel, grid, dim3(256), 0, s, Y, ld, map, sg.seglo, sg.nseg, C,
+                     kcap, sg.stride, sg.kseg, N, nchunk, dist);
   check_launch();
   hipLaunchKernelGGL(assign_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
-                     count, k, cent_slot, dist, flags, mcar, orig, rec);
+                     sg.seglo, sg.nseg, sg.kseg, sg.stride, cent_slot, dist, flags, mcar, orig,
+                     rec);
   check_launch();
 }
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,

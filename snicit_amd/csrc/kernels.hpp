@@ -15,6 +15,17 @@ enum Epilogue : int {
   EPI_LINEAR = 4,       // y = z + b (dense output layer, no activation)
 };
 
+// Batch segments of a plan's columns, each clustered on its own.  Centroid column
+// s*stride + c is centroid c of segment s (c < kseg[s]); kcols = columns in use
+// ((nseg-1)*stride + kseg[nseg-1]).
+struct Segs {
+  int32_t nseg;
+  int32_t stride;
+  int32_t* seglo;        // device [nseg+1]
+  int32_t* kseg;         // device [nseg]
+  int32_t* kcols;        // device [1]
+};
+
 struct SpmmLaunch {
   const Net* net;
   int layer;
@@ -75,17 +86,17 @@ void launch_argmax(const float* logits, int64_t ld, int32_t rows, const int32_t*
                    int64_t max_cols, int32_t* cat, hipStream_t s);
 
 // ---- lossy clustering (src/beyond/SNICIT/kernel.hpp:5-101) ---------------------------
-// pool row y = column y*B/pool; rows gathered sample-major into poolrows[pool][N]
-void launch_lossy_pool(const float* Y, int64_t ld, const int32_t* B, int32_t pool, int32_t N,
+// pool row y of a segment of B_s columns = its column y*B_s/pool, gathered sample-major:
+// poolrows [nseg*pool][N], poolcol [nseg*pool] (column, -1 in an empty segment)
+void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool, int32_t N,
                        float* poolrows, int32_t* poolcol, hipStream_t s);
-void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const int32_t* B,
-                         int32_t pool, int32_t N, float eps, int32_t* cent_col, int32_t* k_out,
+void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
+                         int32_t pool, int32_t N, float eps, int32_t* cent_col, hipStream_t s);
+// dist: [max_cols][128] int32, zero on entry and on exit; centmark[i] = 1 for centroids
+void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max_cols,
+                         const float* C, int32_t kcap, const int32_t* cent_col, int32_t N,
+                         float eps, int32_t* dist, int32_t* mcar, int32_t* centmark, int32_t* rec,
                          hipStream_t s);
-// dist: [max_cols][kcap] int32, zero on entry and on exit; centmark[i] = 1 for centroids
-void launch_lossy_assign(const float* Y, int64_t ld, const int32_t* B, int64_t max_cols,
-                         const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_col, int32_t N, float eps, int32_t* dist,
-                         int32_t* mcar, int32_t* centmark, int32_t* rec, hipStream_t s);
 // D[n*ldD + i] = |Y - C_m| > eps ? Y - C_m : 0 for non-centroid columns; flags = any(v != 0)
 // (or any(v > 0) with positive_only, the reference's test)
 void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap,
@@ -127,18 +138,23 @@ void launch_cat_from_flags(int32_t* flags, const int32_t* count, int64_t max_col
                            hipStream_t s);
 
 // ---- exact clustering (src/SDGC/SNICIT/kernel.hpp:8-69,138-194) -------------------
-void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
-                     int32_t pool, int32_t segments, int32_t N, float* F, hipStream_t s);
-void launch_select_exact(const float* F, const int32_t* count, int32_t pool, int32_t segments,
-                         float eps, int32_t* cent_slot, int32_t* k_out, hipStream_t s);
+// seglo[s] = first list position of batch segment s (lower bound of segstart[s] in the
+// ascending orig list), seglo[nseg] = count.
+void launch_segment_bounds(const int32_t* orig, const int32_t* count, const int32_t* segstart,
+                           const Segs& sg, hipStream_t s);
+// F: [nseg][pool][segments]
+void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const Segs& sg, int32_t pool,
+                     int32_t segments, int32_t N, float* F, hipStream_t s);
+void launch_select_exact(const float* F, const Segs& sg, int32_t pool, int32_t segments, float eps,
+                         int32_t* cent_slot, hipStream_t s);
 void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,
-                             const int32_t* cent_slot, const int32_t* k, int32_t N, float* C,
+                             const int32_t* cent_slot, const Segs& sg, int32_t N, float* C,
                              int32_t kcap, hipStream_t s);
 // dist: [max_cols][32] int32, all zero on entry and left zero on exit.
-void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const int32_t* count,
-                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* k,
-                         const int32_t* cent_slot, int32_t N, int32_t* dist, int32_t* flags,
-                         int32_t* mcar, const int32_t* orig, int32_t* rec, hipStream_t s);
+void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const Segs& sg,
+                         int64_t max_cols, const float* C, int32_t kcap, const int32_t* cent_slot,
+                         int32_t N, int32_t* dist, int32_t* flags, int32_t* mcar,
+                         const int32_t* orig, int32_t* rec, hipStream_t s);
 // D[n*ldD + j] = Y[n*ld + map[j]] - C[n*kcap + m[j]]
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
                        int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
