@@ -22,6 +22,10 @@ WORKLOADS = {
     "c1": dict(N=1024, L=120, B=6000, bias=-0.3, t=30),
     "c2": dict(N=16384, L=120, B=60000, bias=-0.4, t=30),
     "c3": dict(N=65536, L=1920, B=60000, bias=-0.45, t=30),
+    # 64 uneven batches over two resident nets in one captured graph
+    "c4": dict(batches=64, smin=100, smax=8000,
+               A=dict(N=4096, L=480, bias=-0.35, t=30),
+               B=dict(Nin=784, N=1024, L=24, K=103, Nout=10, clip=1.0, t=8)),
 }
 NET_SEED = 2023
 IMG_SEED = 60000
@@ -91,6 +95,112 @@ def cpu_baseline(wl, gpu_cat, target_s):
             "seconds": dt, "categories_match_gpu": match}
 
 
+def prototype_features(Nin, B, seed):
+    """Samples drawn from 10 seeded prototypes of U[0,1) features; half of them get N(0, 0.1)
+    noise on ~5% of the features (MNIST-like class structure for the unstructured net)."""
+    import snicit_amd as sa
+    r = np.random.default_rng(seed)
+    proto = sa.gen_features(Nin, 77, 0, 10)
+    x = proto[r.integers(0, 10, B)]
+    m = (r.random((B, Nin)) < 0.05) & (r.random(B) < 0.5)[:, None]
+    x[m] += r.standard_normal(int(m.sum())).astype(np.float32) * 0.1
+    return np.clip(x, 0, 1).astype(np.float32)
+
+
+def bench_batch(args, wl):
+    """Form (a): 64 batches of seeded uneven sizes over net A (SDGC-like, exact SNICIT) and
+    net B (unstructured MLP with dense ends, lossy SNICIT), one graph launch per step,
+    launch shapes autotuned per net.  The plain (BF) batch of the same segments is timed
+    beside it."""
+    import snicit_amd as sa
+
+    A, Bn = wl["A"], wl["B"]
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(wl["smin"], wl["smax"] + 1, wl["batches"])
+    owner = rng.integers(0, 2, wl["batches"])
+    segments = [(int(o), int(b)) for o, b in zip(owner, sizes)]
+    ctx = sa.Context(0)
+    net_a = sa.Net.generate_sdgc(ctx, A["N"], A["L"], NET_SEED, bias=A["bias"])
+    cols, vals, w_in, w_out = sa.gen_unstructured(Bn["Nin"], Bn["N"], Bn["L"], Bn["K"],
+                                                  Bn["Nout"], NET_SEED)
+    net_b = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=Bn["clip"])
+    net_b.set_io(w_in, np.zeros(Bn["N"], np.float32), 1.0, w_out,
+                 np.zeros(Bn["Nout"], np.float32))
+    cfgs = {"SNICIT": [sa.default_cfg("SNICIT", threshold=A["t"]),
+                       sa.default_cfg("SNICIT", threshold=Bn["t"], cluster=sa.CLUSTER_LOSSY)],
+            "BF": [sa.default_cfg("BF"), sa.default_cfg("BF")]}
+    cache = os.path.join(ROOT, "gpurun_out", "tune_cache.json")
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    feats = {i: prototype_features(Bn["Nin"], b, 1000 + i) for i, (o, b) in enumerate(segments)
+             if o == 1}
+    res = {}
+    for mode in ("SNICIT", "BF"):
+        batch = sa.Batch(ctx, [net_a, net_b], cfgs[mode], segments)
+        for i, (o, b) in enumerate(segments):
+            if o == 0:
+                batch.set_input_images(i, IMG_SEED, 100000 * i)
+            else:
+                batch.set_input(i, feats[i])
+        tune = batch.autotune(cache)
+        for _ in range(args.warmup):
+            batch.run()
+        t0 = time.perf_counter()
+        ev = []
+        for _ in range(args.steps):
+            ms, stats = batch.run()
+            ev.append(ms)
+        wall = (time.perf_counter() - t0) * 1000.0 / args.steps
+        cats = [batch.categories(i) for i in range(len(segments))]
+        res[mode] = dict(ms=wall, ev=float(np.mean(ev)), stats=stats, cats=cats, tune=tune)
+        log(f"{mode} batch: {wall:.3f} ms/step (graph events {np.mean(ev):.3f}); tuned "
+            f"{[(t['key'], t['grid_target'], t['narrow']) for t in tune]}; net A k="
+            f"{stats[0].centroids} nerow {stats[0].nerow_first}->{stats[0].nerow_last}; net B "
+            f"k={stats[1].centroids} nerow {stats[1].nerow_first}->{stats[1].nerow_last}")
+        del batch
+    samples_a = sum(b for o, b in segments if o == 0)
+    samples_b = sum(b for o, b in segments if o == 1)
+    edges = (float(samples_a) * A["N"] * 32 * A["L"] +
+             float(samples_b) * (Bn["N"] * Bn["K"] * Bn["L"] + Bn["N"] * Bn["Nin"] +
+                                 Bn["Nout"] * Bn["N"]))
+    same_a = all(np.array_equal(res["SNICIT"]["cats"][i], res["BF"]["cats"][i])
+                 for i, (o, _) in enumerate(segments) if o == 0)
+    agree_b = float(np.mean(np.concatenate([res["SNICIT"]["cats"][i] == res["BF"]["cats"][i]
+                                            for i, (o, _) in enumerate(segments) if o == 1])))
+    ms_s, ms_b = res["SNICIT"]["ms"], res["BF"]["ms"]
+    out = {
+        "metric": "batched SNICIT edges/s (sum over segments of samples x nnz(W) x layers / "
+                  "time), one graph launch of all segments per step",
+        "value": edges / (ms_s / 1000.0),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SDGC-like net A with blob images; seeded unstructured net B "
+                "with prototype features)",
+        "config": {"workload": f"{wl['batches']} batches of {wl['smin']}-{wl['smax']} samples "
+                               f"over net A (N={A['N']} L={A['L']} bias={A['bias']} t={A['t']}, "
+                               f"exact) and net B ({Bn['Nin']}->{Bn['N']}x{Bn['L']}->"
+                               f"{Bn['Nout']}, {Bn['K']} nnz/row, clip {Bn['clip']}, t={Bn['t']}, "
+                               "lossy)",
+                   "segments_net_a": int((owner == 0).sum()), "samples_net_a": samples_a,
+                   "segments_net_b": int((owner == 1).sum()), "samples_net_b": samples_b},
+        "plain_bf_ms_per_step": ms_b,
+        "plain_bf_edges_per_s": edges / (ms_b / 1000.0),
+        "speedup_vs_plain_bf": ms_b / ms_s,
+        "categories_identical_net_a": same_a,
+        "categories_agreement_net_b_lossy": agree_b,
+        "autotuned_shapes": res["SNICIT"]["tune"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +235,9 @@ def main():
     import snicit_amd as sa
 
     wl = dict(WORKLOADS[args.config])
+    if args.config == "c4":
+        bench_batch(args, wl)
+        return
     if args.threshold is not None:
         wl["t"] = args.threshold
     N, L, B, t = wl["N"], wl["L"], wl["B"], wl["t"]

@@ -180,6 +180,27 @@ int snicit_plan_profile(snicit_plan* plan, double* ms, int64_t* rows, double* ce
 int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
                  int64_t batch, int32_t* cat, snicit_stats* stats);
 
+/* ---- autotuning of the layer-kernel launch shape ----------------------------------
+ * The reference derives its slab width from the device's shared memory per block
+ * (utility.hpp:17-41) and its XY layout from an offline inspector (src/SDGC/inspector/).
+ * Here the launch shape of the layer kernels is measured on the plan's own net and input:
+ * grid_target = workgroups a layer launch aims for, narrow = column count up to which the
+ * thread-per-(row, column) path runs.  Results are cached in a JSON file keyed by kernel
+ * kind, N, row length, rows per panel and batch size (next power of two). */
+typedef struct snicit_tune_info {
+  int64_t grid_target;
+  int32_t narrow;
+  int32_t from_cache;   /* 1: taken from the cache file, nothing measured */
+  double ms_default;    /* one layer launch with the default shape (0 when cached) */
+  double ms_tuned;      /* one layer launch with the chosen shape */
+  char key[96];
+} snicit_tune_info;
+
+/* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
+ * may be null (measure, do not store). */
+int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info);
+int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow);
+
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another
  * (SNICIT.hpp:87-203, main/SDGC.cu:69-110).  A snicit_batch holds any number of batch
@@ -212,6 +233,8 @@ int snicit_batch_run(snicit_batch* batch, double* ms, snicit_stats* per_net);
 int snicit_batch_categories(snicit_batch* batch, int32_t seg, int32_t* cat);
 /* Final hidden activations of one segment, samples x N. */
 int snicit_batch_activations(snicit_batch* batch, int32_t seg, float* y);
+/* Autotunes every net's plan on the batch's inputs (all set); info: n_nets entries or null. */
+int snicit_batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_info* info);
 
 /* ---- host-side seeded generators (splitmix64 streams) ---------------------------- */
 /* cols[(l*N+o)*32 + r] of the SDGC-like net (ascending per row); all values = weight. */

@@ -48,6 +48,17 @@ class RunCfg(ctypes.Structure):
     ]
 
 
+class TuneInfo(ctypes.Structure):
+    _fields_ = [("grid_target", ctypes.c_int64), ("narrow", ctypes.c_int32),
+                ("from_cache", ctypes.c_int32), ("ms_default", ctypes.c_double),
+                ("ms_tuned", ctypes.c_double), ("key", ctypes.c_char * 96)]
+
+    def as_dict(self):
+        return {"grid_target": self.grid_target, "narrow": self.narrow,
+                "from_cache": bool(self.from_cache), "ms_default": self.ms_default,
+                "ms_tuned": self.ms_tuned, "key": self.key.decode()}
+
+
 class Segment(ctypes.Structure):
     _fields_ = [("net", ctypes.c_int32), ("pad", ctypes.c_int32), ("samples", ctypes.c_int64)]
 
@@ -104,6 +115,9 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_int64]),
     "snicit_infer": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
                                     ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
+    "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
+    "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
+    "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_batch_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
                                            ctypes.c_int32, ctypes.POINTER(Segment), ctypes.c_int32,
                                            ctypes.POINTER(_P)]),
@@ -330,6 +344,17 @@ class Plan:
         _check(lib().snicit_plan_categories(self.h, _i32(out)))
         return out
 
+    def autotune(self, cache_path=None):
+        """Measures the layer-kernel launch shape on this plan's input (or takes it from the
+        JSON cache); returns the TuneInfo as a dict."""
+        info = TuneInfo()
+        path = None if cache_path is None else os.fsencode(cache_path)
+        _check(lib().snicit_plan_autotune(self.h, path, ctypes.byref(info)))
+        return info.as_dict()
+
+    def set_shape(self, grid_target, narrow):
+        _check(lib().snicit_plan_set_shape(self.h, grid_target, narrow))
+
     def activations(self):
         out = np.empty((self.B, self.net.N), dtype=np.float32)
         _check(lib().snicit_plan_activations(self.h, _f32(out)))
@@ -413,6 +438,12 @@ class Batch:
         st = (Stats * len(self.nets))()
         _check(lib().snicit_batch_run(self.h, ctypes.byref(ms), st))
         return ms.value, list(st)
+
+    def autotune(self, cache_path=None):
+        info = (TuneInfo * len(self.nets))()
+        path = None if cache_path is None else os.fsencode(cache_path)
+        _check(lib().snicit_batch_autotune(self.h, path, info))
+        return [i.as_dict() for i in info]
 
     def categories(self, seg):
         out = np.empty(self.segments[seg][1], np.int32)

@@ -224,6 +224,7 @@ struct snicit_plan {
   int32_t* d_seglo = nullptr;           // [nseg + 1] segment bounds in the list at t
   int32_t* d_kseg = nullptr;            // [nseg] centroids per segment
   int32_t kcap = kMaxCentroidsExact;    // stride * nseg
+  LaunchShape shape;                    // layer-kernel launch shape (autotuned)
   int32_t stop_layer = -1;  // layers actually run by the last run
   // activations
   float* d_in = nullptr;
@@ -318,6 +319,7 @@ void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
   e.mcent = a.mcent;
   e.eps_post = a.eps_post;
   e.max_cols = a.max_cols;
+  e.shape = a.shape;
   launch_ell_spmm(e, s);
 }
 
@@ -339,6 +341,7 @@ void enqueue_input_layer(snicit_plan* p) {
   e.ldout = p->ld;
   e.count = p->d_counts;
   e.max_cols = p->cap;
+  e.shape = p->shape;
   launch_ell_spmm(e, p->stream);
 }
 
@@ -361,6 +364,7 @@ void enqueue_output_layer(snicit_plan* p, const float* y) {
   e.ldout = p->ld;
   e.count = p->d_counts;
   e.max_cols = p->cap;
+  e.shape = p->shape;
   launch_ell_spmm(e, s);
   launch_argmax(p->d_logits, p->ld, net.Nout, p->d_counts, p->cap, p->d_cat, s);
 }
@@ -378,6 +382,7 @@ SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
   a.kcap = p->kcap;
   a.eps_post = p->cfg.lossy_eps_post;
   a.max_cols = p->cap;
+  a.shape = p->shape;
   return a;
 }
 
@@ -548,7 +553,7 @@ void enqueue_post(snicit_plan* p) {
       launch_panel_post(*p->net, l, lossy, p->d_C[q], p->d_C[q ^ 1], p->kcap, p->d_k,
                         q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
                         p->d_nerow + l, map_in, p->d_mcar[mc], p->d_flags,
-                        p->cfg.lossy_eps_post, p->cap, s);
+                        p->cfg.lossy_eps_post, p->cap, p->shape, s);
       prof_mark(p, l, 1);
       prof_mark(p, l, 2);
       prof_mark(p, l, 3);
@@ -1449,6 +1454,182 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
   });
 }
 
+// ---- launch-shape autotuner -------------------------------------------------------------
+namespace {
+std::string tune_key(const snicit_plan* p) {
+  const Net& n = *p->net;
+  const char* kind = n.is_ell(0) ? "ell" : (n.mfma[0] ? "panel-mfma" : "panel-valu");
+  const int32_t K = n.is_ell(0) ? n.ell_K[0] : kSlots;
+  const int64_t rows = n.is_ell(0) ? 1 : n.N / std::max<int64_t>(1, n.panels(0));
+  int64_t bucket = 1;
+  while (bucket < p->B) bucket <<= 1;
+  char buf[96];
+  std::snprintf(buf, sizeof(buf), "%s/N%d/K%d/R%lld/B%lld", kind, n.N, K, (long long)rows,
+                (long long)bucket);
+  return buf;
+}
+
+// Cache file: a JSON object, one entry per line:
+//   "key": {"grid_target": G, "narrow": W, "ms": T},
+struct TuneEntry {
+  int64_t grid_target;
+  int32_t narrow;
+  double ms;
+};
+std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path) {
+  std::vector<std::pair<std::string, TuneEntry>> out;
+  if (!path) return out;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return out;
+  char line[512];
+  while (std::fgets(line, sizeof(line), f)) {
+    char key[128];
+    long long g = 0;
+    int w = 0;
+    double ms = 0;
+    if (std::sscanf(line, " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ms\": %lf",
+                    key, &g, &w, &ms) == 4)
+      out.push_back({key, TuneEntry{g, w, ms}});
+  }
+  std::fclose(f);
+  return out;
+}
+void tune_cache_write(const char* path,
+                      const std::vector<std::pair<std::string, TuneEntry>>& entries) {
+  if (!path) return;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  SNICIT_REQUIRE(f != nullptr, std::string("cannot write tuning cache ") + tmp);
+  std::fprintf(f, "{\n");
+  for (size_t i = 0; i < entries.size(); ++i)
+    std::fprintf(f, "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ms\": %.6f}%s\n",
+                 entries[i].first.c_str(), (long long)entries[i].second.grid_target,
+                 entries[i].second.narrow, entries[i].second.ms,
+                 i + 1 < entries.size() ? "," : "");
+  std::fprintf(f, "}\n");
+  std::fclose(f);
+  SNICIT_REQUIRE(std::rename(tmp.c_str(), path) == 0,
+                 std::string("cannot replace tuning cache ") + path);
+}
+
+// Median time of one plain launch of layer 0 over *count columns with the given shape.
+double time_layer0(snicit_plan* p, const LaunchShape& shape, const int32_t* count,
+                   hipEvent_t e0, hipEvent_t e1) {
+  SpmmLaunch a = spmm_base(p, 0, EPI_PLAIN);
+  a.yin = hidden_in(p);
+  a.ldin = p->ld;
+  a.yout = p->d_y[0];
+  a.ldout = p->ld;
+  a.count = count;
+  a.shape = shape;
+  hipStream_t s = p->stream;
+  for (int i = 0; i < 2; ++i) launch_layer(p, a, s);
+  std::vector<float> ms(7);
+  for (float& m : ms) {
+    SNICIT_HIP(hipEventRecord(e0, s));
+    launch_layer(p, a, s);
+    SNICIT_HIP(hipEventRecord(e1, s));
+    SNICIT_HIP(hipEventSynchronize(e1));
+    SNICIT_HIP(hipEventElapsedTime(&m, e0, e1));
+  }
+  std::sort(ms.begin(), ms.end());
+  return ms[ms.size() / 2];
+}
+
+void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* info) {
+  if (!p->have_input) throw Error(SNICIT_E_STATE, "autotuning needs the plan's input");
+  SNICIT_REQUIRE(p->B > 0, "autotuning needs a non-empty batch");
+  const std::string key = tune_key(p);
+  auto cache = tune_cache_read(cache_path);
+  snicit_tune_info res{};
+  std::snprintf(res.key, sizeof(res.key), "%s", key.c_str());
+  bool found = false;
+  for (const auto& e : cache)
+    if (e.first == key) {
+      p->shape.grid_target = e.second.grid_target;
+      p->shape.narrow = e.second.narrow;
+      res.ms_tuned = e.second.ms;
+      res.from_cache = 1;
+      found = true;
+    }
+  if (!found) {
+    hipStream_t s = p->stream;
+    SNICIT_HIP(hipStreamSynchronize(s));
+    if (p->net->Nin > 0) enqueue_input_layer(p);  // layer 0 reads the input layer's output
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int32_t* d_cnt = nullptr;
+    try {
+      SNICIT_HIP(hipEventCreate(&e0));
+      SNICIT_HIP(hipEventCreate(&e1));
+      d_cnt = dev_alloc<int32_t>(1);
+      const LaunchShape def{};
+      // launch size for the full batch
+      LaunchShape best = def;
+      res.ms_default = time_layer0(p, def, p->d_counts, e0, e1);
+      double best_ms = res.ms_default;
+      for (int64_t g : {2048, 4096, 8192, 32768, 65536, 131072}) {
+        LaunchShape sh = def;
+        sh.grid_target = g;
+        const double m = time_layer0(p, sh, p->d_counts, e0, e1);
+        if (m < best_ms) {
+          best_ms = m;
+          best = sh;
+        }
+      }
+      // narrow path limit: the largest column count at which it beats the wavefront layout
+      int32_t narrow = 0;
+      for (int32_t c : {32, 64, 128, 256, 512}) {
+        if (c > p->cap) break;
+        SNICIT_HIP(hipMemcpy(d_cnt, &c, 4, hipMemcpyHostToDevice));
+        LaunchShape nar = best, wide = best;
+        nar.narrow = c;
+        wide.narrow = 0;
+        if (time_layer0(p, nar, d_cnt, e0, e1) < time_layer0(p, wide, d_cnt, e0, e1)) narrow = c;
+        else break;
+      }
+      best.narrow = narrow;
+      res.ms_tuned = best_ms;
+      p->shape = best;
+    } catch (...) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      dev_free(d_cnt);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(d_cnt);
+    if (cache_path) {
+      cache.push_back({key, TuneEntry{p->shape.grid_target, p->shape.narrow, res.ms_tuned}});
+      tune_cache_write(cache_path, cache);
+    }
+  }
+  res.grid_target = p->shape.grid_target;
+  res.narrow = p->shape.narrow;
+  drop_graphs(p);
+  p->have_run = false;
+  if (info) *info = res;
+}
+}  // namespace
+
+int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_HIP(hipSetDevice(plan->net->ctx->device));
+    plan_autotune(plan, cache_path, info);
+  });
+}
+
+int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_REQUIRE(grid_target >= 1 && narrow >= 0, "bad launch shape");
+    plan->shape.grid_target = grid_target;
+    plan->shape.narrow = narrow;
+    drop_graphs(plan);
+  });
+}
+
 // ---- batches ------------------------------------------------------------------------
 struct snicit_batch {
   Ctx* ctx = nullptr;
@@ -1649,6 +1830,25 @@ int snicit_batch_activations(snicit_batch* batch, int32_t seg, float* y) {
     const int rc = snicit_plan_activations(p, all.data());
     if (rc != SNICIT_OK) throw Error(rc, snicit_last_error());
     std::memcpy(y, all.data() + (size_t)sg.off * N, (size_t)sg.samples * N * 4);
+  });
+}
+
+int snicit_batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_info* info) {
+  return guarded([&] {
+    SNICIT_REQUIRE(batch != nullptr, "null batch");
+    for (const auto& sg : batch->segs)
+      if (!sg.loaded) throw Error(SNICIT_E_STATE, "every segment needs its input first");
+    SNICIT_HIP(hipSetDevice(batch->ctx->device));
+    for (size_t i = 0; i < batch->plans.size(); ++i) {
+      snicit_plan* p = batch->plans[i];
+      p->have_input = true;
+      plan_autotune(p, cache_path, info ? &info[i] : nullptr);
+    }
+    if (batch->graph) {
+      (void)hipGraphExecDestroy(batch->graph);
+      batch->graph = nullptr;
+    }
+    batch->have_run = false;
   });
 }
 

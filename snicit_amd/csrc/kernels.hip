@@ -28,7 +28,7 @@ __device__ __forceinline__ int lane_prefix(uint64_t mask) {
 }
 
 // Column counts up to this use the narrow path (one thread per output x column).
-constexpr int kNarrowCols = 64;
+constexpr int kNarrowCols = 64;  // default column count up to which narrow paths run
 
 // Epilogue of one output value (see panel_spmm_kernel).
 template <int EPI>
@@ -65,7 +65,7 @@ __device__ __forceinline__ float epilogue(float acc, float b, float clip, bool v
 // Lossy delta (src/beyond/SNICIT/kernel.hpp:141-168): |y| <= eps -> 0.
 // A wavefront whose 64 x 32 inputs are all zero skips the multiply-adds (acc = +0 exactly,
 // what the chain would give), the zero-input skipping of src/SDGC/SNICIT/kernel.hpp:96.
-// With at most kNarrowCols columns the wavefront-per-64-columns layout would leave one
+// With at most `narrow` columns the wavefront-per-64-columns layout would leave one
 // wavefront walking the panel's rows behind dependent scalar loads, so those launches run
 // one thread per (row, column) pair instead, with the same chain and epilogue.
 template <int EPI>
@@ -75,12 +75,12 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
     const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
     int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
     const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
-    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t narrow) {
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
   const int beg = pbeg[p], end = pbeg[p + 1];
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
-  if (count <= kNarrowCols) {
+  if (count <= narrow) {
     if (blockIdx.y != 0) return;
     const int G = end - beg;
     const int64_t pairs = (int64_t)G * count;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void panel_centroid_kernel(
 // inputs of every centroid in LDS, computes the centroid rows zc = W yhat_c + b of the panel
 // (block row 0 also writes yhat_c' = f(zc)), then produces the delta rows of its columns,
 // y = f(zc[m] + W yhat_i) - f(zc[m]), with weights and zc read from LDS.  Columns <=
-// kNarrowCols use one thread per (row, column) pair in block row 0; otherwise each
+// `narrow` use one thread per (row, column) pair in block row 0; otherwise each
 // wavefront owns 64 columns and every row of the panel.  Same fused-multiply-add chain as
 // panel_spmm_kernel.
 template <bool LOSSY>
@@ -207,7 +207,8 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     float* __restrict__ cout, int32_t kcap, const int32_t* __restrict__ k_ptr,
     const float* __restrict__ din, int64_t ldin, float* __restrict__ dout, int64_t ldout,
     const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post) {
+    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post,
+    int32_t narrow) {
   constexpr int kC = kMaxCentroidsExact + 1;  // padded centroid stride in LDS
   __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
   __shared__ float x_s[kSlots * kC];
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   __shared__ int32_t o_s[kMaxPanelRows];
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
-  if (count <= kNarrowCols ? blockIdx.y != 0 : (int64_t)blockIdx.y * kBlock >= count) return;
+  if (count <= narrow ? blockIdx.y != 0 : (int64_t)blockIdx.y * kBlock >= count) return;
   const int k = *k_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     if (LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
     return y;
   };
-  if (count <= kNarrowCols) {
+  if (count <= narrow) {
     const int64_t pairs = (int64_t)G * count;
     for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
       const int jj = (int)(q % G);
@@ -411,17 +412,17 @@ __global__ __launch_bounds__(kBlock, 2) void panel_mfma_kernel(
 // r; src/beyond/general.hpp:5-33,74-102).  Block (32-row chunk, column tile); each
 // wavefront owns 64 columns and walks its rows with the fixed ascending-input chain, entries
 // fetched 16 at a time with scalar loads.  Padded entries carry value 0 (a zero product
-// leaves the chain unchanged).  Columns <= kNarrowCols: one thread per (row, column) pair.
+// leaves the chain unchanged).  Columns <= narrow: one thread per (row, column) pair.
 template <int EPI>
 __global__ __launch_bounds__(kBlock) void ell_spmm_kernel(
     const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
     const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
     int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
     const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
-    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t narrow) {
   const int r0 = blockIdx.x * 32, r1 = min(rows, r0 + 32);
   const int64_t count = *count_ptr;
-  if (count <= kNarrowCols) {
+  if (count <= narrow) {
     if (blockIdx.y != 0) return;
     const int G = r1 - r0;
     const int64_t pairs = (int64_t)G * count;
@@ -1164,7 +1165,7 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const int64_t P = net.panels(l);
   const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
   // Enough workgroups to fill 256 CUs several times over; each loops over column tiles.
-  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(P, 1));
+  int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(P, 1));
   gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
   gy = std::min<int64_t>(gy, 65535);
   SNICIT_REQUIRE(P <= 2147483647LL, "too many panels");
@@ -1185,7 +1186,7 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
     else                                                                                       \
       hipLaunchKernelGGL(panel_spmm_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, \
                          bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,     \
-                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post);                          \
+                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post, a.shape.narrow);          \
   } while (0)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_SPMM(EPI_PLAIN); break;
@@ -1218,12 +1219,12 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
-                       hipStream_t s) {
+                       const LaunchShape& shape, hipStream_t s) {
   SNICIT_REQUIRE(net.mfma[l] == 0, "post kernel needs row-major weights");
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
   const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
-  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(P, 1));
+  int64_t gy = std::max<int64_t>(1, shape.grid_target / std::max<int64_t>(P, 1));
   gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
   gy = std::min<int64_t>(gy, 65535);
   dim3 grid((unsigned)P, (unsigned)gy);
@@ -1234,11 +1235,11 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
   if (lossy)
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
-                       ldin, dout, ldout, count, map, mcent, flags, eps_post);
+                       ldin, dout, ldout, count, map, mcent, flags, eps_post, shape.narrow);
   else
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
-                       ldin, dout, ldout, count, map, mcent, flags, eps_post);
+                       ldin, dout, ldout, count, map, mcent, flags, eps_post, shape.narrow);
   check_launch();
 }
 
@@ -1246,14 +1247,14 @@ void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
   const int64_t chunks = (a.rows + 31) / 32;
   const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
-  int64_t gy = std::max<int64_t>(1, 16384 / std::max<int64_t>(chunks, 1));
+  int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(chunks, 1));
   gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
   gy = std::min<int64_t>(gy, 65535);
   dim3 grid((unsigned)chunks, (unsigned)gy);
 #define SNICIT_ELL(E)                                                                        \
   hipLaunchKernelGGL(ell_spmm_kernel<E>, grid, dim3(kBlock), 0, s, a.cols, a.vals, a.K, a.rows, \
                      a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout, a.ldout, a.count,   \
-                     a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post)
+                     a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post, a.shape.narrow)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_ELL(EPI_PLAIN); break;
     case EPI_CENTROID: SNICIT_ELL(EPI_CENTROID); break;

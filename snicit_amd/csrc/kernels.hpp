@@ -26,6 +26,14 @@ struct Segs {
   int32_t* kcols;        // device [1]
 };
 
+// Launch shape of the layer kernels (chosen per net, batch size and density by the autotuner):
+// grid_target = workgroups a layer launch aims for (panels x column tiles), narrow = column
+// count up to which the thread-per-(row, column) path runs.
+struct LaunchShape {
+  int64_t grid_target = 16384;
+  int32_t narrow = 64;
+};
+
 struct SpmmLaunch {
   const Net* net;
   int layer;
@@ -42,6 +50,7 @@ struct SpmmLaunch {
   const int32_t* mcent;  // per output column centroid index (DELTA)
   float eps_post;
   int64_t max_cols;      // upper bound of *count (grid sizing)
+  LaunchShape shape;
 };
 void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
 
@@ -53,7 +62,7 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
-                       hipStream_t s);
+                       const LaunchShape& shape, hipStream_t s);
 // Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
 void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
                            int32_t kcap, const int32_t* k, hipStream_t s);
@@ -79,6 +88,7 @@ struct EllLaunch {
   const int32_t* mcent;
   float eps_post;
   int64_t max_cols;
+  LaunchShape shape;
 };
 void launch_ell_spmm(const EllLaunch& a, hipStream_t s);
 // cat[j] = argmax_o logits[o][j] (first maximum, starting from -10000), j < *count

@@ -1,0 +1,82 @@
+"""Launch-shape autotuning: any launch shape gives the same bits, and the tuner's choice is
+cached and reused."""
+import json
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import snicit_amd as sa
+from tests import nets
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode", ["BF", "SNICIT"])
+def test_launch_shapes_do_not_change_results(ctx, mode):
+    N, L, B = 1024, 30, 700
+    cols, vals = nets.random_ell(N, L, K=32, seed=3, share=8)
+    y0 = nets.random_inputs(N, B, seed=9, density=0.2)
+    y0[100:140] = y0[0]
+    net = sa.Net.from_ell(ctx, cols, vals, -0.02)
+    cfg = sa.default_cfg(mode, threshold=4)
+    ref = None
+    for grid_target, narrow in [(16384, 64), (1, 0), (1, 100000), (1 << 20, 0), (4096, 256)]:
+        print(f"shape grid_target={grid_target} narrow={narrow}", flush=True)
+        plan = sa.Plan(net, cfg, B)
+        plan.set_shape(grid_target, narrow)
+        plan.set_input(y0)
+        plan.run()
+        got = (plan.categories(), plan.activations())
+        if ref is None:
+            ref = got
+            if mode == "SNICIT":
+                r = orc.snicit_exact(cols, vals, -0.02, 32.0, y0, 4)
+                np.testing.assert_array_equal(got[1], r["yL"])
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+
+
+def test_lossy_ell_launch_shapes(ctx):
+    Nin, N, L, K, Nout, B = 784, 1024, 4, 103, 10, 300
+    cols, vals, w_in, w_out = sa.gen_unstructured(Nin, N, L, K, Nout, 5)
+    b_in = np.zeros(N, np.float32)
+    b_out = np.zeros(Nout, np.float32)
+    net = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=1.0)
+    net.set_io(w_in, b_in, 1.0, w_out, b_out)
+    x = sa.gen_features(Nin, 2, 0, B)
+    cfg = sa.default_cfg("SNICIT", threshold=1, cluster=sa.CLUSTER_LOSSY)
+    r = orc.model_snicit_lossy(cols, vals, 0.0, 1.0, x, 1, w_in=w_in, b_in=b_in, w_out=w_out,
+                               b_out=b_out)
+    for grid_target, narrow in [(1, 0), (1, 100000), (65536, 128)]:
+        plan = sa.Plan(net, cfg, B)
+        plan.set_shape(grid_target, narrow)
+        plan.set_input(x)
+        plan.run()
+        np.testing.assert_array_equal(plan.categories(), r["cat"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
+
+
+def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
+    N, L, B = 4096, 12, 3000
+    net = sa.Net.generate_sdgc(ctx, N, L, 5, bias=-0.35)
+    cfg = sa.default_cfg("SNICIT", threshold=4)
+    cache = tmp_path / "tune.json"
+    plan = sa.Plan(net, cfg, B)
+    plan.set_input_images(3, 0, B)
+    info = plan.autotune(cache)
+    assert not info["from_cache"] and info["ms_tuned"] <= info["ms_default"]
+    assert info["key"] in json.loads(cache.read_text())
+    plan.run()
+    cat = plan.categories()
+    imgs = sa.gen_images(N, 3, 0, B)
+    _, ref, _ = orc.plain_f32(sa.gen_sdgc_cols(N, L, 5), np.full((L, N, 32), 0.0625, np.float32),
+                              -0.35, 32.0, imgs, want_y=False)
+    np.testing.assert_array_equal(cat, ref)
+    plan2 = sa.Plan(net, cfg, B)
+    plan2.set_input_images(3, 0, B)
+    info2 = plan2.autotune(cache)
+    assert info2["from_cache"]
+    assert (info2["grid_target"], info2["narrow"]) == (info["grid_target"], info["narrow"])
+    plan2.run()
+    np.testing.assert_array_equal(plan2.categories(), ref)
