@@ -26,6 +26,10 @@ WORKLOADS = {
     "c4": dict(batches=64, smin=100, smax=8000,
                A=dict(N=4096, L=480, bias=-0.35, t=30),
                B=dict(Nin=784, N=1024, L=24, K=103, Nout=10, clip=1.0, t=8)),
+    # Poisson request stream against two resident nets
+    "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=240,
+               A=dict(N=16384, L=120, bias=-0.4, t=30),
+               B=dict(Nin=3072, N=2048, L=16, K=307, Nout=10, clip=1.0, t=8)),
 }
 NET_SEED = 2023
 IMG_SEED = 60000
@@ -201,6 +205,109 @@ def bench_batch(args, wl):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args, wl):
+    """Form (b): a request loop over a resident SDGC-like net (exact SNICIT) and an
+    unstructured net (lossy SNICIT).  Saturated throughput comes from a burst of requests;
+    then Poisson arrivals at 30/70/100% of it give p50/p99 latency (submit to results on the
+    host) and the achieved edges/s."""
+    import snicit_amd as sa
+
+    A, Bn = wl["A"], wl["B"]
+    ctx = sa.Context(0)
+    net_a = sa.Net.generate_sdgc(ctx, A["N"], A["L"], NET_SEED, bias=A["bias"])
+    cols, vals, w_in, w_out = sa.gen_unstructured(Bn["Nin"], Bn["N"], Bn["L"], Bn["K"],
+                                                  Bn["Nout"], NET_SEED)
+    net_b = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=Bn["clip"])
+    net_b.set_io(w_in, np.zeros(Bn["N"], np.float32), 1.0, w_out,
+                 np.zeros(Bn["Nout"], np.float32))
+    del cols, vals
+    cfgs = [sa.default_cfg("SNICIT", threshold=A["t"]),
+            sa.default_cfg("SNICIT", threshold=Bn["t"], cluster=sa.CLUSTER_LOSSY)]
+    t0 = time.perf_counter()
+    srv = sa.Server(ctx, [net_a, net_b], cfgs, slot_samples=8192, slot_segments=64,
+                    max_slots_per_net=3, group_window_ms=2.0, hbm_budget_bytes=64 << 30)
+    log(f"server: {srv.stats()} ({time.perf_counter() - t0:.1f} s to allocate and capture)")
+    pool = prototype_features(Bn["Nin"], 8192, 99)
+    edges_per = {0: A["N"] * 32 * A["L"],
+                 1: Bn["N"] * Bn["K"] * Bn["L"] + Bn["N"] * Bn["Nin"] + Bn["Nout"] * Bn["N"]}
+    rng = np.random.default_rng(5)
+
+    def draw(n):
+        sizes = np.exp(rng.uniform(np.log(wl["smin"]), np.log(wl["smax"] + 1), n)).astype(int)
+        return [(int(rng.integers(0, 2)), int(min(max(s, wl["smin"]), wl["smax"])))
+                for s in sizes]
+
+    def submit(net, B, i):
+        if net == 0:
+            return srv.submit_images(0, IMG_SEED, 10000 * i, B)
+        off = int(rng.integers(0, 8192 - B + 1))
+        return srv.submit(1, pool[off:off + B])
+
+    def run(reqs, rate):
+        """Submits reqs (Poisson at `rate` requests/s, or all at once when None); returns
+        latencies (ms), edges and the wall time from the first submit to the last result."""
+        ids = []
+        start = time.perf_counter()
+        t_next = start
+        for i, (net, B) in enumerate(reqs):
+            if rate:
+                t_next += rng.exponential(1.0 / rate)
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            ids.append((submit(net, B, i), net, B))
+        lat, edges = [], 0.0
+        for rid, net, B in ids:
+            _, info = srv.wait(rid, 600000.0)
+            lat.append(info["ms_total"])
+            edges += float(B) * edges_per[net]
+        return np.array(lat), edges, time.perf_counter() - start
+
+    run(draw(16), None)  # warm-up
+    lat, edges, wall = run(draw(wl["burst"]), None)
+    sat_rps = wl["burst"] / wall
+    log(f"saturated: {wl['burst']} requests in {wall:.2f} s = {sat_rps:.1f} req/s, "
+        f"{edges / wall:.3e} edges/s; burst latency p50 {np.percentile(lat, 50):.1f} ms")
+    table = {"saturated_burst": {"requests_per_s": sat_rps, "edges_per_s": edges / wall,
+                                 "p50_ms": float(np.percentile(lat, 50)),
+                                 "p99_ms": float(np.percentile(lat, 99))}}
+    for load in wl["loads"]:
+        lat, edges, wall = run(draw(wl["requests"]), load * sat_rps)
+        row = {"offered_requests_per_s": load * sat_rps, "edges_per_s": edges / wall,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50))}
+        table[f"load_{int(load * 100)}"] = row
+        log(f"load {load:.0%}: p50 {row['p50_ms']:.1f} ms p99 {row['p99_ms']:.1f} ms "
+            f"(p99/p50 {row['p99_over_p50']:.2f}), {row['edges_per_s']:.3e} edges/s")
+    st = srv.stats()
+    srv.close()
+    out = {
+        "metric": "serving edges/s at 70% of saturated load, with p50/p99 latency per load",
+        "value": table["load_70"]["edges_per_s"],
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": wl["requests"],
+        "warmup": 16,
+        "ms_per_step": table["load_70"]["p50_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SDGC-like net A with device-rendered blob images; seeded "
+                "unstructured net B with prototype features)",
+        "config": {"workload": f"Poisson requests of {wl['smin']}-{wl['smax']} samples "
+                               f"(log-uniform) over net A (N={A['N']} L={A['L']} "
+                               f"bias={A['bias']} t={A['t']}, exact) and net B ({Bn['Nin']}->"
+                               f"{Bn['N']}x{Bn['L']}->{Bn['Nout']}, {Bn['K']} nnz/row, t={Bn['t']}, "
+                               "lossy); slots of 8192 samples / 64 requests, 3 per net",
+                   "server": st},
+        "serving": table,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +344,9 @@ def main():
     wl = dict(WORKLOADS[args.config])
     if args.config == "c4":
         bench_batch(args, wl)
+        return
+    if args.config == "c5":
+        bench_serve(args, wl)
         return
     if args.threshold is not None:
         wl["t"] = args.threshold

@@ -36,6 +36,7 @@ extern "C" {
 #define SNICIT_E_NOMEM (-3)     /* device or host allocation failed / over budget       */
 #define SNICIT_E_STATE (-4)     /* call out of order (e.g. run before set_input)        */
 #define SNICIT_E_UNSUPPORTED (-5)
+#define SNICIT_E_TIMEOUT (-6)       /* a wait ended before the request completed             */
 
 /* ---- run modes (main/SDGC.cu:15-20,69-110; main/beyond.cu:12-18,104-150) --------- */
 #define SNICIT_MODE_SNICIT 0      /* pre-threshold SpMM, clustering, delta SpMM, recovery */
@@ -235,6 +236,63 @@ int snicit_batch_categories(snicit_batch* batch, int32_t seg, int32_t* cat);
 int snicit_batch_activations(snicit_batch* batch, int32_t seg, float* y);
 /* Autotunes every net's plan on the batch's inputs (all set); info: n_nets entries or null. */
 int snicit_batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_info* info);
+
+/* ---- serving: continuous batching of requests over resident nets -------------------
+ * The reference has no serving path; its driver runs fixed rounds (SNICIT.hpp:87-203).
+ * A server keeps its nets resident and owns a worker thread and a set of slots per net.  A
+ * slot holds device buffers for slot_samples samples in up to slot_segments requests, its
+ * own stream, and captured stage graphs whose batch-segment table is device data, so any mix
+ * of requests replays the same graphs.  Requests queue per net; the worker fills a free slot
+ * from the queue head (FIFO) and launches it at once when no other slot of that net is busy,
+ * when the group is full, or when its oldest request has waited group_window_ms; requests
+ * that arrive while other groups run are admitted into the next free slot, so groups of one
+ * net and of different nets overlap on the device.  Slots are allocated at creation up to
+ * hbm_budget_bytes (at least one per net).  Clustering runs per request, except that with an
+ * exact-clustering net consecutive requests smaller than cluster_min samples share one
+ * clustering segment (the categories of exact SNICIT do not depend on the centroids). */
+typedef struct snicit_server snicit_server;
+typedef struct snicit_server_cfg {
+  int64_t hbm_budget_bytes;  /* device memory for all slots                              */
+  int64_t slot_samples;      /* samples per launch group (>= the largest request)         */
+  int32_t slot_segments;     /* requests (clustering segments) per launch group           */
+  int32_t max_slots_per_net; /* groups of one net in flight at once                       */
+  int32_t cluster_min;       /* exact nets: smaller requests share a clustering segment   */
+  int32_t warm;              /* 1: capture every slot's graphs at creation                */
+  double group_window_ms;    /* longest wait for a group to fill while the net is busy    */
+} snicit_server_cfg;
+
+typedef struct snicit_request_info {
+  double ms_queue;          /* submit -> launch of its group                           */
+  double ms_run;            /* launch -> results on the host                           */
+  double ms_total;          /* submit -> results on the host                           */
+  int64_t group_samples;    /* samples in the request's launch group                   */
+  int32_t group_requests;   /* requests in that group                                  */
+  int32_t slot;
+} snicit_request_info;
+
+typedef struct snicit_server_stats {
+  int64_t requests_done, samples_done, groups;
+  int64_t device_bytes;     /* allocated by the server's slots                          */
+  int32_t slots;            /* slots over all nets                                      */
+  int32_t queued;           /* requests waiting for a slot                              */
+} snicit_server_stats;
+
+void snicit_server_cfg_default(snicit_server_cfg* cfg);
+int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
+                         int32_t n_nets, const snicit_server_cfg* cfg, snicit_server** out);
+/* Stops the worker after the groups in flight; pending requests are dropped. */
+int snicit_server_destroy(snicit_server* server);
+/* samples x (Nin or N) host floats, copied before return. */
+int snicit_server_submit(snicit_server* server, int32_t net, const float* x, int64_t samples,
+                         uint64_t* request_id);
+/* Device-rendered blob images of samples first_sample.. (nets without an input layer). */
+int snicit_server_submit_images(snicit_server* server, int32_t net, uint64_t seed,
+                                int64_t first_sample, int64_t samples, uint64_t* request_id);
+/* Waits up to timeout_ms (< 0: no limit) for the request; on success copies its categories
+ * (samples int32) and releases it.  SNICIT_E_TIMEOUT when it is not done in time. */
+int snicit_server_wait(snicit_server* server, uint64_t request_id, double timeout_ms,
+                       int32_t* cat, snicit_request_info* info);
+int snicit_server_stats_get(snicit_server* server, snicit_server_stats* stats);
 
 /* ---- host-side seeded generators (splitmix64 streams) ---------------------------- */
 /* cols[(l*N+o)*32 + r] of the SDGC-like net (ascending per row); all values = weight. */

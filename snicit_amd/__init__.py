@@ -23,6 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsnicit.so")
 
+E_TIMEOUT = -6
 MODE_SNICIT, MODE_BF, MODE_SNIG, MODE_XY, MODE_PLAIN_DENSE = 0, 1, 2, 3, 4
 MODES = {"SNICIT": MODE_SNICIT, "BF": MODE_BF, "SNIG": MODE_SNIG, "XY": MODE_XY,
          "PLAIN_DENSE": MODE_PLAIN_DENSE}
@@ -34,7 +35,11 @@ SDGC_BIAS = {1024: -0.3, 4096: -0.35, 16384: -0.4, 65536: -0.45}
 
 
 class SnicitError(RuntimeError):
-    pass
+    """A failed library call; ``code`` is the negative SNICIT_E_* status."""
+
+    def __init__(self, msg, code=0):
+        super().__init__(msg)
+        self.code = code
 
 
 class RunCfg(ctypes.Structure):
@@ -57,6 +62,31 @@ class TuneInfo(ctypes.Structure):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
                 "from_cache": bool(self.from_cache), "ms_default": self.ms_default,
                 "ms_tuned": self.ms_tuned, "key": self.key.decode()}
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slot_samples", ctypes.c_int64),
+                ("slot_segments", ctypes.c_int32), ("max_slots_per_net", ctypes.c_int32),
+                ("cluster_min", ctypes.c_int32), ("warm", ctypes.c_int32),
+                ("group_window_ms", ctypes.c_double)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("ms_queue", ctypes.c_double), ("ms_run", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("group_samples", ctypes.c_int64),
+                ("group_requests", ctypes.c_int32), ("slot", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_int64), ("samples_done", ctypes.c_int64),
+                ("groups", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("slots", ctypes.c_int32), ("queued", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Segment(ctypes.Structure):
@@ -118,6 +148,19 @@ SIGNATURES = {
     "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
+    "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
+    "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
+                                            ctypes.c_int32, ctypes.POINTER(ServerCfg),
+                                            ctypes.POINTER(_P)]),
+    "snicit_server_destroy": (ctypes.c_int, [_P]),
+    "snicit_server_submit": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "snicit_server_submit_images": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64,
+                                                   ctypes.c_int64, ctypes.c_int64,
+                                                   ctypes.POINTER(ctypes.c_uint64)]),
+    "snicit_server_wait": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double, _I32P,
+                                          ctypes.POINTER(RequestInfo)]),
+    "snicit_server_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
     "snicit_batch_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
                                            ctypes.c_int32, ctypes.POINTER(Segment), ctypes.c_int32,
                                            ctypes.POINTER(_P)]),
@@ -160,7 +203,7 @@ def lib():
 def _check(rc):
     if rc != 0:
         msg = lib().snicit_last_error().decode(errors="replace")
-        raise SnicitError(f"snicit error {rc}: {msg}")
+        raise SnicitError(f"snicit error {rc}: {msg}", rc)
 
 
 def _i32(a):
@@ -459,6 +502,70 @@ class Batch:
     def close(self):
         if self.h:
             lib().snicit_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_cfg(**kw):
+    cfg = ServerCfg()
+    lib().snicit_server_cfg_default(ctypes.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Server:
+    """Continuous-batching request loop over resident nets (one worker thread, slots of
+    captured stage graphs per net).  ``submit`` returns a request id; ``wait`` returns
+    (categories, info dict)."""
+
+    def __init__(self, ctx, nets, cfgs, cfg=None, **kw):
+        self.nets = list(nets)
+        self.cfg = cfg if cfg is not None else server_cfg(**kw)
+        hs = (_P * len(nets))(*[n.h for n in nets])
+        cf = (RunCfg * len(cfgs))(*cfgs)
+        h = _P()
+        _check(lib().snicit_server_create(ctx.h, hs, cf, len(nets), ctypes.byref(self.cfg),
+                                          ctypes.byref(h)))
+        self.h = h
+        self._sizes = {}
+
+    def submit(self, net, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = self.nets[net]
+        assert x.ndim == 2 and x.shape[1] == (n.Nin or n.N)
+        rid = ctypes.c_uint64()
+        _check(lib().snicit_server_submit(self.h, net, _f32(x), x.shape[0], ctypes.byref(rid)))
+        self._sizes[rid.value] = x.shape[0]
+        return rid.value
+
+    def submit_images(self, net, seed, first_sample, samples):
+        rid = ctypes.c_uint64()
+        _check(lib().snicit_server_submit_images(self.h, net, seed, first_sample, samples,
+                                                 ctypes.byref(rid)))
+        self._sizes[rid.value] = samples
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        out = np.empty(self._sizes.get(rid, 0), np.int32)
+        info = RequestInfo()
+        _check(lib().snicit_server_wait(self.h, rid, timeout_ms, _i32(out), ctypes.byref(info)))
+        self._sizes.pop(rid, None)
+        return out, info.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().snicit_server_stats_get(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().snicit_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["kernels.hip", "gen.cpp", "net.cpp", "api.cpp"]
+SOURCES = ["kernels.hip", "gen.cpp", "net.cpp", "api.cpp", "server.cpp"]
 
 
 def _run(cmd, cwd=ROOT):

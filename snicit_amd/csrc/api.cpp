@@ -14,50 +14,12 @@
 #include <functional>
 #include <memory>
 
-#include "kernels.hpp"
+#include "plan.hpp"
 
 using namespace snicit;
 
-struct snicit_ctx : Ctx {};
-struct snicit_net : Net {};
 
 namespace {
-
-thread_local std::string g_last_error;
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return SNICIT_OK;
-  } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return SNICIT_E_NOMEM;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return SNICIT_E_INVALID;
-  }
-}
-
-template <class T>
-T* dev_alloc(size_t n, int64_t* acct = nullptr) {
-  void* p = nullptr;
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc(&p, n * sizeof(T));
-  if (e != hipSuccess)
-    throw Error(e == hipErrorOutOfMemory ? SNICIT_E_NOMEM : SNICIT_E_HIP,
-                "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
-                    hipGetErrorString(e));
-  if (acct) *acct += (int64_t)(n * sizeof(T));
-  return static_cast<T*>(p);
-}
-
-void dev_free(void* p) {
-  if (p) (void)hipFree(p);
-}
 
 // Upload one layer's host panels into the net's concatenated arrays.
 // Layers whose panels average >= kMfmaMinAvgRows rows get slot-major weights (matrix-core
@@ -209,75 +171,10 @@ void destroy_net(Net* net) {
 }  // namespace
 
 // ---- plan ------------------------------------------------------------------------------
-struct snicit_plan {
-  Net* net = nullptr;
-  snicit_run_cfg cfg{};
-  hipStream_t stream = nullptr;  // the context's stream, or a batch's per-net stream
-  int64_t cap = 0, ld = 0;
-  int64_t B = 0;
-  bool have_input = false, have_run = false;
-  // batch segments: columns [segstart[s], segstart[s+1]) are clustered on their own; one
-  // segment covering every column for a plain plan
-  int32_t nseg = 1;
-  int32_t stride = kMaxCentroidsExact;  // centroid columns per segment
-  int32_t* d_segstart = nullptr;        // [nseg + 1]
-  int32_t* d_seglo = nullptr;           // [nseg + 1] segment bounds in the list at t
-  int32_t* d_kseg = nullptr;            // [nseg] centroids per segment
-  int32_t kcap = kMaxCentroidsExact;    // stride * nseg
-  LaunchShape shape;                    // layer-kernel launch shape (autotuned)
-  int32_t stop_layer = -1;  // layers actually run by the last run
-  // activations
-  float* d_in = nullptr;
-  float* d_y[2] = {nullptr, nullptr};
-  // bookkeeping
-  int32_t* d_flags = nullptr;
-  int32_t* d_map = nullptr;   // pre-threshold / plain live list
-  int32_t* d_mapD = nullptr;  // post-threshold delta list
-  int32_t* d_orig[2] = {nullptr, nullptr};
-  int32_t* d_mcar[2] = {nullptr, nullptr};
-  int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
-  int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
-  int32_t* d_k = nullptr;  // centroid columns in use (k with one segment)
-  float* d_F = nullptr;
-  int32_t* d_cent_slot = nullptr;
-  float* d_C[2] = {nullptr, nullptr};
-  float* d_zc = nullptr;
-  int32_t* d_rec = nullptr;
-  int32_t* d_dist = nullptr;  // [cap][stride] distance accumulators (kept zero between runs)
-  int32_t* d_catc = nullptr;
-  int32_t* d_act = nullptr;
-  int32_t* d_cat = nullptr;
-  int32_t* d_img_params = nullptr;
-  int64_t img_params_cap = 0;
-  int32_t* d_img_blobs = nullptr;
-  int32_t* d_img_proto = nullptr;
-  float* d_stage = nullptr;  // [cap][max(N, Nin, Nout)] staging for host transfers
-  // nets with dense input / output layers
-  float* d_h0 = nullptr;      // [N][ld] input-layer output (hidden loop input), later the
-                              //   recovered hidden activations
-  float* d_logits = nullptr;  // [Nout][ld]
-  int64_t in_rows = 0;        // rows of d_in: N, or Nin padded to 16
-  // lossy clustering
-  float* d_poolrows = nullptr;  // [nseg * pool][N]
-  int32_t* d_poolcol = nullptr;
-  int32_t* d_centmark = nullptr;
-  // where the last run left its final state
-  int last_orig = 0, last_mcar = 0;
-  const float* last_D = nullptr;
-  const float* last_Y = nullptr;
-  const float* last_C = nullptr;
-  bool last_list = false;  // final columns selected by d_map (plain) / d_mapD (SNICIT)
-  // graphs: stage 0 pre, 1 cluster, 2 post, 3 recover
-  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
-  int32_t graph_stop = -2;
-  hipEvent_t ev[5] = {};
-  int64_t device_bytes = 0;
-  // per-launch profiling (eager only): events [4*l .. 4*l+3] = main start/stop, centroid
-  bool profiling = false;
-  std::vector<hipEvent_t> prof_ev;
-};
 
-namespace {
+namespace snicit {
+
+thread_local std::string g_last_error;
 
 bool plain_mode(int32_t mode) { return mode != SNICIT_MODE_SNICIT; }
 bool compacting(const snicit_plan* p) {
@@ -684,7 +581,25 @@ void enqueue_all(snicit_plan* p) {
   }
 }
 
-void plan_stats(snicit_plan* p, snicit_stats* stats, const float* stage_ms);
+void plan_launch(snicit_plan* p) {
+  if (!p->have_input) throw Error(SNICIT_E_STATE, "plan has no input");
+  if (p->graph_stop != -1) {
+    drop_graphs(p);
+    p->graph_stop = -1;
+  }
+  plan_bookkeeping(p, -1);
+  if (plain_mode(p->cfg.mode)) {
+    run_stage(p, 0, [&] {
+      const int c = enqueue_plain_layers(p, p->net->L, true);
+      enqueue_plain_categories(p, p->net->L, c);
+    });
+  } else {
+    run_stage(p, 0, [&] { enqueue_plain_layers(p, p->cfg.threshold, false); });
+    run_stage(p, 1, [&] { enqueue_cluster(p); });
+    run_stage(p, 2, [&] { enqueue_post(p); });
+    run_stage(p, 3, [&] { enqueue_recover(p); });
+  }
+}
 
 void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
   if (!p->have_input) throw Error(SNICIT_E_STATE, "plan has no input (call set_input first)");
@@ -790,7 +705,7 @@ void plan_free(snicit_plan* p) {
   for (void* q : ptrs) dev_free(q);
 }
 
-}  // namespace
+}  // namespace snicit
 
 // ---- C-ABI ------------------------------------------------------------------------------
 extern "C" {
@@ -1100,7 +1015,9 @@ int snicit_plan_destroy(snicit_plan* plan) {
   });
 }
 
-namespace {
+}  // extern "C"
+
+namespace snicit {
 void ensure_stage(snicit_plan* p) {
   const Net& n = *p->net;
   const int64_t w = std::max<int64_t>(n.N, std::max<int64_t>(n.Nin, n.Nout));
@@ -1118,7 +1035,9 @@ void set_count(snicit_plan* p, int64_t B) {
   p->have_input = true;
   p->have_run = false;
 }
-}  // namespace
+}  // namespace snicit
+
+extern "C" {
 
 int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
   return guarded([&] {
@@ -1130,7 +1049,9 @@ int snicit_plan_set_input(snicit_plan* p, const float* y0, int64_t B) {
   });
 }
 
-namespace {
+}  // extern "C"
+
+namespace snicit {
 // Renders the blob images of samples first_sample .. first_sample+B-1 on the device into
 // columns col_off .. col_off+B-1 of the plan's input.
 void load_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B,
@@ -1185,7 +1106,9 @@ void load_host_input(snicit_plan* p, const float* y0, int64_t B, int64_t col_off
   SNICIT_HIP(hipMemcpyAsync(p->d_stage, y0, (size_t)B * width * 4, hipMemcpyHostToDevice, s));
   launch_transpose_in(p->d_stage, B, width, p->d_in + col_off, p->ld, s);
 }
-}  // namespace
+}  // namespace snicit
+
+extern "C" {
 
 int snicit_plan_set_input_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B) {
   return guarded([&] {
@@ -1455,6 +1378,8 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
 }
 
 // ---- launch-shape autotuner -------------------------------------------------------------
+}  // extern "C"
+
 namespace {
 std::string tune_key(const snicit_plan* p) {
   const Net& n = *p->net;
@@ -1612,6 +1537,8 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
 }
 }  // namespace
 
+extern "C" {
+
 int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info) {
   return guarded([&] {
     SNICIT_REQUIRE(plan != nullptr, "null plan");
@@ -1647,6 +1574,8 @@ struct snicit_batch {
   std::vector<hipEvent_t> join;
   bool have_run = false;
 };
+
+}  // extern "C"
 
 namespace {
 void batch_free(snicit_batch* b) {
@@ -1700,6 +1629,8 @@ void batch_capture(snicit_batch* b) {
   SNICIT_HIP(e);
 }
 }  // namespace
+
+extern "C" {
 
 int snicit_batch_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
                         int32_t n_nets, const snicit_segment* segs, int32_t n_segs,

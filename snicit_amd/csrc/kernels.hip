@@ -552,8 +552,8 @@ __global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict
 // Pruning (y_star_gen, src/beyond/SNICIT/kernel.hpp:19-41): in pool order, every present
 // base i removes each other present row y with #{n : |P_y[n] - P_i[n]| > eps} < N*eps.
 // One workgroup per batch segment; wavefront w tests rows w, w+16, ...; lanes stride the
-// neurons.  Centroids (columns) go to cent_col[s*stride ...]; the last segment's block also
-// writes the number of centroid columns in use.
+// neurons.  Centroids (columns) go to cent_col[s*stride ...]; kcols (zero on entry) becomes
+// the number of centroid columns in use.
 __global__ __launch_bounds__(1024) void lossy_select_kernel(
     const float* __restrict__ poolrows, const int32_t* __restrict__ poolcol,
     const int32_t* __restrict__ seglo, int32_t nseg, int32_t pool, int32_t N, float eps,
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(1024) void lossy_select_kernel(
     for (int y = 0; y < P; ++y)
       if (alive[y]) cent_col[sg * stride + k++] = poolcol[sg * pool + y];
     kseg[sg] = k;
-    if (sg == nseg - 1) *kcols = sg * stride + k;
+    if (k > 0) atomicMax(kcols, sg * stride + k);
   }
 }
 
@@ -924,8 +924,8 @@ __global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict_
 // present base c removes each other present row y whose segment sums all lie within eps
 // of c's (count of |F_y - F_c| > eps below segments*eps).  Survivors are the centroids,
 // listed in pool order as list positions in cent_slot[s*stride ...].  One wavefront per
-// batch segment, lane = pool row.  The last segment's block also writes the number of
-// centroid columns in use, (nseg-1)*stride + k_last.
+// batch segment, lane = pool row.  kcols (zero on entry) becomes the number of centroid
+// columns in use, s*stride + k_s of the last segment with centroids.
 __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restrict__ F,
                                                            const int32_t* __restrict__ seglo,
                                                            int32_t nseg, int32_t pool,
@@ -952,8 +952,9 @@ __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restric
   const uint64_t am = __ballot(alive);
   if (alive) cent_slot[sg * stride + lane_prefix(am)] = lo + lane;
   if (lane == 0) {
-    kseg[sg] = __popcll(am);
-    if (sg == nseg - 1) *kcols = sg * stride + __popcll(am);
+    const int k = __popcll(am);
+    kseg[sg] = k;
+    if (k > 0) atomicMax(kcols, sg * stride + k);
   }
 }
 
@@ -1285,6 +1286,7 @@ void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool,
 void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
                          int32_t pool, int32_t N, float eps, int32_t* cent_col, hipStream_t s) {
   SNICIT_REQUIRE(pool <= sg.stride, "pool larger than the centroid stride");
+  launch_fill_i32(sg.kcols, 0, 1, s);
   hipLaunchKernelGGL(lossy_select_kernel, dim3((unsigned)sg.nseg), dim3(1024), 0, s, poolrows,
                      poolcol, sg.seglo, sg.nseg, pool, N, eps, sg.stride, cent_col, sg.kseg,
                      sg.kcols);
@@ -1406,6 +1408,7 @@ void launch_seg_sums(const float* Y, int64_t ld, const int32_t* map, const Segs&
 void launch_select_exact(const float* F, const Segs& sg, int32_t pool, int32_t segments, float eps,
                          int32_t* cent_slot, hipStream_t s) {
   SNICIT_REQUIRE(pool <= sg.stride && sg.stride <= 64, "bad exact centroid layout");
+  launch_fill_i32(sg.kcols, 0, 1, s);
   hipLaunchKernelGGL(select_exact_kernel, dim3((unsigned)sg.nseg), dim3(64), 0, s, F, sg.seglo,
                      sg.nseg, pool, segments, eps, sg.stride, cent_slot, sg.kseg, sg.kcols);
   check_launch();

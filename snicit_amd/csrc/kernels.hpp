@@ -17,7 +17,7 @@ enum Epilogue : int {
 
 // Batch segments of a plan's columns, each clustered on its own.  Centroid column
 // s*stride + c is centroid c of segment s (c < kseg[s]); kcols = columns in use
-// ((nseg-1)*stride + kseg[nseg-1]).
+// (s*stride + kseg[s] of the last segment with centroids).  Segments may be empty.
 struct Segs {
   int32_t nseg;
   int32_t stride;
