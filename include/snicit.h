@@ -186,11 +186,15 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
  * (utility.hpp:17-41) and its XY layout from an offline inspector (src/SDGC/inspector/).
  * Here the launch shape of the layer kernels is measured on the plan's own net and input:
  * grid_target = workgroups a layer launch aims for, narrow = column count up to which the
- * thread-per-(row, column) path runs.  Results are cached in a JSON file keyed by kernel
+ * thread-per-(row, column) path runs, ell_lds_kb = LDS budget (KB, <= 160) of the input tile
+ * that wide-row layers stage (0: gather inputs directly), ell_chunked = 1 to stage 64-column
+ * tiles in chunks of input rows (layers of >= 128 rows).  Results are cached in a JSON file keyed by kernel
  * kind, N, row length, rows per panel and batch size (next power of two). */
 typedef struct snicit_tune_info {
   int64_t grid_target;
   int32_t narrow;
+  int32_t ell_lds_kb;   /* wide-row layers: LDS input-tile budget, 0 = direct gathers */
+  int32_t ell_chunked;  /* wide-row layers: 64-column tiles staged in chunks of input rows */
   int32_t from_cache;   /* 1: taken from the cache file, nothing measured */
   double ms_default;    /* one layer launch with the default shape (0 when cached) */
   double ms_tuned;      /* one layer launch with the chosen shape */
@@ -200,7 +204,8 @@ typedef struct snicit_tune_info {
 /* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
  * may be null (measure, do not store). */
 int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info);
-int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow);
+int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow,
+                          int32_t ell_lds_kb, int32_t ell_chunked);
 
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another

@@ -55,11 +55,13 @@ class RunCfg(ctypes.Structure):
 
 class TuneInfo(ctypes.Structure):
     _fields_ = [("grid_target", ctypes.c_int64), ("narrow", ctypes.c_int32),
+                ("ell_lds_kb", ctypes.c_int32), ("ell_chunked", ctypes.c_int32),
                 ("from_cache", ctypes.c_int32), ("ms_default", ctypes.c_double),
                 ("ms_tuned", ctypes.c_double), ("key", ctypes.c_char * 96)]
 
     def as_dict(self):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
+                "ell_lds_kb": self.ell_lds_kb, "ell_chunked": self.ell_chunked,
                 "from_cache": bool(self.from_cache), "ms_default": self.ms_default,
                 "ms_tuned": self.ms_tuned, "key": self.key.decode()}
 
@@ -146,7 +148,8 @@ SIGNATURES = {
     "snicit_infer": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
                                     ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
     "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
-    "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
+    "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
     "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
@@ -395,8 +398,8 @@ class Plan:
         _check(lib().snicit_plan_autotune(self.h, path, ctypes.byref(info)))
         return info.as_dict()
 
-    def set_shape(self, grid_target, narrow):
-        _check(lib().snicit_plan_set_shape(self.h, grid_target, narrow))
+    def set_shape(self, grid_target, narrow, ell_lds_kb=80, ell_chunked=1):
+        _check(lib().snicit_plan_set_shape(self.h, grid_target, narrow, ell_lds_kb, ell_chunked))
 
     def activations(self):
         out = np.empty((self.B, self.net.N), dtype=np.float32)

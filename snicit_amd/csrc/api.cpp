@@ -200,6 +200,7 @@ void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
   e.vals = net.d_evals + net.ell_off[a.layer];
   e.K = net.ell_K[a.layer];
   e.rows = net.N;
+  e.in_rows = net.N;
   e.bias_vec = net.bias_vec(a.layer);
   e.bias_s = net.bias_scalar(a.layer);
   e.clip = net.clip;
@@ -229,6 +230,7 @@ void enqueue_input_layer(snicit_plan* p) {
   e.vals = net.d_win;
   e.K = net.Kin;
   e.rows = net.N;
+  e.in_rows = net.Kin;
   e.bias_vec = net.d_bin;
   e.clip = net.in_clip;
   e.epilogue = EPI_PLAIN;
@@ -252,6 +254,7 @@ void enqueue_output_layer(snicit_plan* p, const float* y) {
   e.vals = net.d_wout;
   e.K = net.Kout;
   e.rows = net.Nout;
+  e.in_rows = net.Kout;
   e.bias_vec = net.d_bout;
   e.clip = 0.f;
   e.epilogue = EPI_LINEAR;
@@ -1395,10 +1398,12 @@ std::string tune_key(const snicit_plan* p) {
 }
 
 // Cache file: a JSON object, one entry per line:
-//   "key": {"grid_target": G, "narrow": W, "ms": T},
+//   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C, "ms": T},
 struct TuneEntry {
   int64_t grid_target;
   int32_t narrow;
+  int32_t ell_lds_kb;
+  int32_t ell_chunked;
   double ms;
 };
 std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path) {
@@ -1410,11 +1415,13 @@ std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path)
   while (std::fgets(line, sizeof(line), f)) {
     char key[128];
     long long g = 0;
-    int w = 0;
+    int w = 0, e = 0, ch = 0;
     double ms = 0;
-    if (std::sscanf(line, " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ms\": %lf",
-                    key, &g, &w, &ms) == 4)
-      out.push_back({key, TuneEntry{g, w, ms}});
+    if (std::sscanf(line,
+                    " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
+                    "\"ell_chunked\": %d, \"ms\": %lf",
+                    key, &g, &w, &e, &ch, &ms) == 6)
+      out.push_back({key, TuneEntry{g, w, e, ch, ms}});
   }
   std::fclose(f);
   return out;
@@ -1427,9 +1434,12 @@ void tune_cache_write(const char* path,
   SNICIT_REQUIRE(f != nullptr, std::string("cannot write tuning cache ") + tmp);
   std::fprintf(f, "{\n");
   for (size_t i = 0; i < entries.size(); ++i)
-    std::fprintf(f, "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ms\": %.6f}%s\n",
+    std::fprintf(f,
+                 "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
+                 "\"ell_chunked\": %d, \"ms\": %.6f}%s\n",
                  entries[i].first.c_str(), (long long)entries[i].second.grid_target,
-                 entries[i].second.narrow, entries[i].second.ms,
+                 entries[i].second.narrow, entries[i].second.ell_lds_kb,
+                 entries[i].second.ell_chunked, entries[i].second.ms,
                  i + 1 < entries.size() ? "," : "");
   std::fprintf(f, "}\n");
   std::fclose(f);
@@ -1473,6 +1483,8 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
     if (e.first == key) {
       p->shape.grid_target = e.second.grid_target;
       p->shape.narrow = e.second.narrow;
+      p->shape.ell_lds_kb = e.second.ell_lds_kb;
+      p->shape.ell_chunked = e.second.ell_chunked;
       res.ms_tuned = e.second.ms;
       res.from_cache = 1;
       found = true;
@@ -1492,8 +1504,23 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       LaunchShape best = def;
       res.ms_default = time_layer0(p, def, p->d_counts, e0, e1);
       double best_ms = res.ms_default;
+      // wide-row layers: staged-tile LDS budget (0 = direct gathers), then the grid size
+      if (p->net->is_ell(0)) {
+        const int32_t cand[][2] = {{0, 0}, {64, 0}, {128, 0}, {40, 1}, {144, 1}};
+        for (const auto& kc : cand) {
+          LaunchShape sh = def;
+          sh.ell_lds_kb = kc[0];
+          sh.ell_chunked = kc[1];
+          const double m = time_layer0(p, sh, p->d_counts, e0, e1);
+          if (m < best_ms) {
+            best_ms = m;
+            best = sh;
+          }
+        }
+      }
+      const LaunchShape base = best;
       for (int64_t g : {2048, 4096, 8192, 32768, 65536, 131072}) {
-        LaunchShape sh = def;
+        LaunchShape sh = base;
         sh.grid_target = g;
         const double m = time_layer0(p, sh, p->d_counts, e0, e1);
         if (m < best_ms) {
@@ -1525,12 +1552,15 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
     (void)hipEventDestroy(e1);
     dev_free(d_cnt);
     if (cache_path) {
-      cache.push_back({key, TuneEntry{p->shape.grid_target, p->shape.narrow, res.ms_tuned}});
+      cache.push_back({key, TuneEntry{p->shape.grid_target, p->shape.narrow, p->shape.ell_lds_kb,
+                                      p->shape.ell_chunked, res.ms_tuned}});
       tune_cache_write(cache_path, cache);
     }
   }
   res.grid_target = p->shape.grid_target;
   res.narrow = p->shape.narrow;
+  res.ell_lds_kb = p->shape.ell_lds_kb;
+  res.ell_chunked = p->shape.ell_chunked;
   drop_graphs(p);
   p->have_run = false;
   if (info) *info = res;
@@ -1547,12 +1577,17 @@ int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_
   });
 }
 
-int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow) {
+int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow,
+                          int32_t ell_lds_kb, int32_t ell_chunked) {
   return guarded([&] {
     SNICIT_REQUIRE(plan != nullptr, "null plan");
-    SNICIT_REQUIRE(grid_target >= 1 && narrow >= 0, "bad launch shape");
+    SNICIT_REQUIRE(grid_target >= 1 && narrow >= 0 && ell_lds_kb >= 0 && ell_lds_kb <= 160 &&
+                       (ell_chunked == 0 || ell_chunked == 1),
+                   "bad launch shape");
     plan->shape.grid_target = grid_target;
     plan->shape.narrow = narrow;
+    plan->shape.ell_lds_kb = ell_lds_kb;
+    plan->shape.ell_chunked = ell_chunked;
     drop_graphs(plan);
   });
 }

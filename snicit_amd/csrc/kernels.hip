@@ -28,6 +28,8 @@ __device__ __forceinline__ int lane_prefix(uint64_t mask) {
 }
 
 // Column counts up to this use the narrow path (one thread per output x column).
+// Dynamic LDS a workgroup may use (gfx950: 160 KB per workgroup, no attribute needed).
+constexpr int64_t kLdsPerBlock = 160 * 1024;
 constexpr int kNarrowCols = 64;  // default column count up to which narrow paths run
 
 // Epilogue of one output value (see panel_spmm_kernel).
@@ -470,6 +472,220 @@ __global__ __launch_bounds__(kBlock) void ell_spmm_kernel(
       if (valid) yout[(int64_t)o * ldout + pos] = y;
     }
     if (flags && valid && flag) flags[pos] = 1;
+  }
+}
+
+// LDS-staged wide-row SpMM.  A block stages the input tile x_s[r][c] = Y[r][column c0+c] for
+// all in_rows input neurons and TC columns in LDS (row stride TC+4 floats, so the float4
+// reads of different random rows spread over the banks).  Row-parallel form (RPT): thread t
+// owns output row blockIdx.y*256 + t and walks its entries in ascending input order, 4 at a
+// time (int4/float4 loads of its own row, the next four prefetched while the current four
+// are used), reading the TC inputs of each entry from LDS into TC running accumulators, so
+// one weight load feeds TC multiply-adds and no input is gathered from L2 per entry.
+// Few-rows form (!RPT, layers with fewer rows than a block has threads): thread (c, j) owns
+// column c of rows j, j + 256/TC, ...  Same chain and epilogues as ell_spmm_kernel.
+// Per-column flags are ORed over the block's rows with ballots.
+template <int TC>
+__device__ __forceinline__ void ell_stage_tile(const float* __restrict__ yin, int64_t ldin,
+                                               int32_t in_rows, int64_t c0, int64_t count,
+                                               const int32_t* __restrict__ map,
+                                               float* __restrict__ x_s) {
+  constexpr int XS = TC + 4;
+  for (int idx = threadIdx.x; idx < in_rows * TC; idx += 256) {
+    const int r = idx / TC, c = idx % TC;
+    const int64_t pos = c0 + c;
+    float v = 0.f;
+    if (pos < count) v = yin[(int64_t)r * ldin + (map ? (int64_t)map[pos] : pos)];
+    x_s[r * XS + c] = v;
+  }
+}
+
+template <int EPI, int TC, bool RPT>
+__global__ __launch_bounds__(256) void ell_lds_kernel(
+    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
+    int32_t in_rows, const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout, int64_t ldout,
+    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
+    int32_t* __restrict__ flags, float* __restrict__ zc, int32_t kcap,
+    const int32_t* __restrict__ mcent, float eps_post) {
+  constexpr int XS = TC + 4;
+  extern __shared__ float x_s[];  // [in_rows][XS]
+  const int64_t count = *count_ptr;
+  const int lane = lane_id();
+  for (int64_t c0 = (int64_t)blockIdx.x * TC; c0 < count; c0 += (int64_t)gridDim.x * TC) {
+    __syncthreads();  // the previous tile's reads are done
+    ell_stage_tile<TC>(yin, ldin, in_rows, c0, count, map, x_s);
+    __syncthreads();
+    if (RPT) {
+      const int o = blockIdx.y * 256 + threadIdx.x;
+      uint32_t fbits = 0;
+      if (o < rows) {
+        float acc[TC];
+#pragma unroll
+        for (int c = 0; c < TC; ++c) acc[c] = 0.f;
+        const float* __restrict__ vr = vals + (size_t)o * K;
+        const int32_t* __restrict__ cr = cols ? cols + (size_t)o * K : nullptr;
+        float4 v4 = *reinterpret_cast<const float4*>(vr);
+        int4 c4 = cr ? *reinterpret_cast<const int4*>(cr) : make_int4(0, 1, 2, 3);
+        for (int r = 0; r < K; r += 4) {
+          const float wv[4] = {v4.x, v4.y, v4.z, v4.w};
+          const int cv[4] = {c4.x, c4.y, c4.z, c4.w};
+          if (r + 4 < K) {
+            v4 = *reinterpret_cast<const float4*>(vr + r + 4);
+            c4 = cr ? *reinterpret_cast<const int4*>(cr + r + 4)
+                    : make_int4(r + 4, r + 5, r + 6, r + 7);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float* __restrict__ xp = x_s + cv[q] * XS;
+#pragma unroll
+            for (int c = 0; c < TC; c += 4) {
+              const float4 xv = *reinterpret_cast<const float4*>(xp + c);
+              acc[c + 0] = __builtin_fmaf(wv[q], xv.x, acc[c + 0]);
+              acc[c + 1] = __builtin_fmaf(wv[q], xv.y, acc[c + 1]);
+              acc[c + 2] = __builtin_fmaf(wv[q], xv.z, acc[c + 2]);
+              acc[c + 3] = __builtin_fmaf(wv[q], xv.w, acc[c + 3]);
+            }
+          }
+        }
+        const float b = bias_vec ? bias_vec[o] : bias_s;
+#pragma unroll
+        for (int c = 0; c < TC; ++c) {
+          const int64_t pos = c0 + c;
+          const bool valid = pos < count;
+          int32_t mi = 0;
+          if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = mcent[valid ? pos : c0];
+          bool f = false;
+          const float y = epilogue<EPI>(acc[c], b, clip, valid, o, pos, mi, zc, kcap, eps_post, f);
+          if (valid) yout[(int64_t)o * ldout + pos] = y;
+          if (valid && f) fbits |= 1u << c;
+        }
+      }
+      if (flags) {
+#pragma unroll
+        for (int c = 0; c < TC; ++c) {
+          const uint64_t m = __ballot((fbits >> c) & 1u);
+          if (m != 0ull && lane == 0) flags[c0 + c] = 1;
+        }
+      }
+    } else {
+      const int c = threadIdx.x % TC;
+      const int64_t pos = c0 + c;
+      const bool valid = pos < count;
+      int32_t mi = 0;
+      if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = mcent[valid ? pos : c0];
+      bool fl = false;
+      for (int o = threadIdx.x / TC; o < rows; o += 256 / TC) {
+        const float* __restrict__ vr = vals + (size_t)o * K;
+        const int32_t* __restrict__ cr = cols ? cols + (size_t)o * K : nullptr;
+        float acc = 0.f;
+        for (int r = 0; r < K; r += 4) {
+          const float4 v4 = *reinterpret_cast<const float4*>(vr + r);
+          const int4 c4 = cr ? *reinterpret_cast<const int4*>(cr + r) : make_int4(r, r + 1, r + 2, r + 3);
+          acc = __builtin_fmaf(v4.x, x_s[c4.x * XS + c], acc);
+          acc = __builtin_fmaf(v4.y, x_s[c4.y * XS + c], acc);
+          acc = __builtin_fmaf(v4.z, x_s[c4.z * XS + c], acc);
+          acc = __builtin_fmaf(v4.w, x_s[c4.w * XS + c], acc);
+        }
+        const float b = bias_vec ? bias_vec[o] : bias_s;
+        bool f = false;
+        const float y = epilogue<EPI>(acc, b, clip, valid, o, pos, mi, zc, kcap, eps_post, f);
+        if (valid) yout[(int64_t)o * ldout + pos] = y;
+        fl |= valid && f;
+      }
+      if (flags && fl) flags[pos] = 1;
+    }
+  }
+}
+
+// Chunked row-parallel form for wide tiles: thread t owns output row blockIdx.x*256 + t and
+// TC = 64 columns.  The input rows are staged in chunks of rc rows (x_s[r - R0][c], row stride
+// TC+4); a thread's chain pauses at the first entry past the chunk and resumes with the next
+// chunk, so the multiply-add order over its ascending entries is unchanged.  Entries are read
+// four at a time (int4/float4) and each feeds TC multiply-adds from LDS.  Consecutive blocks
+// take the row groups of one column tile, so the tile's inputs come from L2 after the first.
+template <int EPI>
+__global__ __launch_bounds__(256) void ell_chunk_kernel(
+    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
+    int32_t in_rows, int32_t rc, const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout, int64_t ldout,
+    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
+    int32_t* __restrict__ flags, float* __restrict__ zc, int32_t kcap,
+    const int32_t* __restrict__ mcent, float eps_post) {
+  constexpr int TC = 64, XS = TC + 4;
+  extern __shared__ float x_s[];  // [rc][XS]
+  const int64_t count = *count_ptr;
+  const int lane = lane_id();
+  const int o = blockIdx.x * 256 + threadIdx.x;
+  const bool row_ok = o < rows;
+  const float* __restrict__ vr = vals + (size_t)(row_ok ? o : 0) * K;
+  const int32_t* __restrict__ cr = cols ? cols + (size_t)(row_ok ? o : 0) * K : nullptr;
+  for (int64_t c0 = (int64_t)blockIdx.y * TC; c0 < count; c0 += (int64_t)gridDim.y * TC) {
+    float acc[TC];
+#pragma unroll
+    for (int c = 0; c < TC; ++c) acc[c] = 0.f;
+    int p = 0;   // next group of four entries
+    int bi = 4;  // position in the loaded group (4: none loaded)
+    int4 cb = make_int4(0, 0, 0, 0);
+    float4 vb = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int R0 = 0; R0 < in_rows; R0 += rc) {
+      const int nr = min(rc, in_rows - R0);
+      __syncthreads();  // the previous chunk's reads are done
+      for (int idx = threadIdx.x; idx < nr * TC; idx += 256) {
+        const int r = idx / TC, c = idx % TC;
+        const int64_t pos = c0 + c;
+        float v = 0.f;
+        if (pos < count) v = yin[(int64_t)(R0 + r) * ldin + (map ? (int64_t)map[pos] : pos)];
+        x_s[r * XS + c] = v;
+      }
+      __syncthreads();
+      if (!row_ok) continue;
+      const int R1 = R0 + nr;
+      while (true) {
+        if (bi == 4) {
+          if (p >= K) break;
+          vb = *reinterpret_cast<const float4*>(vr + p);
+          cb = cr ? *reinterpret_cast<const int4*>(cr + p) : make_int4(p, p + 1, p + 2, p + 3);
+          bi = 0;
+          p += 4;
+        }
+        const int col = bi == 0 ? cb.x : bi == 1 ? cb.y : bi == 2 ? cb.z : cb.w;
+        if (col >= R1) break;
+        const float w = bi == 0 ? vb.x : bi == 1 ? vb.y : bi == 2 ? vb.z : vb.w;
+        const float* __restrict__ xp = x_s + (col - R0) * XS;
+#pragma unroll
+        for (int c = 0; c < TC; c += 4) {
+          const float4 xv = *reinterpret_cast<const float4*>(xp + c);
+          acc[c + 0] = __builtin_fmaf(w, xv.x, acc[c + 0]);
+          acc[c + 1] = __builtin_fmaf(w, xv.y, acc[c + 1]);
+          acc[c + 2] = __builtin_fmaf(w, xv.z, acc[c + 2]);
+          acc[c + 3] = __builtin_fmaf(w, xv.w, acc[c + 3]);
+        }
+        ++bi;
+      }
+    }
+    uint64_t fbits = 0;
+    if (row_ok) {
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        const int64_t pos = c0 + c;
+        const bool valid = pos < count;
+        int32_t mi = 0;
+        if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = mcent[valid ? pos : c0];
+        bool f = false;
+        const float y = epilogue<EPI>(acc[c], b, clip, valid, o, pos, mi, zc, kcap, eps_post, f);
+        if (valid) yout[(int64_t)o * ldout + pos] = y;
+        if (valid && f) fbits |= 1ull << c;
+      }
+    }
+    if (flags) {
+#pragma unroll
+      for (int c = 0; c < TC; ++c) {
+        const uint64_t m = __ballot((fbits >> c) & 1ull);
+        if (m != 0ull && lane == 0) flags[c0 + c] = 1;
+      }
+    }
   }
 }
 
@@ -1244,8 +1460,91 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
   check_launch();
 }
 
+// Columns per LDS tile of the staged kernel: the largest of 32/16/8/4 whose tile (row stride
+// TC+4 floats) fits the LDS budget, 0 when none does (or the budget is 0: direct gathers).
+int ell_tile_cols(int64_t in_rows, int64_t lds_bytes) {
+  for (int tc : {32, 16, 8, 4})
+    if (in_rows * (tc + 4) * 4 <= lds_bytes) return tc;
+  return 0;
+}
+
+template <int TC, bool RPT>
+void launch_ell_lds(const EllLaunch& a, hipStream_t s) {
+  const int64_t tiles = (a.max_cols + TC - 1) / TC;
+  const int64_t groups = RPT ? (a.rows + 255) / 256 : 1;
+  int64_t gx = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(groups, 1));
+  gx = std::min<int64_t>(gx, std::max<int64_t>(tiles, 1));
+  gx = std::min<int64_t>(gx, 1 << 30);
+  dim3 grid((unsigned)gx, (unsigned)groups);
+  const size_t lds = (size_t)a.in_rows * (TC + 4) * 4;
+#define SNICIT_ELL_LDS(E)                                                                      \
+  hipLaunchKernelGGL((ell_lds_kernel<E, TC, RPT>), grid, dim3(256), lds, s, a.cols, a.vals,    \
+                     a.K, a.rows, a.in_rows, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin,       \
+                     a.yout, a.ldout, a.count, a.map, a.flags, a.zc, a.kcap, a.mcent,           \
+                     a.eps_post)
+  switch (a.epilogue) {
+    case EPI_PLAIN: SNICIT_ELL_LDS(EPI_PLAIN); break;
+    case EPI_CENTROID: SNICIT_ELL_LDS(EPI_CENTROID); break;
+    case EPI_DELTA: SNICIT_ELL_LDS(EPI_DELTA); break;
+    case EPI_DELTA_LOSSY: SNICIT_ELL_LDS(EPI_DELTA_LOSSY); break;
+    case EPI_LINEAR: SNICIT_ELL_LDS(EPI_LINEAR); break;
+    default: throw Error(SNICIT_E_INVALID, "bad epilogue");
+  }
+#undef SNICIT_ELL_LDS
+  check_launch();
+}
+
+template <int TC>
+void launch_ell_lds_form(const EllLaunch& a, hipStream_t s) {
+  // few rows: thread per (column, row slot); otherwise thread per row
+  if (a.rows < 128) launch_ell_lds<TC, false>(a, s);
+  else launch_ell_lds<TC, true>(a, s);
+}
+
+void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
+  constexpr int TC = 64;
+  const int64_t tiles = (a.max_cols + TC - 1) / TC;
+  const int64_t groups = (a.rows + 255) / 256;
+  int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(groups, 1));
+  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
+  gy = std::min<int64_t>(gy, 65535);
+  dim3 grid((unsigned)groups, (unsigned)gy);
+  const size_t lds = (size_t)rc * (TC + 4) * 4;
+#define SNICIT_ELL_CHUNK(E)                                                                    \
+  hipLaunchKernelGGL(ell_chunk_kernel<E>, grid, dim3(256), lds, s, a.cols, a.vals, a.K, a.rows, \
+                     a.in_rows, rc, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout,        \
+                     a.ldout, a.count, a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post)
+  switch (a.epilogue) {
+    case EPI_PLAIN: SNICIT_ELL_CHUNK(EPI_PLAIN); break;
+    case EPI_CENTROID: SNICIT_ELL_CHUNK(EPI_CENTROID); break;
+    case EPI_DELTA: SNICIT_ELL_CHUNK(EPI_DELTA); break;
+    case EPI_DELTA_LOSSY: SNICIT_ELL_CHUNK(EPI_DELTA_LOSSY); break;
+    case EPI_LINEAR: SNICIT_ELL_CHUNK(EPI_LINEAR); break;
+    default: throw Error(SNICIT_E_INVALID, "bad epilogue");
+  }
+#undef SNICIT_ELL_CHUNK
+  check_launch();
+}
+
 void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
+  SNICIT_REQUIRE(a.in_rows > 0, "ELL launch needs the input row count");
+  const int64_t lds = std::min<int64_t>((int64_t)a.shape.ell_lds_kb * 1024, kLdsPerBlock);
+  if (a.rows >= 128 && a.shape.ell_chunked) {
+    // wide tiles, input rows staged in chunks that fit the budget (multiple of 16 rows)
+    const int64_t rc = std::min<int64_t>(a.in_rows, lds / ((64 + 4) * 4)) / 16 * 16;
+    if (rc >= 16) {
+      launch_ell_chunk(a, (int32_t)std::min<int64_t>(rc, a.in_rows), s);
+      return;
+    }
+  }
+  switch (ell_tile_cols(a.in_rows, lds)) {
+    case 32: launch_ell_lds_form<32>(a, s); return;
+    case 16: launch_ell_lds_form<16>(a, s); return;
+    case 8: launch_ell_lds_form<8>(a, s); return;
+    case 4: launch_ell_lds_form<4>(a, s); return;
+    default: break;
+  }
   const int64_t chunks = (a.rows + 31) / 32;
   const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
   int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(chunks, 1));

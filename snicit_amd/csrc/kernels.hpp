@@ -32,6 +32,10 @@ struct Segs {
 struct LaunchShape {
   int64_t grid_target = 16384;
   int32_t narrow = 64;
+  // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
+  int32_t ell_lds_kb = 80;
+  // 1: wide-row layers with >= 128 rows stage 64-column tiles in chunks of input rows
+  int32_t ell_chunked = 1;
 };
 
 struct SpmmLaunch {
@@ -73,6 +77,7 @@ struct EllLaunch {
   const int32_t* cols;  // [rows][K] or null (column r = input r: dense layer)
   const float* vals;    // [rows][K]
   int32_t K, rows;
+  int32_t in_rows;      // input neurons the entries refer to (rows of yin read)
   const float* bias_vec;
   float bias_s, clip;
   int epilogue;

@@ -48,9 +48,13 @@ def test_lossy_ell_launch_shapes(ctx):
     cfg = sa.default_cfg("SNICIT", threshold=1, cluster=sa.CLUSTER_LOSSY)
     r = orc.model_snicit_lossy(cols, vals, 0.0, 1.0, x, 1, w_in=w_in, b_in=b_in, w_out=w_out,
                                b_out=b_out)
-    for grid_target, narrow in [(1, 0), (1, 100000), (65536, 128)]:
+    shapes = [(1, 0, 0, 0), (1, 100000, 64, 0), (65536, 128, 16, 0), (16384, 64, 32, 0),
+              (7, 0, 8, 0), (16384, 64, 80, 1), (1, 0, 5, 1), (3, 64, 160, 1), (65536, 0, 40, 1)]
+    for grid_target, narrow, lds_kb, chunked in shapes:
+        print(f"shape grid_target={grid_target} narrow={narrow} ell_lds_kb={lds_kb} "
+              f"chunked={chunked}", flush=True)
         plan = sa.Plan(net, cfg, B)
-        plan.set_shape(grid_target, narrow)
+        plan.set_shape(grid_target, narrow, lds_kb, chunked)
         plan.set_input(x)
         plan.run()
         np.testing.assert_array_equal(plan.categories(), r["cat"])
