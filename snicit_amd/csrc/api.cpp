@@ -1353,8 +1353,9 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
     if (rc) throw Error(rc, g_last_error);
     std::unique_ptr<snicit_plan, int (*)(snicit_plan*)> hold(p, snicit_plan_destroy);
     snicit_stats acc{};
+    const int64_t width = net->Nin > 0 ? net->Nin : net->N;  // samples are Nin or N floats
     for (int64_t off = 0; off < B; off += batch) {
-      rc = snicit_plan_set_input(p, y0 + (size_t)off * net->N, batch);
+      rc = snicit_plan_set_input(p, y0 + (size_t)off * width, batch);
       if (rc) throw Error(rc, g_last_error);
       snicit_stats st{};
       rc = snicit_plan_run(p, &st);

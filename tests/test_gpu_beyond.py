@@ -146,3 +146,22 @@ def test_lossy_graph_replays_and_rerun(ctx):
                                    b_in=m["b_in"], w_out=m["w_out"], b_out=m["b_out"])
         np.testing.assert_array_equal(plan.categories(), r["cat"])
         np.testing.assert_array_equal(plan.activations(), r["yL"])
+
+
+def test_infer_rounds_with_input_layer(ctx):
+    """snicit_infer steps through host samples of Nin features per round."""
+    Nin, N, L, K, Nout, B, batch = 784, 1024, 4, 103, 10, 600, 200
+    m = _model(Nin, N, L, K, Nout, seed=6)
+    x = _clustered(Nin, B, 4)
+    net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
+    net.set_io(m["w_in"], m["b_in"], 1.0, m["w_out"], m["b_out"])
+    cat = sa.SNICIT(net, threshold=1, cluster=sa.CLUSTER_LOSSY).infer(x, batch=batch)
+    for off in range(0, B, batch):
+        r = orc.model_snicit_lossy(m["cols"], m["vals"], 0.0, 1.0, x[off:off + batch], 1,
+                                   w_in=m["w_in"], b_in=m["b_in"], w_out=m["w_out"],
+                                   b_out=m["b_out"])
+        np.testing.assert_array_equal(cat[off:off + batch], r["cat"])
+    cat_bf = sa.BF(net).infer(x, batch=batch)
+    _, _, ref = orc.model_plain(m["cols"], m["vals"], 0.0, 1.0, x, w_in=m["w_in"], b_in=m["b_in"],
+                                w_out=m["w_out"], b_out=m["b_out"])
+    np.testing.assert_array_equal(cat_bf, ref)
