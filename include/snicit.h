@@ -119,6 +119,14 @@ void snicit_run_cfg_default(snicit_run_cfg* cfg);
 int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K,
                           const int32_t* cols, const float* vals, int32_t bias_kind,
                           const float* bias, float clip, snicit_net** out);
+/* Hidden layers as CSR: row_ptr holds L*(N+1) absolute offsets into cols/vals (layer l's
+ * rows at row_ptr[l*(N+1) ..]).  rows_are_inputs = 1 reads row i as input neuron i feeding
+ * the output neurons in cols, the orientation of the reference's beyond files
+ * (src/beyond/SNICIT/SNICIT.hpp:261-283, general.hpp:36-72); 0 reads rows as output neurons.
+ * Rows are padded to the longest (value-0 entries) and stored as panels or wide ELL. */
+int snicit_net_create_csr(snicit_ctx* ctx, int32_t N, int32_t L, const int64_t* row_ptr,
+                          const int32_t* cols, const float* vals, int32_t rows_are_inputs,
+                          int32_t bias_kind, const float* bias, float clip, snicit_net** out);
 
 /* Seeded SDGC-like net (32 nnz/row of `weight`, banded+permuted 16-chunk pattern),
  * built directly in panel form with the weights filled on the device; no reference

@@ -114,6 +114,7 @@ class Stats(ctypes.Structure):
 _P = ctypes.c_void_p
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _F32P = ctypes.POINTER(ctypes.c_float)
+_I64P = ctypes.POINTER(ctypes.c_int64)
 SIGNATURES = {
     "snicit_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
     "snicit_ctx_destroy": (ctypes.c_int, [_P]),
@@ -125,6 +126,9 @@ SIGNATURES = {
     "snicit_net_generate_sdgc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_uint64, ctypes.c_float, ctypes.c_float,
                                                 ctypes.c_float, ctypes.POINTER(_P)]),
+    "snicit_net_create_csr": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _I64P, _I32P,
+                                             _F32P, ctypes.c_int32, ctypes.c_int32, _F32P,
+                                             ctypes.c_float, ctypes.POINTER(_P)]),
     "snicit_net_set_io": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, _F32P, ctypes.c_float,
                                          ctypes.c_int32, _F32P, _F32P]),
     "snicit_net_destroy": (ctypes.c_int, [_P]),
@@ -315,6 +319,28 @@ class Net:
         h = _P()
         _check(lib().snicit_net_create_ell(ctx.h, N, L, K, _i32(cols), _f32(vals), kind,
                                            _f32(b.ravel()), clip, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csr(cls, ctx, N, row_ptr, cols, vals, bias, clip=32.0, rows_are_inputs=False):
+        """Hidden layers as CSR: row_ptr (L, N+1) absolute offsets into cols / vals.  With
+        rows_are_inputs, row i lists the outputs that input i feeds (the reference's beyond
+        file orientation)."""
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        L = rp.shape[0]
+        assert rp.shape == (L, N + 1)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        b = np.atleast_1d(np.ascontiguousarray(bias, dtype=np.float32))
+        kind = {1: BIAS_SCALAR, L: BIAS_PER_LAYER, L * N: BIAS_PER_NEURON}.get(b.size)
+        if kind is None:
+            raise ValueError("bias must have 1, L or L*N entries")
+        h = _P()
+        _check(lib().snicit_net_create_csr(ctx.h, N, L, rp.ctypes.data_as(_I64P),
+                                           _i32(cols if cols.size else np.zeros(1, np.int32)),
+                                           _f32(vals if vals.size else np.zeros(1, np.float32)),
+                                           int(rows_are_inputs), kind, _f32(b.ravel()), clip,
+                                           ctypes.byref(h)))
         return cls(ctx, h)
 
     @classmethod

@@ -12,6 +12,7 @@ PKG = os.path.join(ROOT, "snicit_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libsnicit.so")
 CLI = os.path.join(PKG, "snicit_sdgc")
+CLI_BEYOND = os.path.join(PKG, "snicit_beyond")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -48,10 +49,9 @@ def build_lib():
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    cli_src = os.path.join(CSRC, "cli_sdgc.cpp")
-    if os.path.exists(cli_src):
-        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), cli_src,
-              "-o", CLI, "-L", PKG, "-lsnicit", "-Wl,-rpath,$ORIGIN"])
+    for name, out in (("cli_sdgc.cpp", CLI), ("cli_beyond.cpp", CLI_BEYOND)):
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+              os.path.join(CSRC, name), "-o", out, "-L", PKG, "-lsnicit", "-Wl,-rpath,$ORIGIN"])
 
 
 def build_oracle():

@@ -794,6 +794,62 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, cons
   });
 }
 
+int snicit_net_create_csr(snicit_ctx* ctx, int32_t N, int32_t L, const int64_t* row_ptr,
+                          const int32_t* cols, const float* vals, int32_t rows_are_inputs,
+                          int32_t bias_kind, const float* bias, float clip, snicit_net** out) {
+  return guarded([&] {
+    SNICIT_REQUIRE(ctx && out && row_ptr && (cols || row_ptr[(size_t)L * (N + 1) - 1] == 0) &&
+                       (vals || row_ptr[(size_t)L * (N + 1) - 1] == 0),
+                   "null argument");
+    SNICIT_REQUIRE(N > 0 && L > 0, "need N > 0 and L > 0");
+    // per output row, (input, value) in ascending input (stable for repeated inputs)
+    std::vector<std::vector<std::vector<std::pair<int32_t, float>>>> rows(L);
+    int32_t K = 1;
+    int64_t nnz = 0;
+    for (int32_t l = 0; l < L; ++l) {
+      const int64_t* rp = row_ptr + (size_t)l * (N + 1);
+      auto& R = rows[l];
+      R.assign(N, {});
+      for (int32_t i = 0; i < N; ++i) {
+        SNICIT_REQUIRE(rp[i] >= 0 && rp[i] <= rp[i + 1], "row_ptr must be non-decreasing");
+        for (int64_t e = rp[i]; e < rp[i + 1]; ++e) {
+          const int32_t c = cols[e];
+          SNICIT_REQUIRE(c >= 0 && c < N, "CSR column out of range in layer " + std::to_string(l));
+          if (rows_are_inputs) R[c].push_back({i, vals[e]});  // input i feeds output c
+          else R[i].push_back({c, vals[e]});
+        }
+      }
+      for (auto& r : R) {
+        if (!rows_are_inputs)
+          std::stable_sort(r.begin(), r.end(),
+                           [](const std::pair<int32_t, float>& x,
+                              const std::pair<int32_t, float>& y) { return x.first < y.first; });
+        K = std::max<int32_t>(K, (int32_t)r.size());
+        nnz += (int64_t)r.size();
+      }
+    }
+    SNICIT_REQUIRE(K <= N, "a row has more entries than N");
+    // padded per-row lists: value-0 entries repeat the last input (or input 0 for empty rows)
+    std::vector<int32_t> ec((size_t)L * N * K);
+    std::vector<float> ev((size_t)L * N * K);
+    for (int32_t l = 0; l < L; ++l)
+      for (int32_t o = 0; o < N; ++o) {
+        const auto& r = rows[l][o];
+        const size_t at = ((size_t)l * N + o) * K;
+        for (int32_t k = 0; k < K; ++k) {
+          const bool real = k < (int32_t)r.size();
+          ec[at + k] = real ? r[k].first : (r.empty() ? 0 : r.back().first);
+          ev[at + k] = real ? r[k].second : 0.f;
+        }
+      }
+    rows.clear();
+    const int rc = snicit_net_create_ell(ctx, N, L, K, ec.data(), ev.data(), bias_kind, bias, clip,
+                                         out);
+    if (rc != SNICIT_OK) throw Error(rc, g_last_error);
+    (*out)->nnz_hidden = nnz / L;
+  });
+}
+
 int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t seed, float weight,
                              float bias, float clip, snicit_net** out) {
   return guarded([&] {

@@ -165,3 +165,29 @@ def test_lossy_clustering_invariants():
     nonempty = ((np.abs(Yt - C[want]) > 0.04).any(axis=1))
     nonempty[cent] = False
     assert r["nerow_at_t"] == nonempty.sum()
+
+
+def test_real_net_a_formats_against_float64():
+    """The reference's net A files (fixture): CSR over input neurons and input-major dense
+    layers.  The oracle's model forward on the transposed lists agrees with a float64 dense
+    forward built straight from the file arrays (y[k] = sum_i x[i] W[i, k])."""
+    a = nets.beyond_net_a()
+    N, L = 128, 18
+    cols, vals = nets.csr_inputs_to_ell(a["row_ptr"], a["cols"], a["vals"], N)
+    x = nets.prototype_features(784, 200, 3)
+    yL, lg, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, w_in=a["w_in"].T.copy(),
+                                  b_in=a["b_in"], in_clip=1.0, w_out=a["w_out"].T.copy(),
+                                  b_out=a["b_out"])
+    h = np.clip(x.astype(np.float64) @ a["w_in"].astype(np.float64) + a["b_in"], 0, 1)
+    for l in range(L):
+        W = np.zeros((N, N))
+        rp = a["row_ptr"][l]
+        for i in range(N):
+            for e in range(rp[i], rp[i + 1]):
+                W[i, a["cols"][e]] += a["vals"][e]
+        h = np.clip(h @ W + a["bias"][l], 0, 1)
+    ref_lg = h @ a["w_out"].astype(np.float64) + a["b_out"]
+    assert 0.05 < (yL > 0).mean() < 0.95
+    np.testing.assert_allclose(yL, h, rtol=0, atol=1e-4)
+    np.testing.assert_allclose(lg, ref_lg, rtol=0, atol=1e-3)
+    assert (cat == np.argmax(ref_lg, axis=1)).mean() > 0.99

@@ -165,3 +165,34 @@ def test_infer_rounds_with_input_layer(ctx):
     _, _, ref = orc.model_plain(m["cols"], m["vals"], 0.0, 1.0, x, w_in=m["w_in"], b_in=m["b_in"],
                                 w_out=m["w_out"], b_out=m["b_out"])
     np.testing.assert_array_equal(cat_bf, ref)
+
+
+@pytest.mark.parametrize("mode,t", [("BF", 0), ("SNICIT", 2), ("SNICIT", 8)])
+def test_real_net_a_from_csr(ctx, mode, t):
+    """The reference's beyond net A (fixture of its TSV files): CSR over input neurons through
+    snicit_net_create_csr, input-major dense ends transposed; BF and lossy SNICIT (t = 8 is the
+    reference's default for this net) equal the oracle bit for bit.  Inputs: 20% of 784 pixels
+    set to U[0,1) values, which leaves hundreds of non-empty delta rows at t = 2."""
+    a = nets.beyond_net_a()
+    N, B = 128, 700
+    r0 = np.random.default_rng(1)
+    x = ((r0.random((B, 784)) < 0.2) * r0.random((B, 784))).astype(np.float32)
+    net = sa.Net.from_csr(ctx, N, a["row_ptr"], a["cols"], a["vals"], a["bias"], clip=1.0,
+                          rows_are_inputs=True)
+    net.set_io(a["w_in"].T.copy(), a["b_in"], 1.0, a["w_out"].T.copy(), a["b_out"])
+    cols, vals = nets.csr_inputs_to_ell(a["row_ptr"], a["cols"], a["vals"], N)
+    kw = dict(w_in=a["w_in"].T.copy(), b_in=a["b_in"], in_clip=1.0, w_out=a["w_out"].T.copy(),
+              b_out=a["b_out"])
+    if mode == "BF":
+        plan, _ = _run(net, sa.default_cfg("BF"), x)
+        yL, lg, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, **kw)
+        np.testing.assert_array_equal(plan.activations(), yL)
+    else:
+        plan, st = _run(net, sa.default_cfg("SNICIT", threshold=t, cluster=sa.CLUSTER_LOSSY), x)
+        r = orc.model_snicit_lossy(cols, vals, a["bias"], 1.0, x, t, **kw)
+        assert st.centroids == r["k"]
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
+        lg, cat = r["logits"], r["cat"]
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LOGITS).reshape(B, 10), lg)
+    np.testing.assert_array_equal(plan.categories(), cat)
