@@ -196,13 +196,15 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
  * grid_target = workgroups a layer launch aims for, narrow = column count up to which the
  * thread-per-(row, column) path runs, ell_lds_kb = LDS budget (KB, <= 160) of the input tile
  * that wide-row layers stage (0: gather inputs directly), ell_chunked = 1 to stage 64-column
- * tiles in chunks of input rows (layers of >= 128 rows).  Results are cached in a JSON file keyed by kernel
- * kind, N, row length, rows per panel and batch size (next power of two). */
+ * tiles in chunks of input rows with a thread per row (layers of >= 128 rows), 2 (default)
+ * for a wavefront per row over double-buffered chunks of 128 input rows (sparse hidden
+ * layers).  Results are cached in a JSON file keyed by kernel kind, N, row length, rows per
+ * panel and batch size (next power of two). */
 typedef struct snicit_tune_info {
   int64_t grid_target;
   int32_t narrow;
   int32_t ell_lds_kb;   /* wide-row layers: LDS input-tile budget, 0 = direct gathers */
-  int32_t ell_chunked;  /* wide-row layers: 64-column tiles staged in chunks of input rows */
+  int32_t ell_chunked;  /* wide-row layers: 1 thread per row, 2 wavefront per row (chunked) */
   int32_t from_cache;   /* 1: taken from the cache file, nothing measured */
   double ms_default;    /* one layer launch with the default shape (0 when cached) */
   double ms_tuned;      /* one layer launch with the chosen shape */

@@ -424,7 +424,7 @@ class Plan:
         _check(lib().snicit_plan_autotune(self.h, path, ctypes.byref(info)))
         return info.as_dict()
 
-    def set_shape(self, grid_target, narrow, ell_lds_kb=80, ell_chunked=1):
+    def set_shape(self, grid_target, narrow, ell_lds_kb=80, ell_chunked=2):
         _check(lib().snicit_plan_set_shape(self.h, grid_target, narrow, ell_lds_kb, ell_chunked))
 
     def activations(self):

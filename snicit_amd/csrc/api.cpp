@@ -52,6 +52,7 @@ struct NetBuilder {
   // wide-ELL layers
   std::vector<int32_t> ecols;
   std::vector<float> evals;
+  std::vector<int32_t> erng;
   void add_ell(const int32_t* cols, const float* vals, int32_t K) {
     const int32_t Kp = (K + 15) / 16 * 16;
     const int32_t N = net->N;
@@ -61,6 +62,8 @@ struct NetBuilder {
     net->w_off.push_back(w_floats);
     net->ell_K.push_back(Kp);
     net->ell_off.push_back((int64_t)ecols.size());
+    net->erng_off.push_back((int64_t)erng.size());
+    const int32_t nch = ell_chunks(N);
     for (int32_t o = 0; o < N; ++o) outs.push_back(o);  // keeps the [L][N] row-id layout
     std::vector<std::pair<int32_t, float>> tmp(K);
     for (int32_t o = 0; o < N; ++o) {
@@ -78,12 +81,23 @@ struct NetBuilder {
         ecols.push_back(r < K ? tmp[r].first : tmp[K - 1].first);
         evals.push_back(r < K ? tmp[r].second : 0.f);
       }
+      // chunk ranges over the row without its trailing value-0 entries (padding, here or
+      // from a CSR row shorter than the longest); a zero product leaves the chain unchanged
+      int32_t end = K;
+      while (end > 0 && tmp[end - 1].second == 0.f) --end;
+      int32_t q = 0;
+      for (int32_t c = 0; c < nch; ++c) {
+        while (q < end && tmp[q].first < c * kEllChunk) ++q;
+        erng.push_back(q);
+      }
+      erng.push_back(end);
     }
   }
   void add(const HostLayerPanels& hp) {
     const int64_t P = (int64_t)hp.pbeg.size() - 1;
     net->ell_K.push_back(0);
     net->ell_off.push_back((int64_t)ecols.size());
+    net->erng_off.push_back((int64_t)erng.size());
     net->panel_off.push_back(net->panel_off.back() + P);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
@@ -127,10 +141,15 @@ void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
     launch_fill_f32(net->d_w, fill_weight, nb.w_floats, s);
   }
   if (!nb.ecols.empty()) {
-    net->d_ecols = dev_alloc<int32_t>(nb.ecols.size(), &net->device_bytes);
-    net->d_evals = dev_alloc<float>(nb.evals.size(), &net->device_bytes);
+    // 16 entries of slack: the wavefront-per-row kernel loads entries in groups of 8
+    net->d_ecols = dev_alloc<int32_t>(nb.ecols.size() + 16, &net->device_bytes);
+    net->d_evals = dev_alloc<float>(nb.evals.size() + 16, &net->device_bytes);
+    SNICIT_HIP(hipMemset(net->d_ecols + nb.ecols.size(), 0, 16 * 4));
+    SNICIT_HIP(hipMemset(net->d_evals + nb.evals.size(), 0, 16 * 4));
     SNICIT_HIP(hipMemcpy(net->d_ecols, nb.ecols.data(), nb.ecols.size() * 4, hipMemcpyHostToDevice));
     SNICIT_HIP(hipMemcpy(net->d_evals, nb.evals.data(), nb.evals.size() * 4, hipMemcpyHostToDevice));
+    net->d_erng = dev_alloc<int32_t>(nb.erng.size(), &net->device_bytes);
+    SNICIT_HIP(hipMemcpy(net->d_erng, nb.erng.data(), nb.erng.size() * 4, hipMemcpyHostToDevice));
   }
   if (net->bias_kind == SNICIT_BIAS_PER_NEURON) {
     net->d_bias = dev_alloc<float>((size_t)net->L * net->N, &net->device_bytes);
@@ -157,6 +176,7 @@ void net_set_bias(Net* net, int32_t bias_kind, const float* bias) {
 void destroy_net(Net* net) {
   dev_free(net->d_ecols);
   dev_free(net->d_evals);
+  dev_free(net->d_erng);
   dev_free(net->d_win);
   dev_free(net->d_bin);
   dev_free(net->d_wout);
@@ -199,6 +219,7 @@ void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
   e.cols = net.d_ecols + net.ell_off[a.layer];
   e.vals = net.d_evals + net.ell_off[a.layer];
   e.K = net.ell_K[a.layer];
+  e.rng = net.d_erng + net.erng_off[a.layer];
   e.rows = net.N;
   e.in_rows = net.N;
   e.bias_vec = net.bias_vec(a.layer);
@@ -1563,7 +1584,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       double best_ms = res.ms_default;
       // wide-row layers: staged-tile LDS budget (0 = direct gathers), then the grid size
       if (p->net->is_ell(0)) {
-        const int32_t cand[][2] = {{0, 0}, {64, 0}, {128, 0}, {40, 1}, {144, 1}};
+        const int32_t cand[][2] = {{0, 0}, {64, 0}, {128, 0}, {40, 1}, {144, 1}, {80, 2}};
         for (const auto& kc : cand) {
           LaunchShape sh = def;
           sh.ell_lds_kb = kc[0];
@@ -1639,7 +1660,7 @@ int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow
   return guarded([&] {
     SNICIT_REQUIRE(plan != nullptr, "null plan");
     SNICIT_REQUIRE(grid_target >= 1 && narrow >= 0 && ell_lds_kb >= 0 && ell_lds_kb <= 160 &&
-                       (ell_chunked == 0 || ell_chunked == 1),
+                       (ell_chunked >= 0 && ell_chunked <= 2),
                    "bad launch shape");
     plan->shape.grid_target = grid_target;
     plan->shape.narrow = narrow;

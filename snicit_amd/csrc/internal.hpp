@@ -45,6 +45,10 @@ constexpr int kMfmaMinAvgRows = 16;
 constexpr int kMaxCentroidsExact = 32;
 // Centroid capacity of the lossy variant (pool of 128 strided rows).
 constexpr int kMaxCentroidsLossy = 128;
+// Input neurons per LDS chunk of the wavefront-per-row wide-row kernel (one chunk of 64
+// columns is kEllChunk * 256 bytes; two are resident, double-buffered).
+constexpr int kEllChunk = 128;
+inline int32_t ell_chunks(int32_t in_rows) { return (in_rows + kEllChunk - 1) / kEllChunk; }
 
 // A fan-in panel: rows of one layer that read the same kSlots input neurons.
 // Host form of one layer, used by the packer and the generator.
@@ -88,6 +92,11 @@ struct Net {
   std::vector<int64_t> ell_off;    // [L] first entry of the layer in d_ecols / d_evals
   int32_t* d_ecols = nullptr;
   float* d_evals = nullptr;
+  // entry range of each row in each chunk of kEllChunk input neurons: row o's entries that
+  // read inputs [c*kEllChunk, (c+1)*kEllChunk) are [rng[o][c], rng[o][c+1]); rng[o][nch] =
+  // the row's length without its trailing value-0 padding.  [N][nch+1] per wide layer.
+  std::vector<int64_t> erng_off;   // [L] first int of the layer in d_erng
+  int32_t* d_erng = nullptr;
   // optional dense input (Nin -> N, clip in_clip) and output (N -> Nout, linear) layers,
   // src/beyond/general.hpp:5-33,74-102; rows zero-padded to Kin / Kout (multiples of 16)
   int32_t Nin = 0, Kin = 0, Nout = 0, Kout = 0;

@@ -5,6 +5,8 @@
 // contiguous 256-byte run; weights, panel inputs and output ids are wavefront-uniform and
 // are fetched with scalar loads.  The bound is HBM bandwidth (one read and one write of
 // every live activation per layer); there is no MFMA.
+#include <atomic>
+
 #include "kernels.hpp"
 
 namespace snicit {
@@ -686,6 +688,165 @@ __global__ __launch_bounds__(256) void ell_chunk_kernel(
         if (m != 0ull && lane == 0) flags[c0 + c] = 1;
       }
     }
+  }
+}
+
+// Wavefront-per-row wide-row SpMM.  A work item is (64-column tile, block of rows); lane =
+// column, so every LDS read of an input neuron is one conflict-free 256-byte run and the
+// row's weight and input index are wave-uniform (scalar loads, the weight an SGPR operand
+// of v_fma_f32).  The tile's inputs are staged in chunks of kEllChunk neurons, two chunks
+// resident: while the block runs chunk c from one buffer, chunk c+1 is already loaded into
+// registers and is written to the other buffer afterwards, one barrier per chunk.  A row's
+// entries that read chunk c are [rng[o][c], rng[o][c+1]) (ascending input), so the chain
+// pauses at each chunk boundary and resumes with the next chunk: the multiply-add order is
+// that of the oracle.  Rows per wavefront (rw <= kEllRW) come from the live column count,
+// so small batches still spread over every CU.  With at least 8 column tiles, the row
+// blocks of one tile are work items of one XCD (workgroup i runs on XCD i mod 8), so the
+// tile is fetched into that XCD's L2 once.
+constexpr int kEllWaves = 8;
+constexpr int kEllRW = 16;
+constexpr int kEllStageRows = kEllChunk / kEllWaves;  // input rows each wavefront stages
+
+template <int EPI>
+__global__ __launch_bounds__(kEllWaves * 64) void ell_wave_kernel(
+    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
+    int32_t in_rows, const int32_t* __restrict__ rng, const float* __restrict__ bias_vec,
+    float bias_s, float clip, const float* __restrict__ yin, int64_t ldin,
+    float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t min_items) {
+  extern __shared__ float xs[];  // [2][kEllChunk][64]
+  const int64_t count = *count_ptr;
+  const int lane = lane_id();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int nch = (in_rows + kEllChunk - 1) / kEllChunk;
+  const int64_t tiles = (count + 63) / 64;
+  // rows per wavefront: the largest power of two <= kEllRW that still gives min_items
+  int rw = kEllRW;
+  while (rw > 1 && tiles * ((rows + kEllWaves * rw - 1) / (kEllWaves * rw)) < min_items) rw >>= 1;
+  const int64_t rbc = (rows + kEllWaves * rw - 1) / (kEllWaves * rw);
+  const bool xcd_major = tiles >= 8;
+  const int64_t items = (xcd_major ? (tiles + 7) / 8 * 8 : tiles) * rbc;
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    int64_t tile, rb;
+    if (xcd_major) {
+      const int64_t jj = it / 8;
+      tile = jj / rbc * 8 + it % 8;
+      rb = jj % rbc;
+    } else {
+      tile = it / rbc;
+      rb = it % rbc;
+    }
+    if (tile >= tiles) continue;  // uniform over the block
+    const int64_t pos = tile * 64 + lane;
+    const bool valid = pos < count;
+    const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
+    const int row0 = (int)(rb * kEllWaves * rw) + wv * rw;
+    float st[kEllStageRows];
+    auto load_chunk = [&](int c) {
+#pragma unroll
+      for (int q = 0; q < kEllStageRows; ++q) {
+        const int r = c * kEllChunk + q * kEllWaves + wv;
+        st[q] = (valid && r < in_rows) ? yin[(int64_t)r * ldin + src] : 0.f;
+      }
+    };
+    auto store_chunk = [&](int buf) {
+      float* __restrict__ x = xs + buf * (kEllChunk * 64);
+#pragma unroll
+      for (int q = 0; q < kEllStageRows; ++q) x[(q * kEllWaves + wv) * 64 + lane] = st[q];
+    };
+    float acc[kEllRW];
+#pragma unroll
+    for (int i = 0; i < kEllRW; ++i) acc[i] = 0.f;
+    __syncthreads();  // the previous item's last chunk has been read
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      if (c + 1 < nch) load_chunk(c + 1);
+      // xb[col * 64] = input col of this lane's column (chunk base folded in once, so an
+      // entry's LDS address is one shift-add of its input index)
+      const float* __restrict__ xb = xs + (c & 1) * (kEllChunk * 64) + lane - c * (kEllChunk * 64);
+#pragma unroll
+      for (int i = 0; i < kEllRW; ++i) {
+        const int o = row0 + i;
+        if (i < rw && o < rows) {
+          const int32_t* __restrict__ rr = rng + (size_t)o * (nch + 1);
+          int p = rr[c];
+          const int e = rr[c + 1];
+          if (p < e) {
+            // entries in groups of 8, two register sets in turn: the next group's scalar
+            // loads go out before the current group's LDS reads, so one wait covers both
+            // (the entry arrays carry 16 entries of slack, so a group load past the row's
+            // end stays inside the allocation)
+            const int32_t* __restrict__ cr = cols + (size_t)o * K;
+            const float* __restrict__ vr = vals + (size_t)o * K;
+            float a = acc[i];
+            int ca[8], cb[8];
+            float wa[8], wb[8];
+            auto load = [&](int* cc, float* ww, int at) {
+#pragma unroll
+              for (int q = 0; q < 8; ++q) {
+                cc[q] = cr[at + q];
+                ww[q] = vr[at + q];
+              }
+            };
+            auto group = [&](const int* cc, const float* ww) {
+              float xv[8];
+#pragma unroll
+              for (int q = 0; q < 8; ++q) xv[q] = xb[cc[q] * 64];
+#pragma unroll
+              for (int q = 0; q < 8; ++q) a = __builtin_fmaf(ww[q], xv[q], a);
+            };
+            auto tail = [&](const int* cc, const float* ww) {
+              const int rem = e - p;  // 0..7 entries left, already loaded
+              float xv[7];
+#pragma unroll
+              for (int q = 0; q < 7; ++q)
+                if (q < rem) xv[q] = xb[cc[q] * 64];
+#pragma unroll
+              for (int q = 0; q < 7; ++q)
+                if (q < rem) a = __builtin_fmaf(ww[q], xv[q], a);
+            };
+            load(ca, wa, p);
+            while (true) {
+              if (p + 8 > e) {
+                tail(ca, wa);
+                break;
+              }
+              p += 8;
+              load(cb, wb, p);
+              __builtin_amdgcn_sched_barrier(0);
+              group(ca, wa);
+              if (p + 8 > e) {
+                tail(cb, wb);
+                break;
+              }
+              p += 8;
+              load(ca, wa, p);
+              __builtin_amdgcn_sched_barrier(0);
+              group(cb, wb);
+            }
+            acc[i] = a;
+          }
+        }
+      }
+      if (c + 1 < nch) store_chunk((c + 1) & 1);
+      __syncthreads();
+    }
+    int32_t mi = 0;
+    if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = valid ? mcent[pos] : 0;
+    bool flag = false;
+#pragma unroll
+    for (int i = 0; i < kEllRW; ++i) {
+      const int o = row0 + i;
+      if (i < rw && o < rows) {
+        const float b = bias_vec ? bias_vec[o] : bias_s;
+        const float y = epilogue<EPI>(acc[i], b, clip, valid, o, pos, mi, zc, kcap, eps_post, flag);
+        if (valid) yout[(int64_t)o * ldout + pos] = y;
+      }
+    }
+    if (flags && valid && flag) flags[pos] = 1;
   }
 }
 
@@ -1526,9 +1687,51 @@ void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
   check_launch();
 }
 
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  SNICIT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  int cus = cache[dev].load(std::memory_order_relaxed);
+  if (cus <= 0) {
+    SNICIT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev].store(cus, std::memory_order_relaxed);
+  }
+  return cus;
+}
+
+void launch_ell_wave(const EllLaunch& a, hipStream_t s) {
+  const int cus = device_cus();
+  // two 64 KB workgroups per CU; at most as many as there are (tile, 8-row block) items
+  const int64_t tiles = (a.max_cols + 63) / 64;
+  const int64_t most = std::max<int64_t>(1, tiles * ((a.rows + kEllWaves - 1) / kEllWaves));
+  const int64_t resident = 2 * (int64_t)std::max(cus, 1);
+  const unsigned grid = (unsigned)std::min<int64_t>(resident, most);
+  const size_t lds = (size_t)2 * kEllChunk * 64 * sizeof(float);
+#define SNICIT_ELL_WAVE(E)                                                                     \
+  hipLaunchKernelGGL(ell_wave_kernel<E>, dim3(grid), dim3(kEllWaves * 64), lds, s, a.cols,     \
+                     a.vals, a.K, a.rows, a.in_rows, a.rng, a.bias_vec, a.bias_s, a.clip,      \
+                     a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags, a.zc, a.kcap,    \
+                     a.mcent, a.eps_post, (int32_t)resident)
+  switch (a.epilogue) {
+    case EPI_PLAIN: SNICIT_ELL_WAVE(EPI_PLAIN); break;
+    case EPI_CENTROID: SNICIT_ELL_WAVE(EPI_CENTROID); break;
+    case EPI_DELTA: SNICIT_ELL_WAVE(EPI_DELTA); break;
+    case EPI_DELTA_LOSSY: SNICIT_ELL_WAVE(EPI_DELTA_LOSSY); break;
+    case EPI_LINEAR: SNICIT_ELL_WAVE(EPI_LINEAR); break;
+    default: throw Error(SNICIT_E_INVALID, "bad epilogue");
+  }
+#undef SNICIT_ELL_WAVE
+  check_launch();
+}
+
 void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
   SNICIT_REQUIRE(a.in_rows > 0, "ELL launch needs the input row count");
+  if (a.shape.ell_chunked == 2 && a.cols && a.rng) {
+    launch_ell_wave(a, s);
+    return;
+  }
   const int64_t lds = std::min<int64_t>((int64_t)a.shape.ell_lds_kb * 1024, kLdsPerBlock);
   if (a.rows >= 128 && a.shape.ell_chunked) {
     // wide tiles, input rows staged in chunks that fit the budget (multiple of 16 rows)

@@ -34,8 +34,10 @@ struct LaunchShape {
   int32_t narrow = 64;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
-  // 1: wide-row layers with >= 128 rows stage 64-column tiles in chunks of input rows
-  int32_t ell_chunked = 1;
+  // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in
+  // chunks of input rows; 2 = wavefront per row, lanes = 64 columns, input chunks of
+  // kEllChunk rows double-buffered in LDS (layers with a chunk range table); 0 = neither
+  int32_t ell_chunked = 2;
 };
 
 struct SpmmLaunch {
@@ -76,6 +78,7 @@ void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout,
 struct EllLaunch {
   const int32_t* cols;  // [rows][K] or null (column r = input r: dense layer)
   const float* vals;    // [rows][K]
+  const int32_t* rng;   // [rows][ell_chunks(in_rows)+1] chunk entry ranges, or null
   int32_t K, rows;
   int32_t in_rows;      // input neurons the entries refer to (rows of yin read)
   const float* bias_vec;
