@@ -261,7 +261,17 @@ def bench_serve(args, wl):
             _, info = srv.wait(rid, 600000.0)
             lat.append(info["ms_total"])
             edges += float(B) * edges_per[net]
-        return np.array(lat), edges, time.perf_counter() - start
+        lat = np.array(lat)
+        nets_of = np.array([n for _, n, _ in ids])
+        sizes_of = np.array([b for _, _, b in ids])
+        for n in (0, 1):
+            sel = lat[nets_of == n]
+            if sel.size:
+                big = lat[(nets_of == n) & (sizes_of >= 1024)]
+                log(f"  net {n}: {sel.size} requests, p50 {np.percentile(sel, 50):.1f} ms, "
+                    f"p99 {np.percentile(sel, 99):.1f} ms; >=1024 samples: {big.size} requests"
+                    + (f", p50 {np.percentile(big, 50):.1f} ms" if big.size else ""))
+        return lat, edges, time.perf_counter() - start
 
     run(draw(16), None)  # warm-up
     lat, edges, wall = run(draw(wl["burst"]), None)

@@ -53,6 +53,7 @@ struct NetBuilder {
   std::vector<int32_t> ecols;
   std::vector<float> evals;
   std::vector<int32_t> erng;
+  std::vector<int2> eent;
   void add_ell(const int32_t* cols, const float* vals, int32_t K) {
     const int32_t Kp = (K + 15) / 16 * 16;
     const int32_t N = net->N;
@@ -63,6 +64,8 @@ struct NetBuilder {
     net->ell_K.push_back(Kp);
     net->ell_off.push_back((int64_t)ecols.size());
     net->erng_off.push_back((int64_t)erng.size());
+    net->eent_off.push_back((int64_t)eent.size());
+    const int64_t stream0 = (int64_t)eent.size();
     const int32_t nch = ell_chunks(N);
     for (int32_t o = 0; o < N; ++o) outs.push_back(o);  // keeps the [L][N] row-id layout
     std::vector<std::pair<int32_t, float>> tmp(K);
@@ -81,16 +84,27 @@ struct NetBuilder {
         ecols.push_back(r < K ? tmp[r].first : tmp[K - 1].first);
         evals.push_back(r < K ? tmp[r].second : 0.f);
       }
-      // chunk ranges over the row without its trailing value-0 entries (padding, here or
-      // from a CSR row shorter than the longest); a zero product leaves the chain unchanged
+      // the row without its trailing value-0 entries (padding, here or from a CSR row
+      // shorter than the longest), chunk by chunk, each chunk's run padded to whole groups
       int32_t end = K;
       while (end > 0 && tmp[end - 1].second == 0.f) --end;
       int32_t q = 0;
       for (int32_t c = 0; c < nch; ++c) {
-        while (q < end && tmp[q].first < c * kEllChunk) ++q;
-        erng.push_back(q);
+        const int64_t at = (int64_t)eent.size() - stream0;
+        SNICIT_REQUIRE(at < (int64_t(1) << 31), "wide layer too large for its entry stream");
+        erng.push_back((int32_t)at);
+        int32_t last = -1;
+        int32_t n = 0;
+        for (; q < end && tmp[q].first < (c + 1) * kEllChunk; ++q, ++n) {
+          int2 e;
+          e.x = tmp[q].first - c * kEllChunk;
+          std::memcpy(&e.y, &tmp[q].second, 4);
+          eent.push_back(e);
+          last = e.x;
+        }
+        for (; n % kEllGroup != 0; ++n) eent.push_back(int2{last, 0});
       }
-      erng.push_back(end);
+      erng.push_back((int32_t)((int64_t)eent.size() - stream0));
     }
   }
   void add(const HostLayerPanels& hp) {
@@ -98,6 +112,7 @@ struct NetBuilder {
     net->ell_K.push_back(0);
     net->ell_off.push_back((int64_t)ecols.size());
     net->erng_off.push_back((int64_t)erng.size());
+    net->eent_off.push_back((int64_t)eent.size());
     net->panel_off.push_back(net->panel_off.back() + P);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
@@ -141,13 +156,14 @@ void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
     launch_fill_f32(net->d_w, fill_weight, nb.w_floats, s);
   }
   if (!nb.ecols.empty()) {
-    // 16 entries of slack: the wavefront-per-row kernel loads entries in groups of 8
-    net->d_ecols = dev_alloc<int32_t>(nb.ecols.size() + 16, &net->device_bytes);
-    net->d_evals = dev_alloc<float>(nb.evals.size() + 16, &net->device_bytes);
-    SNICIT_HIP(hipMemset(net->d_ecols + nb.ecols.size(), 0, 16 * 4));
-    SNICIT_HIP(hipMemset(net->d_evals + nb.evals.size(), 0, 16 * 4));
+    net->d_ecols = dev_alloc<int32_t>(nb.ecols.size(), &net->device_bytes);
+    net->d_evals = dev_alloc<float>(nb.evals.size(), &net->device_bytes);
     SNICIT_HIP(hipMemcpy(net->d_ecols, nb.ecols.data(), nb.ecols.size() * 4, hipMemcpyHostToDevice));
     SNICIT_HIP(hipMemcpy(net->d_evals, nb.evals.data(), nb.evals.size() * 4, hipMemcpyHostToDevice));
+    // one group of slack: the kernel loads the group after a row's last one
+    net->d_eent = dev_alloc<int2>(nb.eent.size() + kEllGroup, &net->device_bytes);
+    SNICIT_HIP(hipMemcpy(net->d_eent, nb.eent.data(), nb.eent.size() * 8, hipMemcpyHostToDevice));
+    SNICIT_HIP(hipMemset(net->d_eent + nb.eent.size(), 0, kEllGroup * 8));
     net->d_erng = dev_alloc<int32_t>(nb.erng.size(), &net->device_bytes);
     SNICIT_HIP(hipMemcpy(net->d_erng, nb.erng.data(), nb.erng.size() * 4, hipMemcpyHostToDevice));
   }
@@ -177,6 +193,7 @@ void destroy_net(Net* net) {
   dev_free(net->d_ecols);
   dev_free(net->d_evals);
   dev_free(net->d_erng);
+  dev_free(net->d_eent);
   dev_free(net->d_win);
   dev_free(net->d_bin);
   dev_free(net->d_wout);
@@ -220,6 +237,7 @@ void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
   e.vals = net.d_evals + net.ell_off[a.layer];
   e.K = net.ell_K[a.layer];
   e.rng = net.d_erng + net.erng_off[a.layer];
+  e.ent = net.d_eent + net.eent_off[a.layer];
   e.rows = net.N;
   e.in_rows = net.N;
   e.bias_vec = net.bias_vec(a.layer);

@@ -45,9 +45,11 @@ constexpr int kMfmaMinAvgRows = 16;
 constexpr int kMaxCentroidsExact = 32;
 // Centroid capacity of the lossy variant (pool of 128 strided rows).
 constexpr int kMaxCentroidsLossy = 128;
-// Input neurons per LDS chunk of the wavefront-per-row wide-row kernel (one chunk of 64
-// columns is kEllChunk * 256 bytes; two are resident, double-buffered).
-constexpr int kEllChunk = 128;
+// Input neurons per LDS chunk of the wavefront-per-row wide-row kernel (one chunk of a
+// 64 x w column tile is kEllChunk * 256 * w bytes; two are resident, double-buffered).
+constexpr int kEllChunk = 64;
+// Entries per group of the wavefront-per-row kernel's stream (one 32-byte load).
+constexpr int kEllGroup = 4;
 inline int32_t ell_chunks(int32_t in_rows) { return (in_rows + kEllChunk - 1) / kEllChunk; }
 
 // A fan-in panel: rows of one layer that read the same kSlots input neurons.
@@ -92,11 +94,16 @@ struct Net {
   std::vector<int64_t> ell_off;    // [L] first entry of the layer in d_ecols / d_evals
   int32_t* d_ecols = nullptr;
   float* d_evals = nullptr;
-  // entry range of each row in each chunk of kEllChunk input neurons: row o's entries that
-  // read inputs [c*kEllChunk, (c+1)*kEllChunk) are [rng[o][c], rng[o][c+1]); rng[o][nch] =
-  // the row's length without its trailing value-0 padding.  [N][nch+1] per wide layer.
+  // The same rows as a stream of (input neuron - first neuron of its chunk, weight bits)
+  // pairs for the wavefront-per-row kernel: per row, per chunk of kEllChunk
+  // input neurons, the row's entries of that chunk in ascending input order, padded to a
+  // multiple of kEllGroup with value-0 entries that repeat the chunk's last real input (a
+  // zero product leaves the chain unchanged).  Row o's entries of chunk c are
+  // [rng[o][c], rng[o][c+1]) of the layer's stream; [N][nch+1] per wide layer.
   std::vector<int64_t> erng_off;   // [L] first int of the layer in d_erng
+  std::vector<int64_t> eent_off;   // [L] first pair of the layer in d_eent
   int32_t* d_erng = nullptr;
+  int2* d_eent = nullptr;
   // optional dense input (Nin -> N, clip in_clip) and output (N -> Nout, linear) layers,
   // src/beyond/general.hpp:5-33,74-102; rows zero-padded to Kin / Kout (multiples of 16)
   int32_t Nin = 0, Kin = 0, Nout = 0, Kout = 0;

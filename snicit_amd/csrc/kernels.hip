@@ -691,36 +691,49 @@ __global__ __launch_bounds__(256) void ell_chunk_kernel(
   }
 }
 
-// Wavefront-per-row wide-row SpMM.  A work item is (64-column tile, block of rows); lane =
-// column, so every LDS read of an input neuron is one conflict-free 256-byte run and the
-// row's weight and input index are wave-uniform (scalar loads, the weight an SGPR operand
-// of v_fma_f32).  The tile's inputs are staged in chunks of kEllChunk neurons, two chunks
-// resident: while the block runs chunk c from one buffer, chunk c+1 is already loaded into
-// registers and is written to the other buffer afterwards, one barrier per chunk.  A row's
-// entries that read chunk c are [rng[o][c], rng[o][c+1]) (ascending input), so the chain
-// pauses at each chunk boundary and resumes with the next chunk: the multiply-add order is
-// that of the oracle.  Rows per wavefront (rw <= kEllRW) come from the live column count,
-// so small batches still spread over every CU.  With at least 8 column tiles, the row
-// blocks of one tile are work items of one XCD (workgroup i runs on XCD i mod 8), so the
-// tile is fetched into that XCD's L2 once.
-constexpr int kEllWaves = 8;
-constexpr int kEllRW = 16;
+// Wavefront-per-row wide-row SpMM.  A work item is (column tile, block of rows): the tile is
+// 64*W columns, lane l owning columns W*l .. W*l+W-1, so every LDS read of an input neuron
+// is one conflict-free run (W floats per lane, one ds_read_b32 or ds_read_b128).  Each entry
+// is wave-uniform: an (input neuron - first neuron of its chunk, weight) pair of the layer's
+// padded stream, read kEllGroup at a time and used for W columns per lane, so the cost of
+// broadcasting it is shared by 64*W multiply-adds.  The tile's inputs are staged in chunks
+// of kEllChunk neurons, two chunks resident: while the block runs chunk c from one buffer,
+// chunk c+1 is already loaded into registers and is written to the other buffer afterwards,
+// one barrier per chunk.  A row's entries that read chunk c are [rng[o][c], rng[o][c+1])
+// of the stream (ascending input, padded to whole groups with value-0 entries), so the chain
+// of every column pauses at each chunk boundary and resumes with the next chunk: the
+// multiply-add order is that of the oracle.  Rows per wavefront (rw <= kEllRW) come from the
+// live column count, so small batches still spread over every CU.  With at least 8 column
+// tiles, the row blocks of one tile are work items of one XCD (workgroup i runs on XCD
+// i mod 8), so the tile is fetched into that XCD's L2 once.
+constexpr int kEllWaves = 16;
+constexpr int kEllRW = 8;
 constexpr int kEllStageRows = kEllChunk / kEllWaves;  // input rows each wavefront stages
 
-template <int EPI>
+template <int W> struct ColVec { typedef float4 type; };
+template <> struct ColVec<1> { typedef float type; };
+__device__ __forceinline__ float col_of(float v, int) { return v; }
+__device__ __forceinline__ float col_of(const float4& v, int j) {
+  return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+}
+
+template <int EPI, int W>
 __global__ __launch_bounds__(kEllWaves * 64) void ell_wave_kernel(
-    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
-    int32_t in_rows, const int32_t* __restrict__ rng, const float* __restrict__ bias_vec,
-    float bias_s, float clip, const float* __restrict__ yin, int64_t ldin,
-    float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
-    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
-    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t min_items) {
-  extern __shared__ float xs[];  // [2][kEllChunk][64]
+    const int2* __restrict__ ent, int32_t rows, int32_t in_rows,
+    const int32_t* __restrict__ rng, const float* __restrict__ bias_vec, float bias_s,
+    float clip, const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout,
+    int64_t ldout, const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
+    int32_t* __restrict__ flags, float* __restrict__ zc, int32_t kcap,
+    const int32_t* __restrict__ mcent, float eps_post, int32_t min_items) {
+  static_assert(W == 1 || W == 4, "1 or 4 columns per lane");
+  typedef typename ColVec<W>::type Vec;
+  constexpr int TC = 64 * W;  // columns per tile
+  extern __shared__ float xs[];  // [2][kEllChunk][TC]
   const int64_t count = *count_ptr;
   const int lane = lane_id();
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int nch = (in_rows + kEllChunk - 1) / kEllChunk;
-  const int64_t tiles = (count + 63) / 64;
+  const int64_t tiles = (count + TC - 1) / TC;
   // rows per wavefront: the largest power of two <= kEllRW that still gives min_items
   int rw = kEllRW;
   while (rw > 1 && tiles * ((rows + kEllWaves * rw - 1) / (kEllWaves * rw)) < min_items) rw >>= 1;
@@ -738,115 +751,117 @@ __global__ __launch_bounds__(kEllWaves * 64) void ell_wave_kernel(
       rb = it % rbc;
     }
     if (tile >= tiles) continue;  // uniform over the block
-    const int64_t pos = tile * 64 + lane;
-    const bool valid = pos < count;
-    const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
+    int64_t pos[W], src[W];
+    bool valid[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      pos[j] = tile * TC + lane * W + j;
+      valid[j] = pos[j] < count;
+      src[j] = valid[j] ? (map ? (int64_t)map[pos[j]] : pos[j]) : 0;
+    }
     const int row0 = (int)(rb * kEllWaves * rw) + wv * rw;
-    float st[kEllStageRows];
+    float st[kEllStageRows][W];
     auto load_chunk = [&](int c) {
 #pragma unroll
       for (int q = 0; q < kEllStageRows; ++q) {
         const int r = c * kEllChunk + q * kEllWaves + wv;
-        st[q] = (valid && r < in_rows) ? yin[(int64_t)r * ldin + src] : 0.f;
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          st[q][j] = (valid[j] && r < in_rows) ? yin[(int64_t)r * ldin + src[j]] : 0.f;
       }
     };
     auto store_chunk = [&](int buf) {
-      float* __restrict__ x = xs + buf * (kEllChunk * 64);
 #pragma unroll
-      for (int q = 0; q < kEllStageRows; ++q) x[(q * kEllWaves + wv) * 64 + lane] = st[q];
+      for (int q = 0; q < kEllStageRows; ++q) {
+        float* __restrict__ x = xs + buf * (kEllChunk * TC) + (q * kEllWaves + wv) * TC + lane * W;
+        if constexpr (W == 4) *reinterpret_cast<float4*>(x) = make_float4(st[q][0], st[q][1], st[q][2], st[q][3]);
+        else x[0] = st[q][0];
+      }
     };
-    float acc[kEllRW];
+    float acc[kEllRW][W];
 #pragma unroll
-    for (int i = 0; i < kEllRW; ++i) acc[i] = 0.f;
-    __syncthreads();  // the previous item's last chunk has been read
+    for (int i = 0; i < kEllRW; ++i)
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[i][j] = 0.f;
     load_chunk(0);
     store_chunk(0);
     __syncthreads();
     for (int c = 0; c < nch; ++c) {
       if (c + 1 < nch) load_chunk(c + 1);
-      // xb[col * 64] = input col of this lane's column (chunk base folded in once, so an
-      // entry's LDS address is one shift-add of its input index)
-      const float* __restrict__ xb = xs + (c & 1) * (kEllChunk * 64) + lane - c * (kEllChunk * 64);
+      // this lane's columns in the chunk's buffer; an entry adds its input row's offset
+      const char* __restrict__ xb =
+          reinterpret_cast<const char*>(xs + (c & 1) * (kEllChunk * TC) + lane * W);
 #pragma unroll
       for (int i = 0; i < kEllRW; ++i) {
-        const int o = row0 + i;
+        int o = row0 + i;
+        // recomputed per chunk: hoisting every row's pointers out of the chunk loop would
+        // crowd the scalar registers (empty asm, no instruction)
+        asm volatile("" : "+s"(o));
         if (i < rw && o < rows) {
           const int32_t* __restrict__ rr = rng + (size_t)o * (nch + 1);
           int p = rr[c];
           const int e = rr[c + 1];
           if (p < e) {
-            // entries in groups of 8, two register sets in turn: the next group's scalar
-            // loads go out before the current group's LDS reads, so one wait covers both
-            // (the entry arrays carry 16 entries of slack, so a group load past the row's
-            // end stays inside the allocation)
-            const int32_t* __restrict__ cr = cols + (size_t)o * K;
-            const float* __restrict__ vr = vals + (size_t)o * K;
-            float a = acc[i];
-            int ca[8], cb[8];
-            float wa[8], wb[8];
-            auto load = [&](int* cc, float* ww, int at) {
+            // whole groups (runs are padded), two register sets in turn: the next group's
+            // load goes out before the current group's LDS reads (the stream carries a
+            // group of slack past its end)
+            float a[W];
 #pragma unroll
-              for (int q = 0; q < 8; ++q) {
-                cc[q] = cr[at + q];
-                ww[q] = vr[at + q];
-              }
+            for (int j = 0; j < W; ++j) a[j] = acc[i][j];
+            int2 ga[kEllGroup], gb[kEllGroup];
+            auto load = [&](int2* g, int at) {
+#pragma unroll
+              for (int q = 0; q < kEllGroup; ++q) g[q] = ent[at + q];
             };
-            auto group = [&](const int* cc, const float* ww) {
-              float xv[8];
+            auto group = [&](const int2* g) {
+              Vec xv[kEllGroup];
 #pragma unroll
-              for (int q = 0; q < 8; ++q) xv[q] = xb[cc[q] * 64];
+              for (int q = 0; q < kEllGroup; ++q)
+                xv[q] = *reinterpret_cast<const Vec*>(xb + g[q].x * (TC * (int)sizeof(float)));
 #pragma unroll
-              for (int q = 0; q < 8; ++q) a = __builtin_fmaf(ww[q], xv[q], a);
+              for (int q = 0; q < kEllGroup; ++q)
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                  a[j] = __builtin_fmaf(__int_as_float(g[q].y), col_of(xv[q], j), a[j]);
             };
-            auto tail = [&](const int* cc, const float* ww) {
-              const int rem = e - p;  // 0..7 entries left, already loaded
-              float xv[7];
-#pragma unroll
-              for (int q = 0; q < 7; ++q)
-                if (q < rem) xv[q] = xb[cc[q] * 64];
-#pragma unroll
-              for (int q = 0; q < 7; ++q)
-                if (q < rem) a = __builtin_fmaf(ww[q], xv[q], a);
-            };
-            load(ca, wa, p);
+            load(ga, p);
             while (true) {
-              if (p + 8 > e) {
-                tail(ca, wa);
-                break;
-              }
-              p += 8;
-              load(cb, wb, p);
+              p += kEllGroup;
+              load(gb, p);
               __builtin_amdgcn_sched_barrier(0);
-              group(ca, wa);
-              if (p + 8 > e) {
-                tail(cb, wb);
-                break;
-              }
-              p += 8;
-              load(ca, wa, p);
+              group(ga);
+              if (p >= e) break;
+              p += kEllGroup;
+              load(ga, p);
               __builtin_amdgcn_sched_barrier(0);
-              group(cb, wb);
+              group(gb);
+              if (p >= e) break;
             }
-            acc[i] = a;
+#pragma unroll
+            for (int j = 0; j < W; ++j) acc[i][j] = a[j];
           }
         }
       }
       if (c + 1 < nch) store_chunk((c + 1) & 1);
       __syncthreads();
     }
-    int32_t mi = 0;
-    if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = valid ? mcent[pos] : 0;
-    bool flag = false;
 #pragma unroll
-    for (int i = 0; i < kEllRW; ++i) {
-      const int o = row0 + i;
-      if (i < rw && o < rows) {
-        const float b = bias_vec ? bias_vec[o] : bias_s;
-        const float y = epilogue<EPI>(acc[i], b, clip, valid, o, pos, mi, zc, kcap, eps_post, flag);
-        if (valid) yout[(int64_t)o * ldout + pos] = y;
+    for (int j = 0; j < W; ++j) {
+      int32_t mi = 0;
+      if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = valid[j] ? mcent[pos[j]] : 0;
+      bool flag = false;
+#pragma unroll
+      for (int i = 0; i < kEllRW; ++i) {
+        const int o = row0 + i;
+        if (i < rw && o < rows) {
+          const float b = bias_vec ? bias_vec[o] : bias_s;
+          const float y =
+              epilogue<EPI>(acc[i][j], b, clip, valid[j], o, pos[j], mi, zc, kcap, eps_post, flag);
+          if (valid[j]) yout[(int64_t)o * ldout + pos[j]] = y;
+        }
       }
+      if (flags && valid[j] && flag) flags[pos[j]] = 1;
     }
-    if (flags && valid && flag) flags[pos] = 1;
   }
 }
 
@@ -929,8 +944,10 @@ __global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict
 // Pruning (y_star_gen, src/beyond/SNICIT/kernel.hpp:19-41): in pool order, every present
 // base i removes each other present row y with #{n : |P_y[n] - P_i[n]| > eps} < N*eps.
 // One workgroup per batch segment; wavefront w tests rows w, w+16, ...; lanes stride the
-// neurons.  Centroids (columns) go to cent_col[s*stride ...]; kcols (zero on entry) becomes
-// the number of centroid columns in use.
+// neurons, 256 per step.  The count only grows, so a pair stops as soon as it reaches N*eps
+// (the row stays): most pairs of different samples are settled within the first steps.
+// Centroids (columns) go to cent_col[s*stride ...]; kcols (zero on entry) becomes the
+// number of centroid columns in use.
 __global__ __launch_bounds__(1024) void lossy_select_kernel(
     const float* __restrict__ poolrows, const int32_t* __restrict__ poolcol,
     const int32_t* __restrict__ seglo, int32_t nseg, int32_t pool, int32_t N, float eps,
@@ -951,10 +968,13 @@ __global__ __launch_bounds__(1024) void lossy_select_kernel(
       if (y == i || !alive[y]) continue;
       const float* __restrict__ py = rows + (int64_t)y * N;
       int cnt = 0;
-      for (int n0 = 0; n0 < N; n0 += kWave) {
-        const int n = n0 + lane;
-        const bool d = n < N && fabsf(py[n] - pi[n]) > eps;
-        cnt += __popcll(__ballot(d));
+      for (int n0 = 0; n0 < N && (double)cnt < limit; n0 += 4 * kWave) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int n = n0 + u * kWave + lane;
+          const bool d = n < N && fabsf(py[n] - pi[n]) > eps;
+          cnt += __popcll(__ballot(d));
+        }
       }
       if ((double)cnt < limit && lane == 0) alive[y] = 0;
     }
@@ -1700,19 +1720,22 @@ int device_cus() {
   return cus;
 }
 
-void launch_ell_wave(const EllLaunch& a, hipStream_t s) {
+template <int W>
+void launch_ell_wave_w(const EllLaunch& a, hipStream_t s) {
   const int cus = device_cus();
-  // two 64 KB workgroups per CU; at most as many as there are (tile, 8-row block) items
-  const int64_t tiles = (a.max_cols + 63) / 64;
+  // LDS per workgroup: two chunks of a 64*W column tile (32 KB for W = 1, 128 KB for
+  // W = 4): two workgroups per CU, or one
+  const int per_cu = W == 1 ? 2 : 1;
+  const int64_t tiles = (a.max_cols + 64 * W - 1) / (64 * W);
   const int64_t most = std::max<int64_t>(1, tiles * ((a.rows + kEllWaves - 1) / kEllWaves));
-  const int64_t resident = 2 * (int64_t)std::max(cus, 1);
+  const int64_t resident = (int64_t)per_cu * std::max(cus, 1);
   const unsigned grid = (unsigned)std::min<int64_t>(resident, most);
-  const size_t lds = (size_t)2 * kEllChunk * 64 * sizeof(float);
+  const size_t lds = (size_t)2 * kEllChunk * 64 * W * sizeof(float);
 #define SNICIT_ELL_WAVE(E)                                                                     \
-  hipLaunchKernelGGL(ell_wave_kernel<E>, dim3(grid), dim3(kEllWaves * 64), lds, s, a.cols,     \
-                     a.vals, a.K, a.rows, a.in_rows, a.rng, a.bias_vec, a.bias_s, a.clip,      \
-                     a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags, a.zc, a.kcap,    \
-                     a.mcent, a.eps_post, (int32_t)resident)
+  hipLaunchKernelGGL((ell_wave_kernel<E, W>), dim3(grid), dim3(kEllWaves * 64), lds, s, a.ent, \
+                     a.rows, a.in_rows, a.rng, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin,     \
+                     a.yout, a.ldout, a.count, a.map, a.flags, a.zc, a.kcap, a.mcent,           \
+                     a.eps_post, (int32_t)resident)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_ELL_WAVE(EPI_PLAIN); break;
     case EPI_CENTROID: SNICIT_ELL_WAVE(EPI_CENTROID); break;
@@ -1725,10 +1748,17 @@ void launch_ell_wave(const EllLaunch& a, hipStream_t s) {
   check_launch();
 }
 
+// 256-column tiles (four columns per lane) once the plan holds at least 4096 columns,
+// 64-column tiles below that
+void launch_ell_wave(const EllLaunch& a, hipStream_t s) {
+  if (a.max_cols >= 4096) launch_ell_wave_w<4>(a, s);
+  else launch_ell_wave_w<1>(a, s);
+}
+
 void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
   SNICIT_REQUIRE(a.in_rows > 0, "ELL launch needs the input row count");
-  if (a.shape.ell_chunked == 2 && a.cols && a.rng) {
+  if (a.shape.ell_chunked == 2 && a.cols && a.rng && a.ent) {
     launch_ell_wave(a, s);
     return;
   }

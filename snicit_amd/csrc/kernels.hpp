@@ -78,7 +78,8 @@ void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout,
 struct EllLaunch {
   const int32_t* cols;  // [rows][K] or null (column r = input r: dense layer)
   const float* vals;    // [rows][K]
-  const int32_t* rng;   // [rows][ell_chunks(in_rows)+1] chunk entry ranges, or null
+  const int32_t* rng;   // [rows][ell_chunks(in_rows)+1] ranges in ent, or null
+  const int2* ent;      // padded (chunk tile byte offset, weight bits) stream, with rng
   int32_t K, rows;
   int32_t in_rows;      // input neurons the entries refer to (rows of yin read)
   const float* bias_vec;
