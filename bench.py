@@ -27,7 +27,8 @@ WORKLOADS = {
                A=dict(N=4096, L=480, bias=-0.35, t=30),
                B=dict(Nin=784, N=1024, L=24, K=103, Nout=10, clip=1.0, t=8)),
     # Poisson request stream against two resident nets
-    "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=240,
+    "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=600,
+               slots=3, window_ms=2.0,
                A=dict(N=16384, L=120, bias=-0.4, t=30),
                B=dict(Nin=3072, N=2048, L=16, K=307, Nout=10, clip=1.0, t=8)),
 }
@@ -225,7 +226,8 @@ def bench_serve(args, wl):
             sa.default_cfg("SNICIT", threshold=Bn["t"], cluster=sa.CLUSTER_LOSSY)]
     t0 = time.perf_counter()
     srv = sa.Server(ctx, [net_a, net_b], cfgs, slot_samples=8192, slot_segments=64,
-                    max_slots_per_net=3, group_window_ms=2.0, hbm_budget_bytes=64 << 30)
+                    max_slots_per_net=wl["slots"], group_window_ms=wl["window_ms"],
+                    hbm_budget_bytes=64 << 30)
     log(f"server: {srv.stats()} ({time.perf_counter() - t0:.1f} s to allocate and capture)")
     pool = prototype_features(Bn["Nin"], 8192, 99)
     edges_per = {0: A["N"] * 32 * A["L"],
@@ -309,7 +311,8 @@ def bench_serve(args, wl):
                                f"(log-uniform) over net A (N={A['N']} L={A['L']} "
                                f"bias={A['bias']} t={A['t']}, exact) and net B ({Bn['Nin']}->"
                                f"{Bn['N']}x{Bn['L']}->{Bn['Nout']}, {Bn['K']} nnz/row, t={Bn['t']}, "
-                               "lossy); slots of 8192 samples / 64 requests, 3 per net",
+                               f"lossy); slots of 8192 samples / 64 requests, {wl['slots']} "
+                               f"per net, grouping window {wl['window_ms']} ms",
                    "server": st},
         "serving": table,
         "roofline": None,

@@ -85,22 +85,27 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   const int beg = pbeg[p], end = pbeg[p + 1];
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
   if (count <= narrow) {
-    if (blockIdx.y != 0) return;
+    // (row, column) pairs over every workgroup of the panel; each pair's 32 inputs are
+    // loaded together, then the chain runs
     const int G = end - beg;
     const int64_t pairs = (int64_t)G * count;
-    for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
+    for (int64_t q = (int64_t)blockIdx.y * kBlock + threadIdx.x; q < pairs;
+         q += (int64_t)gridDim.y * kBlock) {
       const int j = beg + (int)(q % G);
       const int64_t pos = q / G;
       const int64_t src = map ? (int64_t)map[pos] : pos;
       const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
+      float x[kSlots];
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) x[r] = yin[(int64_t)pc[r] * ldin + src];
       float acc = 0.f;
-#pragma unroll 2
+#pragma unroll
       for (int r4 = 0; r4 < kSlots / 4; ++r4) {
         const float4 wv = w4[r4];
-        acc = __builtin_fmaf(wv.x, yin[(int64_t)pc[4 * r4 + 0] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.y, yin[(int64_t)pc[4 * r4 + 1] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.z, yin[(int64_t)pc[4 * r4 + 2] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.w, yin[(int64_t)pc[4 * r4 + 3] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.x, x[4 * r4 + 0], acc);
+        acc = __builtin_fmaf(wv.y, x[4 * r4 + 1], acc);
+        acc = __builtin_fmaf(wv.z, x[4 * r4 + 2], acc);
+        acc = __builtin_fmaf(wv.w, x[4 * r4 + 3], acc);
       }
       const int o = outs[j];
       const float b = bias_vec ? bias_vec[o] : bias_s;
@@ -220,7 +225,12 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   __shared__ int32_t o_s[kMaxPanelRows];
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
-  if (count <= narrow ? blockIdx.y != 0 : (int64_t)blockIdx.y * kBlock >= count) return;
+  const int beg0 = pbeg[p], G0 = pbeg[p + 1] - beg0;
+  // narrow: every workgroup of the panel takes (row, column) pairs; wide: 256 columns each.
+  // Workgroup row 0 always runs: it writes the centroid rows, delta rows or not.
+  if (blockIdx.y != 0 && (count <= narrow ? (int64_t)blockIdx.y * kBlock >= (int64_t)G0 * count
+                                          : (int64_t)blockIdx.y * kBlock >= count))
+    return;
   const int k = *k_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
@@ -258,18 +268,22 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   };
   if (count <= narrow) {
     const int64_t pairs = (int64_t)G * count;
-    for (int64_t q = threadIdx.x; q < pairs; q += kBlock) {
+    for (int64_t q = (int64_t)blockIdx.y * kBlock + threadIdx.x; q < pairs;
+         q += (int64_t)gridDim.y * kBlock) {
       const int jj = (int)(q % G);
       const int64_t pos = q / G;
       const int64_t src = map ? (int64_t)map[pos] : pos;
+      float xn[kSlots];
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) xn[r] = din[(int64_t)pc[r] * ldin + src];
       float acc = 0.f;
-#pragma unroll 2
+#pragma unroll
       for (int r4 = 0; r4 < kSlots / 4; ++r4) {
         const float4 wv = w_s[jj * (kSlots / 4) + r4];
-        acc = __builtin_fmaf(wv.x, din[(int64_t)pc[4 * r4 + 0] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.y, din[(int64_t)pc[4 * r4 + 1] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.z, din[(int64_t)pc[4 * r4 + 2] * ldin + src], acc);
-        acc = __builtin_fmaf(wv.w, din[(int64_t)pc[4 * r4 + 3] * ldin + src], acc);
+        acc = __builtin_fmaf(wv.x, xn[4 * r4 + 0], acc);
+        acc = __builtin_fmaf(wv.y, xn[4 * r4 + 1], acc);
+        acc = __builtin_fmaf(wv.z, xn[4 * r4 + 2], acc);
+        acc = __builtin_fmaf(wv.w, xn[4 * r4 + 3], acc);
       }
       const float y = residue(acc, zc_s[jj * kC + mcent[pos]]);
       dout[(int64_t)o_s[jj] * ldout + pos] = y;
@@ -613,11 +627,45 @@ __global__ __launch_bounds__(256) void ell_chunk_kernel(
     const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout, int64_t ldout,
     const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
     int32_t* __restrict__ flags, float* __restrict__ zc, int32_t kcap,
-    const int32_t* __restrict__ mcent, float eps_post) {
+    const int32_t* __restrict__ mcent, float eps_post, int32_t narrow) {
   constexpr int TC = 64, XS = TC + 4;
   extern __shared__ float x_s[];  // [rc][XS]
   const int64_t count = *count_ptr;
   const int lane = lane_id();
+  if (count <= narrow) {
+    // Few columns (a small serving group): the row-per-thread form would keep only the
+    // first column tile's workgroups busy, each running 64 column chains for a handful of
+    // live columns.  Instead every workgroup of the grid takes (row, column) chains, the
+    // lanes of a wavefront along the columns of one row and on to the next row, reading
+    // inputs and weights from L2.  Same ascending chain and epilogue per (row, column).
+    const int64_t pairs = (int64_t)rows * count;
+    const int64_t nthreads = (int64_t)gridDim.x * gridDim.y * 256;
+    for (int64_t q = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+         q < pairs; q += nthreads) {
+      const int oo = (int)(q / count);
+      const int64_t pos = q % count;
+      const int64_t src = map ? (int64_t)map[pos] : pos;
+      const float* __restrict__ v = vals + (size_t)oo * K;
+      const int32_t* __restrict__ c = cols ? cols + (size_t)oo * K : nullptr;
+      float acc = 0.f;
+      int r = 0;
+      for (; r + 8 <= K; r += 8) {
+        float x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = yin[(int64_t)(c ? c[r + u] : r + u) * ldin + src];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = __builtin_fmaf(v[r + u], x[u], acc);
+      }
+      for (; r < K; ++r) acc = __builtin_fmaf(v[r], yin[(int64_t)(c ? c[r] : r) * ldin + src], acc);
+      const float b = bias_vec ? bias_vec[oo] : bias_s;
+      const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
+      bool flag = false;
+      const float y = epilogue<EPI>(acc, b, clip, true, oo, pos, mi, zc, kcap, eps_post, flag);
+      yout[(int64_t)oo * ldout + pos] = y;
+      if (flags && flag) flags[pos] = 1;
+    }
+    return;
+  }
   const int o = blockIdx.x * 256 + threadIdx.x;
   const bool row_ok = o < rows;
   const float* __restrict__ vr = vals + (size_t)(row_ok ? o : 0) * K;
@@ -1542,6 +1590,18 @@ __global__ __launch_bounds__(256) void scatter_delta_kernel(
     Y[(int64_t)n * ld + dst] = D[(int64_t)n * ldD + col] + C[(int64_t)n * kcap + c];
 }
 
+// Workgroups along the column-tile axis of a (row group x column tile) grid: grid_target /
+// groups, at most one per tile, and at most one per kMinTilesPerBlock tiles of the plan's
+// capacity.  Launch sizes follow the capacity, not the live count, so a plan whose
+// capacity is only a few tiles deep (a serving slot) would otherwise launch mostly
+// workgroups that find no columns; each workgroup loops over its tiles.
+constexpr int64_t kMinTilesPerBlock = 4;
+inline int64_t grid_rows(int64_t grid_target, int64_t groups, int64_t tiles) {
+  int64_t gy = std::max<int64_t>(1, grid_target / std::max<int64_t>(groups, 1));
+  gy = std::min<int64_t>(gy, std::max<int64_t>(1, (tiles + kMinTilesPerBlock - 1) / kMinTilesPerBlock));
+  return std::min<int64_t>(gy, 65535);
+}
+
 inline unsigned grid1(int64_t n, int block, int64_t cap = 65535) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
@@ -1563,9 +1623,7 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const int64_t P = net.panels(l);
   const int64_t tiles = (a.max_cols + kBlock - 1) / kBlock;
   // Enough workgroups to fill 256 CUs several times over; each loops over column tiles.
-  int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(P, 1));
-  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
-  gy = std::min<int64_t>(gy, 65535);
+  const int64_t gy = grid_rows(a.shape.grid_target, P, tiles);
   SNICIT_REQUIRE(P <= 2147483647LL, "too many panels");
   dim3 grid((unsigned)P, (unsigned)gy);
   const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
@@ -1622,9 +1680,7 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
   const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
-  int64_t gy = std::max<int64_t>(1, shape.grid_target / std::max<int64_t>(P, 1));
-  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
-  gy = std::min<int64_t>(gy, 65535);
+  const int64_t gy = grid_rows(shape.grid_target, P, tiles);
   dim3 grid((unsigned)P, (unsigned)gy);
   const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
   const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
@@ -1686,15 +1742,14 @@ void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
   constexpr int TC = 64;
   const int64_t tiles = (a.max_cols + TC - 1) / TC;
   const int64_t groups = (a.rows + 255) / 256;
-  int64_t gy = std::max<int64_t>(1, a.shape.grid_target / std::max<int64_t>(groups, 1));
-  gy = std::min<int64_t>(gy, std::max<int64_t>(tiles, 1));
-  gy = std::min<int64_t>(gy, 65535);
+  const int64_t gy = grid_rows(a.shape.grid_target, groups, tiles);
   dim3 grid((unsigned)groups, (unsigned)gy);
   const size_t lds = (size_t)rc * (TC + 4) * 4;
 #define SNICIT_ELL_CHUNK(E)                                                                    \
   hipLaunchKernelGGL(ell_chunk_kernel<E>, grid, dim3(256), lds, s, a.cols, a.vals, a.K, a.rows, \
                      a.in_rows, rc, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout,        \
-                     a.ldout, a.count, a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post)
+                     a.ldout, a.count, a.map, a.flags, a.zc, a.kcap, a.mcent, a.eps_post,       \
+                     a.shape.narrow)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_ELL_CHUNK(EPI_PLAIN); break;
     case EPI_CENTROID: SNICIT_ELL_CHUNK(EPI_CENTROID); break;
