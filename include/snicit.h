@@ -274,6 +274,8 @@ typedef struct snicit_server_cfg {
   int32_t cluster_min;       /* exact nets: smaller requests share a clustering segment   */
   int32_t warm;              /* 1: capture every slot's graphs at creation                */
   double group_window_ms;    /* longest wait for a group to fill while the net is busy    */
+  int64_t group_samples;     /* a group closes at this many samples (0: slot_samples); a  */
+                             /* larger single request still runs, alone                   */
 } snicit_server_cfg;
 
 typedef struct snicit_request_info {

@@ -159,13 +159,21 @@ void launch_group(snicit_server* S, Slot* sl) {
   SNICIT_HIP(hipEventRecord(sl->done, s));
 }
 
-// Moves queued requests of `net` into the free slot while they fit.
+// Samples at which a group closes.
+int64_t group_cap(const snicit_server* S) {
+  const int64_t g = S->cfg.group_samples;
+  return g > 0 && g < S->cfg.slot_samples ? g : S->cfg.slot_samples;
+}
+
+// Moves queued requests of `net` into the free slot while they fit (the first one always
+// does: every request fits a slot).
 void fill_group(snicit_server* S, Slot* sl) {
   auto& q = S->queues[sl->net];
+  const int64_t cap = group_cap(S);
   int64_t total = 0;
   sl->reqs.clear();
   while (!q.empty() && (int32_t)sl->reqs.size() < S->cfg.slot_segments &&
-         total + q.front()->samples <= S->cfg.slot_samples) {
+         (sl->reqs.empty() || total + q.front()->samples <= cap)) {
     Request* r = q.front();
     q.pop_front();
     total += r->samples;
@@ -181,14 +189,15 @@ bool group_ready(snicit_server* S, int32_t net, Clock::time_point now) {
     if (sl->net == net && sl->busy) net_busy = true;
   if (!net_busy) return true;
   if (ms_between(q.front()->t_submit, now) >= S->cfg.group_window_ms) return true;
+  const int64_t cap = group_cap(S);
   int64_t total = 0;
   int32_t n = 0;
   for (const Request* r : q) {
-    if (n == S->cfg.slot_segments || total + r->samples > S->cfg.slot_samples) return true;
+    if (n == S->cfg.slot_segments || (n > 0 && total + r->samples > cap)) return true;
     total += r->samples;
     ++n;
   }
-  return false;
+  return total >= cap;
 }
 
 void finish_group(snicit_server* S, Slot* sl, Clock::time_point now, int status,
@@ -338,6 +347,7 @@ void snicit_server_cfg_default(snicit_server_cfg* cfg) {
   cfg->cluster_min = 32;
   cfg->warm = 1;
   cfg->group_window_ms = 2.0;
+  cfg->group_samples = 0;
 }
 
 int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
@@ -349,7 +359,9 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
                    "slot_samples out of range");
     SNICIT_REQUIRE(cfg->slot_segments >= 1 && cfg->max_slots_per_net >= 1,
                    "slot_segments and max_slots_per_net must be >= 1");
-    SNICIT_REQUIRE(cfg->group_window_ms >= 0.0 && cfg->cluster_min >= 0, "bad grouping policy");
+    SNICIT_REQUIRE(cfg->group_window_ms >= 0.0 && cfg->cluster_min >= 0 &&
+                       cfg->group_samples >= 0,
+                   "bad grouping policy");
     SNICIT_HIP(hipSetDevice(ctx->device));
     auto S = std::make_unique<snicit_server>();
     S->ctx = ctx;
