@@ -195,6 +195,7 @@ void destroy_net(Net* net) {
   dev_free(net->d_erng);
   dev_free(net->d_eent);
   dev_free(net->d_win);
+  dev_free(net->d_wint);
   dev_free(net->d_bin);
   dev_free(net->d_wout);
   dev_free(net->d_bout);
@@ -267,6 +268,7 @@ void enqueue_input_layer(snicit_plan* p) {
   EllLaunch e{};
   e.cols = nullptr;
   e.vals = net.d_win;
+  e.wt = net.d_wint;
   e.K = net.Kin;
   e.rows = net.N;
   e.in_rows = net.Kin;
@@ -935,10 +937,11 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
     SNICIT_HIP(hipSetDevice(net->ctx->device));
     const int32_t N = net->N;
     dev_free(net->d_win);
+    dev_free(net->d_wint);
     dev_free(net->d_bin);
     dev_free(net->d_wout);
     dev_free(net->d_bout);
-    net->d_win = net->d_bin = net->d_wout = net->d_bout = nullptr;
+    net->d_win = net->d_wint = net->d_bin = net->d_wout = net->d_bout = nullptr;
     net->Nin = Nin;
     net->Nout = Nout;
     net->Kin = (Nin + 15) / 16 * 16;
@@ -948,9 +951,14 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
       std::vector<float> w((size_t)N * net->Kin, 0.f);
       for (int32_t o = 0; o < N; ++o)
         std::memcpy(&w[(size_t)o * net->Kin], w_in + (size_t)o * Nin, (size_t)Nin * 4);
+      std::vector<float> wt((size_t)net->Kin * N, 0.f);
+      for (int32_t o = 0; o < N; ++o)
+        for (int32_t k = 0; k < Nin; ++k) wt[(size_t)k * N + o] = w_in[(size_t)o * Nin + k];
       net->d_win = dev_alloc<float>(w.size(), &net->device_bytes);
+      net->d_wint = dev_alloc<float>(wt.size(), &net->device_bytes);
       net->d_bin = dev_alloc<float>(N, &net->device_bytes);
       SNICIT_HIP(hipMemcpy(net->d_win, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(net->d_wint, wt.data(), wt.size() * 4, hipMemcpyHostToDevice));
       SNICIT_HIP(hipMemcpy(net->d_bin, b_in, (size_t)N * 4, hipMemcpyHostToDevice));
     }
     if (Nout > 0) {

@@ -109,6 +109,7 @@ struct Net {
   int32_t Nin = 0, Kin = 0, Nout = 0, Kout = 0;
   float in_clip = 1.f;
   float* d_win = nullptr;   // [N][Kin]
+  float* d_wint = nullptr;  // [Kin][N], the same weights transposed (matrix-core operand)
   float* d_bin = nullptr;   // [N]
   float* d_wout = nullptr;  // [Nout][Kout]
   float* d_bout = nullptr;  // [Nout]

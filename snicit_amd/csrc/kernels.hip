@@ -614,6 +614,44 @@ __global__ __launch_bounds__(256) void ell_lds_kernel(
   }
 }
 
+// (row, column) chains for launches with few live columns: thread `first` of `nthreads`
+// takes pairs first, first + nthreads, ..., the lanes of a wavefront along the columns of
+// one row and on to the next row, reading inputs and weights from L2.  Same ascending chain
+// and epilogue per (row, column) as the tiled forms.
+template <int EPI>
+__device__ __forceinline__ void ell_pair_chains(
+    const int32_t* __restrict__ cols, const float* __restrict__ vals, int32_t K, int32_t rows,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, int64_t count,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int64_t first,
+    int64_t nthreads) {
+  const int64_t pairs = (int64_t)rows * count;
+  for (int64_t q = first; q < pairs; q += nthreads) {
+    const int oo = (int)(q / count);
+    const int64_t pos = q % count;
+    const int64_t src = map ? (int64_t)map[pos] : pos;
+    const float* __restrict__ v = vals + (size_t)oo * K;
+    const int32_t* __restrict__ c = cols ? cols + (size_t)oo * K : nullptr;
+    float acc = 0.f;
+    int r = 0;
+    for (; r + 8 <= K; r += 8) {
+      float x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = yin[(int64_t)(c ? c[r + u] : r + u) * ldin + src];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = __builtin_fmaf(v[r + u], x[u], acc);
+    }
+    for (; r < K; ++r) acc = __builtin_fmaf(v[r], yin[(int64_t)(c ? c[r] : r) * ldin + src], acc);
+    const float b = bias_vec ? bias_vec[oo] : bias_s;
+    const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
+    bool flag = false;
+    const float y = epilogue<EPI>(acc, b, clip, true, oo, pos, mi, zc, kcap, eps_post, flag);
+    yout[(int64_t)oo * ldout + pos] = y;
+    if (flags && flag) flags[pos] = 1;
+  }
+}
+
 // Chunked row-parallel form for wide tiles: thread t owns output row blockIdx.x*256 + t and
 // TC = 64 columns.  The input rows are staged in chunks of rc rows (x_s[r - R0][c], row stride
 // TC+4); a thread's chain pauses at the first entry past the chunk and resumes with the next
@@ -635,35 +673,11 @@ __global__ __launch_bounds__(256) void ell_chunk_kernel(
   if (count <= narrow) {
     // Few columns (a small serving group): the row-per-thread form would keep only the
     // first column tile's workgroups busy, each running 64 column chains for a handful of
-    // live columns.  Instead every workgroup of the grid takes (row, column) chains, the
-    // lanes of a wavefront along the columns of one row and on to the next row, reading
-    // inputs and weights from L2.  Same ascending chain and epilogue per (row, column).
-    const int64_t pairs = (int64_t)rows * count;
-    const int64_t nthreads = (int64_t)gridDim.x * gridDim.y * 256;
-    for (int64_t q = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
-         q < pairs; q += nthreads) {
-      const int oo = (int)(q / count);
-      const int64_t pos = q % count;
-      const int64_t src = map ? (int64_t)map[pos] : pos;
-      const float* __restrict__ v = vals + (size_t)oo * K;
-      const int32_t* __restrict__ c = cols ? cols + (size_t)oo * K : nullptr;
-      float acc = 0.f;
-      int r = 0;
-      for (; r + 8 <= K; r += 8) {
-        float x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = yin[(int64_t)(c ? c[r + u] : r + u) * ldin + src];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) acc = __builtin_fmaf(v[r + u], x[u], acc);
-      }
-      for (; r < K; ++r) acc = __builtin_fmaf(v[r], yin[(int64_t)(c ? c[r] : r) * ldin + src], acc);
-      const float b = bias_vec ? bias_vec[oo] : bias_s;
-      const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
-      bool flag = false;
-      const float y = epilogue<EPI>(acc, b, clip, true, oo, pos, mi, zc, kcap, eps_post, flag);
-      yout[(int64_t)oo * ldout + pos] = y;
-      if (flags && flag) flags[pos] = 1;
-    }
+    // live columns.  Instead every workgroup of the grid takes (row, column) chains.
+    ell_pair_chains<EPI>(cols, vals, K, rows, bias_vec, bias_s, clip, yin, ldin, yout, ldout,
+                         count, map, flags, zc, kcap, mcent, eps_post,
+                         ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x,
+                         (int64_t)gridDim.x * gridDim.y * 256);
     return;
   }
   const int o = blockIdx.x * 256 + threadIdx.x;
@@ -1590,6 +1604,112 @@ __global__ __launch_bounds__(256) void scatter_delta_kernel(
     Y[(int64_t)n * ld + dst] = D[(int64_t)n * ldD + col] + C[(int64_t)n * kcap + c];
 }
 
+// ---- dense layers on the matrix cores -----------------------------------------------
+// Dense input layer (src/beyond/general.hpp:5-33) as a GEMM on the matrix cores.  A
+// wavefront owns RB 32-row blocks x CB 64-column blocks; step k multiplies the weights of
+// input k for its rows (A: lane l holds row l mod 32 of a block, both halves) with input k
+// of its columns (B: lane l holds column l of a block) into RB*CB accumulators, k
+// ascending.  With K = 1 per step every output element takes one fused multiply-add per
+// input, so the result is exactly the chain acc = fma(w[K-1], x[K-1], ... fma(w[0], x[0],
+// 0)) of the vector kernels and the oracle (D layout as in panel_mfma_kernel).  Operands
+// come from the transposed weights wt[k][row] and the input rows, kDenseU steps ahead in a
+// second register set.  The four wavefronts of a workgroup take four row tiles of one column
+// tile, so the tile's inputs come from L1 after the first.  Columns past the live count read
+// the last live column; their outputs are dropped.  With at most `narrow` live columns every
+// thread of the grid takes (row, column) chains over the row-major weights instead.  A
+// launch handles live counts in [lo, hi) and returns at once otherwise, so two tile forms
+// can be launched back to back and the live count picks one on the device.
+constexpr int kDenseU = 8;
+
+template <int EPI, int RB, int CB>
+__global__ __launch_bounds__(256) void dense_mfma_kernel(
+    const float* __restrict__ wt, const float* __restrict__ vals, int32_t K, int32_t rows,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    int32_t narrow, int64_t lo, int64_t hi) {
+  constexpr int U = kDenseU;
+  const int64_t count = *count_ptr;
+  if (count < lo || count >= hi) return;
+  if (count <= narrow) {
+    ell_pair_chains<EPI>(nullptr, vals, K, rows, bias_vec, bias_s, clip, yin, ldin, yout, ldout,
+                         count, nullptr, nullptr, nullptr, 0, nullptr, 0.f,
+                         (int64_t)blockIdx.x * 256 + threadIdx.x, (int64_t)gridDim.x * 256);
+    return;
+  }
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int half = lane >> 5, j = lane & 31;
+  const int row_tiles = rows / (32 * RB);
+  const int64_t items = (int64_t)row_tiles * ((count + 64 * CB - 1) / (64 * CB));
+  for (int64_t it = (int64_t)blockIdx.x * 4 + wv; it < items; it += (int64_t)gridDim.x * 4) {
+    const int r0 = (int)(it % row_tiles) * 32 * RB;
+    const int64_t c0 = it / row_tiles * 64 * CB;
+    const float* __restrict__ ap = wt + r0 + j;
+    const float* __restrict__ bp[CB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) bp[cb] = yin + min(c0 + 64 * cb + lane, count - 1);
+    float a0[U][RB], b0[U][CB], a1[U][RB], b1[U][CB];
+    floatx32 acc[RB][CB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int q = 0; q < 32; ++q) acc[rb][cb][q] = 0.f;
+    auto load = [&](float (&a)[U][RB], float (&b)[U][CB], int k0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) a[u][rb] = ap[(int64_t)(k0 + u) * rows + 32 * rb];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) b[u][cb] = bp[cb][(int64_t)(k0 + u) * ldin];
+      }
+    };
+    auto mma = [&](const float (&a)[U][RB], const float (&b)[U][CB]) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x1f32(a[u][rb], b[u][cb], acc[rb][cb],
+                                                               0, 0, 0);
+    };
+    // K is a multiple of 16 = 2 U
+    load(a0, b0, 0);
+    for (int k0 = 0; k0 < K; k0 += 2 * U) {
+      load(a1, b1, k0 + U);
+      mma(a0, b0);
+      if (k0 + 2 * U < K) load(a0, b0, k0 + 2 * U);
+      mma(a1, b1);
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      float bias[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int o = r0 + 32 * rb + 8 * (q >> 2) + 4 * half + (q & 3);
+        bias[q] = bias_vec ? bias_vec[o] : bias_s;
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const int64_t pos = c0 + 64 * cb + 32 * b + j;
+          if (pos >= count) continue;
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int o = r0 + 32 * rb + 8 * (q >> 2) + 4 * half + (q & 3);
+            bool f = false;
+            const float y = epilogue<EPI>(acc[rb][cb][16 * b + q], bias[q], clip, true, o, pos,
+                                          0, nullptr, 0, 0.f, f);
+            yout[(int64_t)o * ldout + pos] = y;
+          }
+        }
+    }
+  }
+}
+
 // Workgroups along the column-tile axis of a (row group x column tile) grid: grid_target /
 // groups, at most one per tile, and at most one per kMinTilesPerBlock tiles of the plan's
 // capacity.  Launch sizes follow the capacity, not the live count, so a plan whose
@@ -1738,11 +1858,29 @@ void launch_ell_lds_form(const EllLaunch& a, hipStream_t s) {
   else launch_ell_lds<TC, true>(a, s);
 }
 
+// CU count of the current device (cached per device)
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  SNICIT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  int cus = cache[dev].load(std::memory_order_relaxed);
+  if (cus <= 0) {
+    SNICIT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev].store(cus, std::memory_order_relaxed);
+  }
+  return cus;
+}
+
 void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
   constexpr int TC = 64;
   const int64_t tiles = (a.max_cols + TC - 1) / TC;
   const int64_t groups = (a.rows + 255) / 256;
-  const int64_t gy = grid_rows(a.shape.grid_target, groups, tiles);
+  // enough workgroups for the (row, column) chains when the count may be narrow
+  const int64_t pairs = a.rows * std::min<int64_t>(a.max_cols, a.shape.narrow);
+  const int64_t gy_pairs = std::min<int64_t>((pairs + 255) / 256, (int64_t)8 * device_cus());
+  const int64_t gy = std::min<int64_t>(
+      65535, std::max(grid_rows(a.shape.grid_target, groups, tiles), (gy_pairs + groups - 1) / groups));
   dim3 grid((unsigned)groups, (unsigned)gy);
   const size_t lds = (size_t)rc * (TC + 4) * 4;
 #define SNICIT_ELL_CHUNK(E)                                                                    \
@@ -1762,18 +1900,6 @@ void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
   check_launch();
 }
 
-int device_cus() {
-  static std::atomic<int> cache[64];
-  int dev = 0;
-  SNICIT_HIP(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) dev = 0;
-  int cus = cache[dev].load(std::memory_order_relaxed);
-  if (cus <= 0) {
-    SNICIT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    cache[dev].store(cus, std::memory_order_relaxed);
-  }
-  return cus;
-}
 
 template <int W>
 void launch_ell_wave_w(const EllLaunch& a, hipStream_t s) {
@@ -1810,11 +1936,55 @@ void launch_ell_wave(const EllLaunch& a, hipStream_t s) {
   else launch_ell_wave_w<1>(a, s);
 }
 
+// Live column count from which a dense-layer wavefront takes 64 rows x 128 columns (four
+// accumulators per input step, one wavefront per SIMD) rather than 32 x 64 (one accumulator,
+// four wavefronts per SIMD): below it the 32 x 64 tiles keep more of the CUs busy.
+constexpr int64_t kDenseWideCols = 2304;
+
+template <int RB, int CB>
+void launch_dense_mfma_t(const EllLaunch& a, int64_t lo, int64_t hi, hipStream_t s) {
+  const int64_t row_tiles = a.rows / (32 * RB);
+  const int64_t cols = std::min<int64_t>(a.max_cols, hi);
+  const int64_t tiles = (cols + 64 * CB - 1) / (64 * CB);
+  // four wavefronts per workgroup, or a thread per (row, column) pair when the count may be
+  // narrow; at most eight resident workgroups per CU
+  const int64_t pairs = lo <= a.shape.narrow ? a.rows * std::min<int64_t>(cols, a.shape.narrow) : 0;
+  const int64_t most = std::max<int64_t>({1, (row_tiles * tiles + 3) / 4, (pairs + 255) / 256});
+  const unsigned grid = (unsigned)std::min<int64_t>(most, (int64_t)8 * std::max(device_cus(), 1));
+#define SNICIT_DENSE(E)                                                                        \
+  hipLaunchKernelGGL((dense_mfma_kernel<E, RB, CB>), dim3(grid), dim3(256), 0, s, a.wt, a.vals, \
+                     a.K, a.rows, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout, a.ldout, \
+                     a.count, a.shape.narrow, lo, hi)
+  if (a.epilogue == EPI_PLAIN) SNICIT_DENSE(EPI_PLAIN);
+  else SNICIT_DENSE(EPI_LINEAR);
+#undef SNICIT_DENSE
+  check_launch();
+}
+
+// 32 x 64 tiles, and from kDenseWideCols live columns 64 x 128 tiles (both launched when the
+// capacity reaches it; the (row, column) chains stay with the 32 x 64 form)
+void launch_dense_mfma(const EllLaunch& a, hipStream_t s) {
+  constexpr int64_t kAll = INT64_MAX;
+  if (a.max_cols >= kDenseWideCols && a.shape.narrow < kDenseWideCols) {
+    launch_dense_mfma_t<1, 1>(a, 0, kDenseWideCols, s);
+    launch_dense_mfma_t<2, 2>(a, kDenseWideCols, kAll, s);
+  } else {
+    launch_dense_mfma_t<1, 1>(a, 0, kAll, s);
+  }
+}
+
 void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   SNICIT_REQUIRE(a.K % 16 == 0 && a.K > 0, "ELL rows must be padded to a multiple of 16");
   SNICIT_REQUIRE(a.in_rows > 0, "ELL launch needs the input row count");
   if (a.shape.ell_chunked == 2 && a.cols && a.rng && a.ent) {
     launch_ell_wave(a, s);
+    return;
+  }
+  // dense layers without per-column flags, gathers or centroid epilogues: matrix cores
+  if (a.shape.ell_chunked == 2 && !a.cols && a.wt && !a.map && !a.flags &&
+      (a.epilogue == EPI_PLAIN || a.epilogue == EPI_LINEAR) && a.rows % 64 == 0 &&
+      a.in_rows == a.K) {
+    launch_dense_mfma(a, s);
     return;
   }
   const int64_t lds = std::min<int64_t>((int64_t)a.shape.ell_lds_kb * 1024, kLdsPerBlock);

@@ -36,7 +36,8 @@ struct LaunchShape {
   int32_t ell_lds_kb = 80;
   // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in
   // chunks of input rows; 2 = wavefront per row, lanes = 64 columns, input chunks of
-  // kEllChunk rows double-buffered in LDS (layers with a chunk range table); 0 = neither
+  // kEllChunk rows double-buffered in LDS (layers with a chunk range table), and dense
+  // layers with a transposed weight copy on the matrix cores; 0 = neither
   int32_t ell_chunked = 2;
 };
 
@@ -80,6 +81,7 @@ struct EllLaunch {
   const float* vals;    // [rows][K]
   const int32_t* rng;   // [rows][ell_chunks(in_rows)+1] ranges in ent, or null
   const int2* ent;      // padded (chunk tile byte offset, weight bits) stream, with rng
+  const float* wt;      // dense layers: [in_rows][rows] transposed weights, or null
   int32_t K, rows;
   int32_t in_rows;      // input neurons the entries refer to (rows of yin read)
   const float* bias_vec;

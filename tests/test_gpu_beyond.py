@@ -88,6 +88,33 @@ def test_model_io_plain_bitexact(ctx, mode):
     np.testing.assert_array_equal(plan.categories(), cat)
 
 
+@pytest.mark.parametrize("B,cap", [(1, 1), (64, 4096), (65, 65), (777, 3000), (2303, 2400),
+                                   (2304, 2304), (8200, 8200)])
+def test_dense_input_layer_forms(ctx, B, cap):
+    """Dense input layer on the matrix cores: sample counts that take the (row, column)
+    chains (<= 64 columns), 32 x 64 wavefront tiles (< 2304 columns) and 64 x 128 tiles,
+    each with a partial last tile, in plans whose capacity launches one tile form or both;
+    300 inputs pad to 304, not a multiple of 32.  The vector path (ell_chunked = 1) gives
+    the same bits."""
+    Nin, N, L, K, Nout = 300, 256, 1, 40, 10
+    m = _model(Nin, N, L, K, Nout, seed=11)
+    x = sa.gen_features(Nin, 5, 0, B)
+    net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
+    net.set_io(m["w_in"], m["b_in"], 1.0, m["w_out"], m["b_out"])
+    yL, lg, cat = orc.model_plain(m["cols"], m["vals"], 0.0, 1.0, x, w_in=m["w_in"],
+                                  b_in=m["b_in"], in_clip=1.0, w_out=m["w_out"],
+                                  b_out=m["b_out"])
+    for chunked in (2, 1):
+        print(f"B={B} capacity={cap} ell_chunked={chunked}", flush=True)
+        plan = sa.Plan(net, sa.default_cfg("BF"), cap)
+        plan.set_shape(16384, 64, 80, chunked)
+        plan.set_input(x)
+        plan.run()
+        np.testing.assert_array_equal(plan.activations(), yL)
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LOGITS).reshape(B, Nout), lg)
+        np.testing.assert_array_equal(plan.categories(), cat)
+
+
 @pytest.mark.parametrize("t,positive_only", [(0, 0), (2, 0), (2, 1), (5, 0)])
 def test_lossy_snicit_matches_oracle(ctx, t, positive_only):
     Nin, N, L, K, Nout, B = 784, 1024, 6, 103, 10, 700

@@ -50,7 +50,7 @@ def test_lossy_ell_launch_shapes(ctx):
                                b_out=b_out)
     shapes = [(1, 0, 0, 0), (1, 100000, 64, 0), (65536, 128, 16, 0), (16384, 64, 32, 0),
               (7, 0, 8, 0), (16384, 64, 80, 1), (1, 0, 5, 1), (3, 64, 160, 1), (65536, 0, 40, 1),
-              (16384, 64, 80, 2), (1, 0, 0, 2)]
+              (16384, 64, 80, 2), (1, 0, 0, 2), (1, 100000, 80, 2), (4096, 128, 80, 2)]
     for grid_target, narrow, lds_kb, chunked in shapes:
         print(f"shape grid_target={grid_target} narrow={narrow} ell_lds_kb={lds_kb} "
               f"chunked={chunked}", flush=True)
