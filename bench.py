@@ -28,7 +28,7 @@ WORKLOADS = {
                B=dict(Nin=784, N=1024, L=24, K=103, Nout=10, clip=1.0, t=8)),
     # Poisson request stream against two resident nets
     "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=1000,
-               slots=3, window_ms=2.0, group_samples=0,
+               slots=2, window_ms=2.0, group_samples=0, pinned_mb=1024,
                A=dict(N=16384, L=120, bias=-0.4, t=30),
                B=dict(Nin=3072, N=2048, L=16, K=307, Nout=10, clip=1.0, t=8)),
 }
@@ -231,7 +231,8 @@ def bench_serve(args, wl):
     t0 = time.perf_counter()
     srv = sa.Server(ctx, [net_a, net_b], cfgs, slot_samples=8192, slot_segments=64,
                     max_slots_per_net=wl["slots"], group_window_ms=wl["window_ms"],
-                    group_samples=wl["group_samples"], hbm_budget_bytes=64 << 30)
+                    group_samples=wl["group_samples"], hbm_budget_bytes=64 << 30,
+                    pinned_bytes=wl["pinned_mb"] << 20)
     log(f"server: {srv.stats()} ({time.perf_counter() - t0:.1f} s to allocate and capture)")
     pool = prototype_features(Bn["Nin"], 8192, 99)
     edges_per = {0: A["N"] * 32 * A["L"],
@@ -317,7 +318,8 @@ def bench_serve(args, wl):
                                f"{Bn['N']}x{Bn['L']}->{Bn['Nout']}, {Bn['K']} nnz/row, t={Bn['t']}, "
                                f"lossy); slots of 8192 samples / 64 requests, {wl['slots']} "
                                f"per net, grouping window {wl['window_ms']} ms, groups close "
-                               f"at {wl['group_samples'] or 8192} samples",
+                               f"at {wl['group_samples'] or 8192} samples, request inputs "
+                               f"in a {wl['pinned_mb']} MiB pinned arena",
                    "server": st},
         "serving": table,
         "roofline": None,

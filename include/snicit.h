@@ -276,6 +276,9 @@ typedef struct snicit_server_cfg {
   double group_window_ms;    /* longest wait for a group to fill while the net is busy    */
   int64_t group_samples;     /* a group closes at this many samples (0: slot_samples); a  */
                              /* larger single request still runs, alone                   */
+  int64_t pinned_bytes;      /* pinned host memory that holds request inputs from submit  */
+                             /* until their group is done (0: none); inputs that do not   */
+                             /* fit are kept in pageable memory and staged at launch      */
 } snicit_server_cfg;
 
 typedef struct snicit_request_info {

@@ -70,7 +70,8 @@ class ServerCfg(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slot_samples", ctypes.c_int64),
                 ("slot_segments", ctypes.c_int32), ("max_slots_per_net", ctypes.c_int32),
                 ("cluster_min", ctypes.c_int32), ("warm", ctypes.c_int32),
-                ("group_window_ms", ctypes.c_double), ("group_samples", ctypes.c_int64)]
+                ("group_window_ms", ctypes.c_double), ("group_samples", ctypes.c_int64),
+                ("pinned_bytes", ctypes.c_int64)]
 
 
 class RequestInfo(ctypes.Structure):

@@ -11,6 +11,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -34,7 +36,9 @@ struct Request {
   bool images = false;
   uint64_t seed = 0;
   int64_t first = 0;
-  std::vector<float> x;
+  std::vector<float> x;    // host input when it is not in the pinned arena
+  float* pin = nullptr;    // host input in the server's pinned arena, or null
+  int64_t pin_bytes = 0;
   Clock::time_point t_submit, t_launch, t_done;
   std::vector<int32_t> cat;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 failed (status code)
@@ -71,6 +75,10 @@ struct snicit_server {
   std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
   uint64_t next_id = 1;
   bool stop = false;
+  // pinned arena for request inputs: free blocks by offset, first fit, merged on release
+  char* pin_base = nullptr;
+  std::map<int64_t, int64_t> pin_free;
+  std::mutex pin_mu;
   std::thread worker;
   int64_t requests_done = 0, samples_done = 0, groups = 0;
 };
@@ -78,6 +86,46 @@ struct snicit_server {
 namespace {
 
 int32_t width_of(const snicit_net* n) { return n->Nin > 0 ? n->Nin : n->N; }
+
+constexpr int64_t kPinAlign = 256;
+// Inputs at least this large are copied to the device straight from their pinned block;
+// smaller ones are packed into the slot's staging rows first (one copy for all of them).
+constexpr int64_t kPinDirectBytes = int64_t(1) << 20;
+
+// A block of `bytes` from the pinned arena, or null when no free block is large enough.
+float* pin_alloc(snicit_server* S, int64_t bytes) {
+  if (!S->pin_base) return nullptr;
+  const int64_t need = (bytes + kPinAlign - 1) / kPinAlign * kPinAlign;
+  std::lock_guard<std::mutex> g(S->pin_mu);
+  for (auto it = S->pin_free.begin(); it != S->pin_free.end(); ++it) {
+    if (it->second < need) continue;
+    const int64_t off = it->first, len = it->second;
+    S->pin_free.erase(it);
+    if (len > need) S->pin_free[off + need] = len - need;
+    return reinterpret_cast<float*>(S->pin_base + off);
+  }
+  return nullptr;
+}
+
+void pin_release(snicit_server* S, float* p, int64_t bytes) {
+  const int64_t need = (bytes + kPinAlign - 1) / kPinAlign * kPinAlign;
+  int64_t off = reinterpret_cast<char*>(p) - S->pin_base, len = need;
+  std::lock_guard<std::mutex> g(S->pin_mu);
+  auto next = S->pin_free.lower_bound(off);
+  if (next != S->pin_free.end() && next->first == off + len) {
+    len += next->second;
+    next = S->pin_free.erase(next);
+  }
+  if (next != S->pin_free.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      off = prev->first;
+      len += prev->second;
+      S->pin_free.erase(prev);
+    }
+  }
+  S->pin_free[off] = len;
+}
 
 void slot_free(Slot* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -123,22 +171,36 @@ void launch_group(snicit_server* S, Slot* sl) {
   hipStream_t s = sl->stream;
   int64_t total = 0;
   for (const Request* r : sl->reqs) total += r->samples;
-  // host inputs: pack into the pinned staging rows of their columns
-  int64_t off = 0;
+  // host inputs to the device staging rows of their samples: large pinned inputs straight
+  // from their arena blocks, the others packed into the slot's pinned staging rows and
+  // copied in runs of consecutive samples
+  int64_t off = 0, run = -1;  // first sample of the open packed run
   bool any_host = false;
+  auto flush = [&](int64_t end) {
+    if (run < 0) return;
+    SNICIT_HIP(hipMemcpyAsync(p->d_stage + (size_t)run * width, sl->h_in + (size_t)run * width,
+                              (size_t)(end - run) * width * 4, hipMemcpyHostToDevice, s));
+    run = -1;
+  };
   for (const Request* r : sl->reqs) {
-    if (!r->images) {
-      std::memcpy(sl->h_in + (size_t)off * width, r->x.data(), (size_t)r->samples * width * 4);
+    const size_t bytes = (size_t)r->samples * width * 4;
+    if (r->images) {
+      flush(off);
+    } else if (r->pin && (int64_t)bytes >= kPinDirectBytes) {
+      flush(off);
+      SNICIT_HIP(hipMemcpyAsync(p->d_stage + (size_t)off * width, r->pin, bytes,
+                                hipMemcpyHostToDevice, s));
+      any_host = true;
+    } else {
+      std::memcpy(sl->h_in + (size_t)off * width, r->pin ? r->pin : r->x.data(), bytes);
+      if (run < 0) run = off;
       any_host = true;
     }
     off += r->samples;
   }
-  if (any_host) {
-    // rows of image requests in the staging buffer are overwritten by their renders below
-    SNICIT_HIP(hipMemcpyAsync(p->d_stage, sl->h_in, (size_t)total * width * 4,
-                              hipMemcpyHostToDevice, s));
-    launch_transpose_in(p->d_stage, total, width, p->d_in, p->ld, s);
-  }
+  flush(off);
+  // rows of image requests in the staging buffer are overwritten by their renders below
+  if (any_host) launch_transpose_in(p->d_stage, total, width, p->d_in, p->ld, s);
   off = 0;
   for (const Request* r : sl->reqs) {
     if (r->images) load_images(p, r->seed, r->first, r->samples, off);
@@ -205,6 +267,10 @@ void finish_group(snicit_server* S, Slot* sl, Clock::time_point now, int status,
   int64_t off = 0, total = 0;
   for (const Request* r : sl->reqs) total += r->samples;
   for (Request* r : sl->reqs) {
+    // the group's copies are done (or failed): the input is no longer needed
+    if (r->pin) pin_release(S, r->pin, r->pin_bytes);
+    r->pin = nullptr;
+    std::vector<float>().swap(r->x);
     if (status == 0) r->cat.assign(sl->h_cat + off, sl->h_cat + off + r->samples);
     r->status = status == 0 ? 2 : status;
     r->error = err;
@@ -284,6 +350,9 @@ void server_free(snicit_server* S) {
   if (S->worker.joinable()) S->worker.join();
   for (auto& sl : S->slots) slot_free(sl.get());
   S->slots.clear();
+  if (S->pin_base) (void)hipHostFree(S->pin_base);
+  S->pin_base = nullptr;
+  S->pin_free.clear();
 }
 
 // One slot of `net`: plan with the segment capacity, stream, event, pinned staging.
@@ -348,6 +417,7 @@ void snicit_server_cfg_default(snicit_server_cfg* cfg) {
   cfg->warm = 1;
   cfg->group_window_ms = 2.0;
   cfg->group_samples = 0;
+  cfg->pinned_bytes = int64_t(256) << 20;
 }
 
 int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_run_cfg* cfgs,
@@ -362,6 +432,7 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
     SNICIT_REQUIRE(cfg->group_window_ms >= 0.0 && cfg->cluster_min >= 0 &&
                        cfg->group_samples >= 0,
                    "bad grouping policy");
+    SNICIT_REQUIRE(cfg->pinned_bytes >= 0, "pinned_bytes must be >= 0");
     SNICIT_HIP(hipSetDevice(ctx->device));
     auto S = std::make_unique<snicit_server>();
     S->ctx = ctx;
@@ -373,6 +444,13 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
     }
     S->queues.resize(n_nets);
     try {
+      if (cfg->pinned_bytes > 0) {
+        const int64_t bytes = cfg->pinned_bytes / kPinAlign * kPinAlign;
+        if (bytes > 0) {
+          SNICIT_HIP(hipHostMalloc((void**)&S->pin_base, (size_t)bytes, 0));
+          S->pin_free[0] = bytes;
+        }
+      }
       // slots round-robin over the nets while the budget allows (at least one per net)
       for (int32_t round = 0; round < cfg->max_slots_per_net; ++round) {
         for (int32_t n = 0; n < n_nets; ++n) {
@@ -433,11 +511,19 @@ int snicit_server_submit(snicit_server* server, int32_t net, const float* x, int
   r->net = net;
   r->samples = samples;
   const int rc = guarded([&] {
-    SNICIT_REQUIRE(server && x, "null argument");
+    SNICIT_REQUIRE(server && x && request_id, "null argument");
     SNICIT_REQUIRE(net >= 0 && net < (int32_t)server->nets.size(), "no such net");
     SNICIT_REQUIRE(samples >= 1 && samples <= server->cfg.slot_samples,
                    "request size must be in [1, slot_samples]");
-    r->x.assign(x, x + (size_t)samples * width_of(server->nets[net]));
+    // (the checks above are the ones submit() repeats, so the block cannot leak)
+    const size_t n = (size_t)samples * width_of(server->nets[net]);
+    r->pin = pin_alloc(server, (int64_t)n * 4);
+    if (r->pin) {
+      r->pin_bytes = (int64_t)n * 4;
+      std::memcpy(r->pin, x, n * 4);
+    } else {
+      r->x.assign(x, x + n);
+    }
   });
   if (rc != SNICIT_OK) return rc;
   return submit(server, std::move(r), request_id);

@@ -25,6 +25,33 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(sa.SIGNATURES), declared ^ set(sa.SIGNATURES)
 
 
+@pytest.mark.parametrize("cname,py", [("snicit_run_cfg", "RunCfg"), ("snicit_stats", "Stats"),
+                                      ("snicit_tune_info", "TuneInfo"),
+                                      ("snicit_segment", "Segment"),
+                                      ("snicit_server_cfg", "ServerCfg"),
+                                      ("snicit_request_info", "RequestInfo"),
+                                      ("snicit_server_stats", "ServerStats")])
+def test_ctypes_structs_follow_the_header(cname, py):
+    """Every struct of the C-ABI and its ctypes mirror list the same fields in the same order
+    with the same element types and counts."""
+    hdr = open(os.path.join(ROOT, "include", "snicit.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), hdr, re.S).group(1)
+    ctype = {"int32_t": ctypes.c_int32, "int64_t": ctypes.c_int64, "double": ctypes.c_double,
+             "float": ctypes.c_float, "uint64_t": ctypes.c_uint64, "char": ctypes.c_char}
+    want = []
+    for decl in filter(None, (d.strip() for d in body.split(";"))):
+        typ, names = decl.split(None, 1)
+        for name in (n.strip() for n in names.split(",")):
+            m = re.fullmatch(r"(\w+)\[(\d+)\]", name)
+            want.append((m.group(1), ctype[typ] * int(m.group(2))) if m else (name, ctype[typ]))
+    got = getattr(sa, py)._fields_
+    assert [n for n, _ in got] == [n for n, _ in want]
+    for (n, a), (_, b) in zip(got, want):
+        assert ctypes.sizeof(a) == ctypes.sizeof(b), n
+        assert getattr(a, "_type_", a) == getattr(b, "_type_", b), n
+
+
 def test_error_path_without_device():
     # bad arguments are rejected before any device work
     rc = sa.lib().snicit_gen_sdgc_ell(1000, 2, 1, None)

@@ -102,3 +102,30 @@ def test_server_argument_errors_and_timeout(ctx):
     srv.close()
     with pytest.raises(sa.SnicitError):
         sa.Server(ctx, [net], [sa.default_cfg("BF")], slot_samples=256, hbm_budget_bytes=1)
+
+
+@pytest.mark.parametrize("pinned_mb", [0, 2, 256])
+def test_server_pinned_input_arena(ctx, pinned_mb):
+    """Request inputs held in the server's pinned arena (large ones copied to the device from
+    it directly, small ones packed with the others), in pageable memory when the arena is
+    off or full, and arena blocks reused as groups finish: every request's categories are
+    those of its run alone."""
+    Nin, N1, L1, K1, Nout = 784, 1024, 6, 103, 10
+    cols, vals, w_in, w_out = sa.gen_unstructured(Nin, N1, L1, K1, Nout, 5)
+    net = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=1.0)
+    net.set_io(w_in, np.zeros(N1, np.float32), 1.0, w_out, np.zeros(Nout, np.float32))
+    cfg = sa.default_cfg("SNICIT", threshold=2, cluster=sa.CLUSTER_LOSSY)
+    srv = sa.Server(ctx, [net], [cfg], slot_samples=2048, slot_segments=16, max_slots_per_net=2,
+                    group_window_ms=1.0, hbm_budget_bytes=8 << 30, pinned_bytes=pinned_mb << 20)
+    r = np.random.default_rng(pinned_mb + 1)
+    reqs = []
+    for i in range(40):
+        B = int(np.exp(r.uniform(0, np.log(1500))))  # >= 335 samples: at least 1 MiB
+        x = _clustered(Nin, B, 100 + i)
+        reqs.append((srv.submit(0, x), B, x))
+    got = {rid: srv.wait(rid, 60000.0)[0] for rid, _, _ in reqs}
+    assert srv.stats()["requests_done"] == 40
+    srv.close()
+    for rid, B, x in reqs:
+        print(f"request {rid}: {B} samples", flush=True)
+        np.testing.assert_array_equal(got[rid], _alone(net, cfg, lambda p: p.set_input(x), B))
