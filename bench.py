@@ -206,6 +206,28 @@ def bench_batch(args, wl):
     print(json.dumps(out), flush=True)
 
 
+def pass_serve_net_b(args, wl):
+    """One eager forward (no graph) of the serving workload's unstructured net B in the given
+    mode for 1024, 4096 and 8192 samples (plan capacity 8192), for kernel traces and counters:
+    the dense 3072 -> 2048 input layer, the 16 wide-row layers and the output layer."""
+    import snicit_amd as sa
+
+    Bn = wl["B"]
+    ctx = sa.Context(0)
+    cols, vals, w_in, w_out = sa.gen_unstructured(Bn["Nin"], Bn["N"], Bn["L"], Bn["K"],
+                                                  Bn["Nout"], NET_SEED)
+    net = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=Bn["clip"])
+    net.set_io(w_in, np.zeros(Bn["N"], np.float32), 1.0, w_out,
+               np.zeros(Bn["Nout"], np.float32))
+    lossy = dict(cluster=sa.CLUSTER_LOSSY) if args.pmc_pass == "SNICIT" else {}
+    cfg = sa.default_cfg(args.pmc_pass, threshold=Bn["t"], use_graph=0, **lossy)
+    plan = sa.Plan(net, cfg, 8192)
+    for B in (1024, 4096, 8192):
+        plan.set_input(prototype_features(Bn["Nin"], B, 7))
+        st = plan.run()
+        log(f"net B {args.pmc_pass} pass, {B} samples: {st.ms_total:.3f} ms")
+
+
 def bench_serve(args, wl):
     """Form (b): a request loop over a resident SDGC-like net (exact SNICIT) and an
     unstructured net (lossy SNICIT).  Saturated throughput comes from a burst of requests;
@@ -366,7 +388,10 @@ def main():
         bench_batch(args, wl)
         return
     if args.config == "c5":
-        bench_serve(args, wl)
+        if args.pmc_pass:
+            pass_serve_net_b(args, wl)
+        else:
+            bench_serve(args, wl)
         return
     if args.threshold is not None:
         wl["t"] = args.threshold
