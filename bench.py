@@ -206,6 +206,31 @@ def bench_batch(args, wl):
     print(json.dumps(out), flush=True)
 
 
+def stream_ceiling():
+    """Device-memory rates of plain streams (best of 5): a sum reads 2 GiB, a fill writes
+    2 GiB, a copy reads and writes 2 GiB each.  The ceiling the layer kernels' HBM rates are
+    compared with, beside the 8 TB/s peak."""
+    import torch
+
+    n = 1 << 29
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    out = {}
+    for name, fn, moved in [("read", lambda: a.sum(), 4 * n), ("write", lambda: b.fill_(2.0), 4 * n),
+                            ("copy", lambda: b.copy_(a), 8 * n)]:
+        best = float("inf")
+        for _ in range(6):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[name] = moved / best / 1e6
+        log(f"{name}: {best:.3f} ms, {out[name]:.0f} GB/s")
+    print(json.dumps({"stream_GBps": out}), flush=True)
+
+
 def pass_serve_net_b(args, wl):
     """One eager forward (no graph) of the serving workload's unstructured net B in the given
     mode for 1024, 4096 and 8192 samples (plan capacity 8192), for kernel traces and counters:
@@ -365,7 +390,12 @@ def main():
                          "write per-layer rows to gpurun_out/pmc_rows_<mode>.json")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="three 1 GiB device-to-device copies (torch), for scaling counters")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="read, write and copy rates of 2 GiB device buffers (torch)")
     args = ap.parse_args()
+    if args.stream_ceiling:
+        stream_ceiling()
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
