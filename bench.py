@@ -97,7 +97,20 @@ def cpu_baseline(wl, gpu_cat, target_s):
     return {"value": edges / dt, "unit": "edges/s", "cores": cores, "kind": "port",
             "sample": f"first {n} of the {wl['B']} samples x all {L} layers "
                       f"(plain FP32 forward with dead-row skipping, OpenMP); {dt:.2f} s",
-            "seconds": dt, "categories_match_gpu": match}
+            "seconds": dt, "categories_match_gpu": match, "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count()}
+
+
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def prototype_features(Nin, B, seed):
