@@ -140,6 +140,15 @@ int snicit_net_generate_sdgc(snicit_ctx* ctx, int32_t N, int32_t L, uint64_t see
  * may be 0.  Call before creating plans on the net. */
 int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const float* b_in,
                       float in_clip, int32_t Nout, const float* w_out, const float* b_out);
+/* Keep the hidden layers' weights in pinned host memory instead of device memory: each
+ * layer's arrays move into one pinned block, and every plan on the net copies the blocks
+ * into `buffers` device slots (2..8) as its layers run, the copy of the next layers
+ * overlapping the current layer's kernels.  Replaces the reference's streamed weights:
+ * BF's two device buffers refilled per layer from pinned memory (src/SDGC/BF/BF.hpp:
+ * 168-178) and SNIG's weight-buffer copies inside its graph (src/SDGC/SNIG/SNIG.hpp:
+ * 207-255).  Results are unchanged.  Dense input/output layers and per-neuron biases stay
+ * in device memory.  Call before creating plans on the net; it cannot be undone. */
+int snicit_net_stream_weights(snicit_net* net, int32_t buffers);
 int snicit_net_destroy(snicit_net* net);
 int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,
                     int64_t* panels_total, int64_t* device_bytes);

@@ -132,6 +132,7 @@ SIGNATURES = {
                                              ctypes.c_float, ctypes.POINTER(_P)]),
     "snicit_net_set_io": (ctypes.c_int, [_P, ctypes.c_int32, _F32P, _F32P, ctypes.c_float,
                                          ctypes.c_int32, _F32P, _F32P]),
+    "snicit_net_stream_weights": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_net_destroy": (ctypes.c_int, [_P]),
     "snicit_net_info": (ctypes.c_int, [_P, _I32P, _I32P, _I32P,
                                        ctypes.POINTER(ctypes.c_int64),
@@ -367,6 +368,11 @@ class Net:
         _check(lib().snicit_net_set_io(self.h, Nin, _f32(wi), _f32(bi), in_clip, Nout, _f32(wo),
                                        _f32(bo)))
         self.Nin, self.Nout = Nin, Nout
+
+    def stream_weights(self, buffers=2):
+        """Move the hidden layers' weights to pinned host memory; plans copy each layer into
+        one of `buffers` device slots as it runs.  Call before creating plans."""
+        _check(lib().snicit_net_stream_weights(self.h, buffers))
 
     def close(self):
         if self.h:

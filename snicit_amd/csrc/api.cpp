@@ -190,6 +190,8 @@ void net_set_bias(Net* net, int32_t bias_kind, const float* bias) {
 }
 
 void destroy_net(Net* net) {
+  if (net->h_blocks) (void)hipHostFree(net->h_blocks);
+  net->h_blocks = nullptr;
   dev_free(net->d_ecols);
   dev_free(net->d_evals);
   dev_free(net->d_erng);
@@ -222,6 +224,83 @@ bool compacting(const snicit_plan* p) {
 // Input of hidden layer 0: the input-layer output when the net has one.
 const float* hidden_in(const snicit_plan* p) { return p->net->Nin > 0 ? p->d_h0 : p->d_in; }
 
+LayerArrays layer_arrays(const Net& net, int l, const char* slot) {
+  LayerArrays a;
+  if (slot) {
+    const BlockLayout& b = net.blk[l];
+    auto at = [&](int64_t off) { return off < 0 ? nullptr : slot + off; };
+    a.pcols = reinterpret_cast<const int32_t*>(at(b.pcols));
+    a.pbeg = reinterpret_cast<const int32_t*>(at(b.pbeg));
+    a.outs = reinterpret_cast<const int32_t*>(at(b.outs));
+    a.w = reinterpret_cast<const float*>(at(b.w));
+    a.ecols = reinterpret_cast<const int32_t*>(at(b.ecols));
+    a.evals = reinterpret_cast<const float*>(at(b.evals));
+    a.erng = reinterpret_cast<const int32_t*>(at(b.erng));
+    a.eent = reinterpret_cast<const int2*>(at(b.eent));
+    return a;
+  }
+  if (net.is_ell(l)) {
+    a.ecols = net.d_ecols + net.ell_off[l];
+    a.evals = net.d_evals + net.ell_off[l];
+    a.erng = net.d_erng + net.erng_off[l];
+    a.eent = net.d_eent + net.eent_off[l];
+  } else {
+    a.pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
+    a.pbeg = net.d_pbeg + net.panel_off[l] + l;
+    a.outs = net.d_outs + (size_t)l * net.N;
+    a.w = net.d_w + net.w_off[l];
+  }
+  return a;
+}
+
+// Layer l's arrays for this plan: resident, or its block in slot l mod stream_buffers.
+LayerArrays plan_layer_arrays(const snicit_plan* p, int l) {
+  const Net& net = *p->net;
+  return layer_arrays(net, l, net.streamed() ? p->d_wslot[l % net.stream_buffers] : nullptr);
+}
+
+// Streamed nets, within one stage on the plan's stream.  weights_begin starts the stage's
+// bookkeeping.  fetch_layer copies layer l's block into slot l mod slots on the copy stream
+// (forking it off the plan's stream at the stage's first copy), after the layer that last
+// held the slot in this stage is done with it.  use_layer fetches layer l and the next
+// slots-1 layers of the stage, then makes the plan's stream wait for layer l's copy;
+// done_layer frees the slot after the layer's kernels.  Every block fetched in a stage is
+// used in that stage, so the copy stream joins the plan's stream again before the stage
+// ends (graph capture needs that).
+void weights_begin(snicit_plan* p) {
+  if (!p->net->streamed()) return;
+  std::fill(p->slot_layer.begin(), p->slot_layer.end(), -1);
+  p->copy_forked = false;
+}
+
+void fetch_layer(snicit_plan* p, int l) {
+  const Net& net = *p->net;
+  const int k = l % net.stream_buffers;
+  if (p->slot_layer[k] == l) return;
+  if (!p->copy_forked) {
+    SNICIT_HIP(hipEventRecord(p->ev_fork, p->stream));
+    SNICIT_HIP(hipStreamWaitEvent(p->copy_stream, p->ev_fork, 0));
+    p->copy_forked = true;
+  }
+  if (p->slot_layer[k] >= 0) SNICIT_HIP(hipStreamWaitEvent(p->copy_stream, p->ev_free[k], 0));
+  SNICIT_HIP(hipMemcpyAsync(p->d_wslot[k], net.h_blocks + net.blk_off[l], (size_t)net.blk_bytes[l],
+                            hipMemcpyHostToDevice, p->copy_stream));
+  SNICIT_HIP(hipEventRecord(p->ev_copied[k], p->copy_stream));
+  p->slot_layer[k] = l;
+}
+
+void use_layer(snicit_plan* p, int l, int l_end) {
+  const Net& net = *p->net;
+  if (!net.streamed()) return;
+  for (int j = l; j < std::min(l + net.stream_buffers, l_end); ++j) fetch_layer(p, j);
+  SNICIT_HIP(hipStreamWaitEvent(p->stream, p->ev_copied[l % net.stream_buffers], 0));
+}
+
+void done_layer(snicit_plan* p, int l) {
+  if (!p->net->streamed()) return;
+  SNICIT_HIP(hipEventRecord(p->ev_free[l % p->net->stream_buffers], p->stream));
+}
+
 Segs plan_segs(snicit_plan* p) {
   return Segs{p->nseg, p->stride, p->d_seglo, p->d_kseg, p->d_k};
 }
@@ -234,11 +313,11 @@ void launch_layer(snicit_plan* p, const SpmmLaunch& a, hipStream_t s) {
     return;
   }
   EllLaunch e{};
-  e.cols = net.d_ecols + net.ell_off[a.layer];
-  e.vals = net.d_evals + net.ell_off[a.layer];
+  e.cols = a.la.ecols;
+  e.vals = a.la.evals;
   e.K = net.ell_K[a.layer];
-  e.rng = net.d_erng + net.erng_off[a.layer];
-  e.ent = net.d_eent + net.eent_off[a.layer];
+  e.rng = a.la.erng;
+  e.ent = a.la.eent;
   e.rows = net.N;
   e.in_rows = net.N;
   e.bias_vec = net.bias_vec(a.layer);
@@ -319,6 +398,7 @@ SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
   SpmmLaunch a{};
   a.net = p->net;
   a.layer = l;
+  a.la = plan_layer_arrays(p, l);
   a.epilogue = epi;
   a.kcap = p->kcap;
   a.eps_post = p->cfg.lossy_eps_post;
@@ -336,8 +416,10 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
   const bool cmp = compacting(p);
   launch_iota_i32(p->d_orig[0], p->cap, s);
   enqueue_input_layer(p);
+  weights_begin(p);
   int cur = 0;
   for (int l = 0; l < l_end; ++l) {
+    use_layer(p, l, l_end);
     SpmmLaunch a = spmm_base(p, l, EPI_PLAIN);
     a.yin = l == 0 ? hidden_in(p) : p->d_y[(l - 1) & 1];
     a.ldin = p->ld;
@@ -355,6 +437,7 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
     prof_mark(p, l, 0);
     launch_layer(p, a, s);
     prof_mark(p, l, 1);
+    done_layer(p, l);
     if (cmp) {
       launch_compact(p->d_flags, p->d_counts + l, p->d_counts + l + 1, p->d_map, nullptr,
                      p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, s);
@@ -486,12 +569,15 @@ void enqueue_post(snicit_plan* p) {
   // clustering leaves the delta list's original ids in slot cur^1 and centroids in slot 1
   int orig = cluster_cur(p) ^ 1, mc = 1;
   const bool lossy = p->cfg.cluster == SNICIT_CLUSTER_LOSSY;
+  weights_begin(p);
   for (int l = t; l < L; ++l) {
+    use_layer(p, l, L);
     const int q = (l - t) & 1;
     const int32_t* map_in = (l == t && !lossy) ? nullptr : p->d_mapD;
     if (post_layer_fused(*p->net, l) && !p->net->is_ell(l) && p->kcap == kMaxCentroidsExact) {
       prof_mark(p, l, 0);
-      launch_panel_post(*p->net, l, lossy, p->d_C[q], p->d_C[q ^ 1], p->kcap, p->d_k,
+      launch_panel_post(*p->net, l, plan_layer_arrays(p, l), lossy, p->d_C[q], p->d_C[q ^ 1],
+                        p->kcap, p->d_k,
                         q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
                         p->d_nerow + l, map_in, p->d_mcar[mc], p->d_flags,
                         p->cfg.lossy_eps_post, p->cap, p->shape, s);
@@ -524,6 +610,7 @@ void enqueue_post(snicit_plan* p) {
       launch_layer(p, d, s);
       prof_mark(p, l, 1);
     }
+    done_layer(p, l);
     launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
                    p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1], s);
     orig ^= 1;
@@ -734,6 +821,15 @@ void plan_stats(snicit_plan* p, snicit_stats* stats, const float* stage_ms) {
 
 void plan_free(snicit_plan* p) {
   drop_graphs(p);
+  if (p->copy_stream) (void)hipStreamSynchronize(p->copy_stream);
+  for (char* q : p->d_wslot) dev_free(q);
+  p->d_wslot.clear();
+  for (auto* v : {&p->ev_copied, &p->ev_free})
+    for (auto& e : *v)
+      if (e) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
+  p->copy_stream = nullptr;
   for (auto& e : p->prof_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : p->ev)
@@ -973,6 +1069,101 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
   });
 }
 
+int snicit_net_stream_weights(snicit_net* net, int32_t buffers) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net != nullptr, "null net");
+    SNICIT_REQUIRE(buffers >= 2 && buffers <= 8, "buffers must be in [2, 8]");
+    SNICIT_REQUIRE(!net->streamed(), "the net's weights are already streamed");
+    SNICIT_REQUIRE(net->live_plans == 0, "stream the weights before creating plans on the net");
+    SNICIT_HIP(hipSetDevice(net->ctx->device));
+    const int L = net->L;
+    const int64_t N = net->N;
+    const int64_t nch = ell_chunks(net->N);
+    auto align = [](int64_t b) { return (b + 255) / 256 * 256; };
+    std::vector<BlockLayout> blk(L);
+    std::vector<int64_t> bytes(L), eent_len(L, 0);
+    int64_t freed = 0;  // device bytes of the arrays that move
+    for (int l = 0; l < L; ++l) {
+      BlockLayout& b = blk[l];
+      int64_t at = 0;
+      if (net->is_ell(l)) {
+        const int64_t K = net->ell_K[l];
+        b.ecols = at;
+        at = align(at + N * K * 4);
+        b.evals = at;
+        at = align(at + N * K * 4);
+        b.erng = at;
+        at = align(at + N * (nch + 1) * 4);
+        // the layer's stream length is its last row's end (relative to the layer)
+        int32_t end = 0;
+        SNICIT_HIP(hipMemcpy(&end, net->d_erng + net->erng_off[l] + N * (nch + 1) - 1, 4,
+                             hipMemcpyDeviceToHost));
+        eent_len[l] = end;
+        b.eent = at;
+        at = align(at + ((int64_t)end + kEllGroup) * 8);
+        freed += N * K * 8 + N * (nch + 1) * 4 + (int64_t)end * 8;
+      } else {
+        const int64_t P = net->panels(l);
+        const int64_t wf = net->mfma[l] ? P * kSlots * 32 : N * kSlots;
+        b.pcols = at;
+        at = align(at + P * kSlots * 4);
+        b.pbeg = at;
+        at = align(at + (P + 1) * 4);
+        b.outs = at;
+        at = align(at + N * 4);
+        b.w = at;
+        at = align(at + wf * 4);
+        freed += P * kSlots * 4 + (P + 1) * 4 + N * 4 + wf * 4;
+      }
+      bytes[l] = at;
+    }
+    std::vector<int64_t> off(L + 1, 0);
+    for (int l = 0; l < L; ++l) off[l + 1] = off[l] + bytes[l];
+    char* h = nullptr;
+    SNICIT_HIP(hipHostMalloc((void**)&h, (size_t)std::max<int64_t>(off[L], 256), 0));
+    try {
+      for (int l = 0; l < L; ++l) {
+        const LayerArrays a = layer_arrays(*net, l, nullptr);
+        const BlockLayout& b = blk[l];
+        char* base = h + off[l];
+        if (net->is_ell(l)) {
+          const int64_t K = net->ell_K[l];
+          SNICIT_HIP(hipMemcpy(base + b.ecols, a.ecols, N * K * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.evals, a.evals, N * K * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.erng, a.erng, N * (nch + 1) * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.eent, a.eent, eent_len[l] * 8, hipMemcpyDeviceToHost));
+          // one group of zero slack after the stream, as in device memory
+          std::memset(base + b.eent + eent_len[l] * 8, 0, kEllGroup * 8);
+        } else {
+          const int64_t P = net->panels(l);
+          const int64_t wf = net->mfma[l] ? P * kSlots * 32 : N * kSlots;
+          SNICIT_HIP(hipMemcpy(base + b.pcols, a.pcols, P * kSlots * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.pbeg, a.pbeg, (P + 1) * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.outs, a.outs, N * 4, hipMemcpyDeviceToHost));
+          SNICIT_HIP(hipMemcpy(base + b.w, a.w, wf * 4, hipMemcpyDeviceToHost));
+        }
+      }
+    } catch (...) {
+      (void)hipHostFree(h);
+      throw;
+    }
+    for (void* q : {(void*)net->d_pcols, (void*)net->d_pbeg, (void*)net->d_outs,
+                    (void*)net->d_w, (void*)net->d_ecols, (void*)net->d_evals,
+                    (void*)net->d_erng, (void*)net->d_eent})
+      dev_free(q);
+    net->d_pcols = net->d_pbeg = net->d_outs = net->d_ecols = net->d_erng = nullptr;
+    net->d_w = net->d_evals = nullptr;
+    net->d_eent = nullptr;
+    net->device_bytes = std::max<int64_t>(0, net->device_bytes - freed);
+    net->h_blocks = h;
+    net->blk = std::move(blk);
+    net->blk_off.assign(off.begin(), off.end() - 1);
+    net->blk_bytes = bytes;
+    net->blk_max = std::max<int64_t>(256, *std::max_element(bytes.begin(), bytes.end()));
+    net->stream_buffers = buffers;
+  });
+}
+
 int snicit_net_destroy(snicit_net* net) {
   return guarded([&] {
     if (!net) return;
@@ -1091,6 +1282,21 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_HIP(hipMemset(p->d_C[1], 0, (size_t)N * p->kcap * 4));
       SNICIT_HIP(hipMemset(p->d_zc, 0, (size_t)N * p->kcap * 4));
       for (auto& e : p->ev) SNICIT_HIP(hipEventCreate(&e));
+      if (net->streamed()) {
+        const int nb = net->stream_buffers;
+        p->slot_layer.assign(nb, -1);
+        p->d_wslot.assign(nb, nullptr);
+        p->ev_copied.assign(nb, nullptr);
+        p->ev_free.assign(nb, nullptr);
+        for (int i = 0; i < nb; ++i) {
+          p->d_wslot[i] = dev_alloc<char>((size_t)net->blk_max, acct);
+          SNICIT_HIP(hipEventCreateWithFlags(&p->ev_copied[i], hipEventDisableTiming));
+          SNICIT_HIP(hipEventCreateWithFlags(&p->ev_free[i], hipEventDisableTiming));
+        }
+        SNICIT_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+        // default priority, like every stream of the library
+        SNICIT_HIP(hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking));
+      }
     } catch (...) {
       plan_free(p.get());
       throw;
@@ -1562,6 +1768,12 @@ double time_layer0(snicit_plan* p, const LaunchShape& shape, const int32_t* coun
   a.count = count;
   a.shape = shape;
   hipStream_t s = p->stream;
+  const Net& net = *p->net;
+  if (net.streamed()) {  // layer 0's block into its slot, after any copy still running
+    SNICIT_HIP(hipStreamSynchronize(p->copy_stream));
+    SNICIT_HIP(hipMemcpyAsync(p->d_wslot[0], net.h_blocks + net.blk_off[0],
+                              (size_t)net.blk_bytes[0], hipMemcpyHostToDevice, s));
+  }
   for (int i = 0; i < 2; ++i) launch_layer(p, a, s);
   std::vector<float> ms(7);
   for (float& m : ms) {

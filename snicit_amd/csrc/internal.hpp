@@ -67,6 +67,25 @@ struct HostLayerPanels {
 //  slot-major w[p][r][g], g<32  (row g of panel p, zero for g >= rows), the matrix-core
 //             operand layout: one 32-float run per slot.
 
+// Device arrays of one hidden layer, as the layer kernels read them: resident in the net's
+// arrays, or in a plan's device slot holding the layer's streamed block.
+struct LayerArrays {
+  const int32_t* pcols = nullptr;  // panel layers: [P][kSlots]
+  const int32_t* pbeg = nullptr;   //   [P + 1]
+  const int32_t* outs = nullptr;   //   [N]
+  const float* w = nullptr;        //   row-major or slot-major weights
+  const int32_t* ecols = nullptr;  // wide-row layers: [N][K]
+  const float* evals = nullptr;    //   [N][K]
+  const int32_t* erng = nullptr;   //   [N][chunks + 1]
+  const int2* eent = nullptr;      //   entry stream (+ one group of zero slack)
+};
+
+// Byte offsets of a layer's arrays inside its streamed block (-1: the layer has none).
+struct BlockLayout {
+  int64_t pcols = -1, pbeg = -1, outs = -1, w = -1, ecols = -1, evals = -1, erng = -1,
+          eent = -1;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -113,11 +132,22 @@ struct Net {
   float* d_bin = nullptr;   // [N]
   float* d_wout = nullptr;  // [Nout][Kout]
   float* d_bout = nullptr;  // [Nout]
+  // Streamed hidden layers (snicit_net_stream_weights, the reference's double-buffered
+  // weight copies, src/SDGC/BF/BF.hpp:168-178): layer l's arrays sit in one pinned host
+  // block of blk_bytes[l] bytes at h_blocks + blk_off[l], laid out as blk[l]; the device
+  // copies above are freed, and every plan copies the blocks into stream_buffers device
+  // slots of blk_max bytes as its layers run.
+  int32_t stream_buffers = 0;  // 0: every layer resident in device memory
+  char* h_blocks = nullptr;
+  std::vector<int64_t> blk_off, blk_bytes;
+  std::vector<BlockLayout> blk;
+  int64_t blk_max = 0;
   int64_t nnz_hidden = 0;   // nonzeros per hidden layer as given (edges per sample per layer)
   int64_t device_bytes = 0;
   int32_t live_plans = 0;   // plans sized for this net's input/output layers
 
   bool has_io() const { return Nin > 0 || Nout > 0; }
+  bool streamed() const { return stream_buffers > 0; }
   bool is_ell(int l) const { return ell_K[l] > 0; }
 
   int64_t panels(int l) const { return panel_off[l + 1] - panel_off[l]; }
@@ -130,6 +160,10 @@ struct Net {
     return bias_kind == SNICIT_BIAS_PER_NEURON ? d_bias + (size_t)l * N : nullptr;
   }
 };
+
+// Layer l's arrays: the net's resident arrays (slot null), or the layer's block copied to
+// the device at `slot` (streamed nets).
+LayerArrays layer_arrays(const Net& net, int l, const char* slot);
 
 // ---- host-side generators (gen.cpp) -------------------------------------------
 uint64_t sm_mix(uint64_t z);

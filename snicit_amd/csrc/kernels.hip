@@ -1746,10 +1746,10 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const int64_t gy = grid_rows(a.shape.grid_target, P, tiles);
   SNICIT_REQUIRE(P <= 2147483647LL, "too many panels");
   dim3 grid((unsigned)P, (unsigned)gy);
-  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
-  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
-  const int32_t* outs = net.d_outs + (size_t)l * net.N;
-  const float* w = net.d_w + net.w_off[l];
+  const int32_t* pcols = a.la.pcols;
+  const int32_t* pbeg = a.la.pbeg;
+  const int32_t* outs = a.la.outs;
+  const float* w = a.la.w;
   const float* bv = net.bias_vec(l);
   const float bs = net.bias_scalar(l);
   const bool mfma = net.mfma[l] != 0;
@@ -1777,21 +1777,23 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
 
 bool post_layer_fused(const Net& net, int l) { return net.mfma[l] == 0; }
 
-void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
-                           int32_t kcap, const int32_t* k, hipStream_t s) {
+void launch_panel_centroid(const Net& net, int l, const LayerArrays& la, const float* cin,
+                           float* cout, float* zc, int32_t kcap, const int32_t* k,
+                           hipStream_t s) {
   SNICIT_REQUIRE(net.mfma[l] == 0, "centroid kernel needs row-major weights");
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "centroid kernel expects 32 centroid slots");
-  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
-  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
-  const int32_t* outs = net.d_outs + (size_t)l * net.N;
-  const float* w = net.d_w + net.w_off[l];
+  const int32_t* pcols = la.pcols;
+  const int32_t* pbeg = la.pbeg;
+  const int32_t* outs = la.outs;
+  const float* w = la.w;
   hipLaunchKernelGGL(panel_centroid_kernel, dim3((unsigned)net.panels(l)), dim3(kBlock), 0, s,
                      pcols, pbeg, outs, w, net.bias_vec(l), net.bias_scalar(l), net.clip, cin,
                      cout, zc, kcap, k);
   check_launch();
 }
 
-void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, float* cout,
+void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
+                       const float* cin, float* cout,
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
@@ -1802,10 +1804,10 @@ void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, floa
   const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
   const int64_t gy = grid_rows(shape.grid_target, P, tiles);
   dim3 grid((unsigned)P, (unsigned)gy);
-  const int32_t* pcols = net.d_pcols + (size_t)net.panel_off[l] * kSlots;
-  const int32_t* pbeg = net.d_pbeg + net.panel_off[l] + l;
-  const int32_t* outs = net.d_outs + (size_t)l * net.N;
-  const float* w = net.d_w + net.w_off[l];
+  const int32_t* pcols = la.pcols;
+  const int32_t* pbeg = la.pbeg;
+  const int32_t* outs = la.outs;
+  const float* w = la.w;
   if (lossy)
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,

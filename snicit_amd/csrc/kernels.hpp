@@ -44,6 +44,7 @@ struct LaunchShape {
 struct SpmmLaunch {
   const Net* net;
   int layer;
+  LayerArrays la;        // the layer's device arrays
   int epilogue;
   const float* yin;
   int64_t ldin;
@@ -65,14 +66,16 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
 // and delta rows (din -> dout through map, centroid index mcent).  Needs row-major weights
 // (post_layer_fused); otherwise the centroid and delta epilogues of launch_panel_spmm run.
 bool post_layer_fused(const Net& net, int l);
-void launch_panel_post(const Net& net, int l, bool lossy, const float* cin, float* cout,
+void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
+                       const float* cin, float* cout,
                        int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
                        int64_t ldout, const int32_t* count, const int32_t* map,
                        const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
                        const LaunchShape& shape, hipStream_t s);
 // Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
-void launch_panel_centroid(const Net& net, int l, const float* cin, float* cout, float* zc,
-                           int32_t kcap, const int32_t* k, hipStream_t s);
+void launch_panel_centroid(const Net& net, int l, const LayerArrays& la, const float* cin,
+                           float* cout, float* zc, int32_t kcap, const int32_t* k,
+                           hipStream_t s);
 
 // Wide ELL layer (rows with more than 32 nonzeros, or a dense layer when cols is null):
 // rows x K entries, ascending input per row, zero-padded; yin has any number of rows.

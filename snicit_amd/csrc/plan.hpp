@@ -77,6 +77,14 @@ struct snicit_plan {
   // per-launch profiling (eager only): events [4*l .. 4*l+3] = main start/stop, centroid
   bool profiling = false;
   std::vector<hipEvent_t> prof_ev;
+  // streamed nets: device slots for layer blocks (layer l in slot l mod slots), the copy
+  // stream, per-slot events (block copied, slot free) and the fork event of a stage
+  std::vector<char*> d_wslot;
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> ev_copied, ev_free;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<int32_t> slot_layer;  // layer each slot holds in the stage being enqueued
+  bool copy_forked = false;         // the copy stream has joined the stage being enqueued
 };
 
 namespace snicit {
