@@ -403,6 +403,9 @@ def main():
                          "write per-layer rows to gpurun_out/pmc_rows_<mode>.json")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="three 1 GiB device-to-device copies (torch), for scaling counters")
+    ap.add_argument("--stream-weights", type=int, default=0, metavar="SLOTS",
+                    help="keep the hidden layers' weights in pinned host memory and stream "
+                         "them through SLOTS device buffers (c1-c3)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="read, write and copy rates of 2 GiB device buffers (torch)")
     args = ap.parse_args()
@@ -445,6 +448,12 @@ def main():
     log(f"net N={N} L={L}: {net.panels} panels, {net.device_bytes / 1e9:.2f} GB resident, "
         f"{time.perf_counter() - t0:.1f} s")
     P = net.panels // L
+    weights = "resident in device memory"
+    if args.stream_weights:
+        net.stream_weights(args.stream_weights)
+        weights = (f"streamed: pinned host blocks copied into {args.stream_weights} device "
+                   f"slots per layer")
+        log(f"weights {weights}; {net.device_bytes / 1e9:.2f} GB left on the device")
 
     if args.pmc_pass:
         plan = sa.Plan(net, sa.default_cfg(args.pmc_pass, threshold=t, use_graph=0), B)
@@ -536,7 +545,8 @@ def main():
         "data": "synthetic (seeded SDGC-like banded+permuted net, binary blob images)",
         "config": {"workload": f"SDGC-like SNICIT vs plain, N={N} L={L} batch={B} "
                                f"bias={wl['bias']} t={t}", "N": N, "L": L, "batch": B,
-                   "threshold": t, "bias": wl["bias"], "clip": 32.0, "nnz_per_row": 32},
+                   "threshold": t, "bias": wl["bias"], "clip": 32.0, "nnz_per_row": 32,
+                   "weights": weights},
         "plain_bf_ms_per_step": ms_b,
         "plain_bf_edges_per_s": edges / (ms_b / 1000.0),
         "plain_dense_ms_per_step": ms_d,

@@ -73,7 +73,8 @@ typedef struct snicit_run_cfg {
   float lossy_eps_pool;    /* lossy: pruning tolerance between pool rows (0.03)       */
   float lossy_eps_assign;  /* lossy: L0 tolerance and delta pruning at clustering (0.04)*/
   float lossy_eps_post;    /* lossy: delta pruning after each post layer (0.05)       */
-  int32_t use_graph;       /* 1: replay the layer loop as captured hipGraphs          */
+  int32_t use_graph;       /* 1: replay the layer loop as captured hipGraphs (plans   */
+                           /* on nets with streamed weights always run directly)      */
   int32_t lossy_positive_only; /* lossy: a delta row is non-empty iff some entry > 0 at
                               * clustering (the reference's test, beyond/SNICIT/kernel.hpp:95);
                               * 0: iff some entry != 0                                    */
@@ -147,7 +148,10 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
  * BF's two device buffers refilled per layer from pinned memory (src/SDGC/BF/BF.hpp:
  * 168-178) and SNIG's weight-buffer copies inside its graph (src/SDGC/SNIG/SNIG.hpp:
  * 207-255).  Results are unchanged.  Dense input/output layers and per-neuron biases stay
- * in device memory.  Call before creating plans on the net; it cannot be undone. */
+ * in device memory.  Plans on the net issue their stages directly rather than as captured
+ * graphs, so the copies run on a DMA engine beside the layer kernels; a batch still
+ * captures one graph, in which the copies run in line with the kernels.  Call before
+ * creating plans on the net; it cannot be undone. */
 int snicit_net_stream_weights(snicit_net* net, int32_t buffers);
 int snicit_net_destroy(snicit_net* net);
 int snicit_net_info(const snicit_net* net, int32_t* N, int32_t* L, int32_t* K,

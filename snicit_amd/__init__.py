@@ -373,6 +373,9 @@ class Net:
         """Move the hidden layers' weights to pinned host memory; plans copy each layer into
         one of `buffers` device slots as it runs.  Call before creating plans."""
         _check(lib().snicit_net_stream_weights(self.h, buffers))
+        b = ctypes.c_int64()
+        _check(lib().snicit_net_info(self.h, None, None, None, None, ctypes.byref(b)))
+        self.device_bytes = b.value
 
     def close(self):
         if self.h:

@@ -641,10 +641,14 @@ void enqueue_recover(snicit_plan* p) {
                       p->d_nerow + L, p->cap, p->last_C, p->kcap, N, p->d_act, p->d_cat, s);
 }
 
-// Runs `body` directly or through a captured graph cached in slot `g`.
+// Runs `body` directly or through a captured graph cached in slot `g`.  Plans on streamed
+// nets run directly: a captured graph runs its host-to-device copy nodes as blit kernels on
+// the queue it is launched on, in line with the layer kernels (measured: every copy then
+// adds its full time), while directly issued copies run on the copy stream's DMA engine
+// beside them.
 void run_stage(snicit_plan* p, int g, const std::function<void()>& body) {
   hipStream_t s = p->stream;
-  if (!p->cfg.use_graph) {
+  if (!p->cfg.use_graph || p->net->streamed()) {
     body();
     return;
   }
