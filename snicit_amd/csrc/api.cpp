@@ -502,7 +502,7 @@ void enqueue_cluster_lossy(snicit_plan* p) {
   launch_segment_bounds(p->d_orig[0], p->d_counts, p->d_segstart, sg, s);
   launch_lossy_pool(st.Yt, p->ld, sg, c.lossy_pool, N, p->d_poolrows, p->d_poolcol, s);
   launch_lossy_select(p->d_poolrows, p->d_poolcol, sg, c.lossy_pool, N, c.lossy_eps_pool,
-                      p->d_cent_slot, s);
+                      p->d_close, p->d_cent_slot, s);
   launch_gather_centroids(st.Yt, p->ld, nullptr, p->d_cent_slot, sg, N, p->d_C[0], p->kcap, s);
   launch_lossy_assign(st.Yt, p->ld, sg, p->cap, p->d_C[0], p->kcap, p->d_cent_slot, N,
                       c.lossy_eps_assign, p->d_dist, p->d_mcar[0], p->d_centmark, p->d_rec, s);
@@ -843,7 +843,7 @@ void plan_free(snicit_plan* p) {
                   p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
                   p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
-                  p->d_centmark, p->d_segstart, p->d_seglo, p->d_kseg,
+                  p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
                   p->d_stage};
   for (void* q : ptrs) dev_free(q);
@@ -1248,6 +1248,7 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         p->d_poolrows = dev_alloc<float>((size_t)kMaxCentroidsLossy * nseg * N, acct);
         p->d_poolcol = dev_alloc<int32_t>((size_t)kMaxCentroidsLossy * nseg, acct);
         p->d_centmark = dev_alloc<int32_t>(p->ld, acct);
+        p->d_close = dev_alloc<uint64_t>((size_t)kMaxCentroidsLossy * 2 * nseg, acct);
       }
       p->d_segstart = dev_alloc<int32_t>(nseg + 1, acct);
       p->d_seglo = dev_alloc<int32_t>(nseg + 1, acct);

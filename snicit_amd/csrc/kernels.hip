@@ -1005,29 +1005,31 @@ __global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict
 
 // Pruning (y_star_gen, src/beyond/SNICIT/kernel.hpp:19-41): in pool order, every present
 // base i removes each other present row y with #{n : |P_y[n] - P_i[n]| > eps} < N*eps.
-// One workgroup per batch segment; wavefront w tests rows w, w+16, ...; lanes stride the
-// neurons, 256 per step.  The count only grows, so a pair stops as soon as it reaches N*eps
-// (the row stays): most pairs of different samples are settled within the first steps.
-// Centroids (columns) go to cent_col[s*stride ...]; kcols (zero on entry) becomes the
-// number of centroid columns in use.
-__global__ __launch_bounds__(1024) void lossy_select_kernel(
-    const float* __restrict__ poolrows, const int32_t* __restrict__ poolcol,
-    const int32_t* __restrict__ seglo, int32_t nseg, int32_t pool, int32_t N, float eps,
-    int32_t stride, int32_t* __restrict__ cent_col, int32_t* __restrict__ kseg,
-    int32_t* __restrict__ kcols) {
-  __shared__ int32_t alive[kMaxCentroidsLossy];
-  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave, sg = blockIdx.x;
+// Two kernels.  lossy_close_kernel: workgroup (row i, batch segment) decides for every later
+// pool row y > i whether it is that close to row i, wavefront w testing rows i+1+w,
+// i+17+w, ...; lanes stride the neurons, 256 per step, and a pair stops as soon as its count
+// reaches N*eps (the count only grows), so most pairs of different rows end after a step or
+// two.  The pair tests are independent, so every pool row's tests run at once.  Row i's
+// 128-bit mask of close later rows goes to close[segment][i].  lossy_prune_kernel: one
+// wavefront per segment walks the rows in pool order; a row still present removes the rows
+// in its mask.  A present earlier row close to row i would have removed i, so later rows are
+// the only ones the walk can remove, and the survivors are those of the sequential rule.
+__global__ __launch_bounds__(1024) void lossy_close_kernel(const float* __restrict__ poolrows,
+                                                          const int32_t* __restrict__ seglo,
+                                                          int32_t pool, int32_t N, float eps,
+                                                          uint64_t* __restrict__ close) {
+  __shared__ uint64_t bits[2];
+  const int i = blockIdx.x, sg = blockIdx.y;
+  const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
   const int64_t Bs = seglo[sg + 1] - seglo[sg];
   const int P = Bs > 0 ? pool : 0;
-  const float* __restrict__ rows = poolrows + (int64_t)sg * pool * N;
-  for (int y = t; y < kMaxCentroidsLossy; y += blockDim.x) alive[y] = y < P;
+  if (t < 2) bits[t] = 0ull;
   __syncthreads();
-  const double limit = (double)N * (double)eps;
-  for (int i = 0; i < P; ++i) {
-    if (!alive[i]) continue;  // uniform: read after the barrier below
+  if (i < P) {
+    const float* __restrict__ rows = poolrows + (int64_t)sg * pool * N;
     const float* __restrict__ pi = rows + (int64_t)i * N;
-    for (int y = wv; y < P; y += 16) {
-      if (y == i || !alive[y]) continue;
+    const double limit = (double)N * (double)eps;
+    for (int y = i + 1 + wv; y < P; y += 16) {
       const float* __restrict__ py = rows + (int64_t)y * N;
       int cnt = 0;
       for (int n0 = 0; n0 < N && (double)cnt < limit; n0 += 4 * kWave) {
@@ -1038,14 +1040,38 @@ __global__ __launch_bounds__(1024) void lossy_select_kernel(
           cnt += __popcll(__ballot(d));
         }
       }
-      if ((double)cnt < limit && lane == 0) alive[y] = 0;
+      if ((double)cnt < limit && lane == 0) atomicOr(&bits[y >> 6], 1ull << (y & 63));
     }
-    __syncthreads();
   }
-  if (t == 0) {
+  __syncthreads();
+  if (t < 2) close[((int64_t)sg * kMaxCentroidsLossy + i) * 2 + t] = bits[t];
+}
+
+__global__ __launch_bounds__(kWave) void lossy_prune_kernel(
+    const uint64_t* __restrict__ close, const int32_t* __restrict__ poolcol,
+    const int32_t* __restrict__ seglo, int32_t pool, int32_t stride,
+    int32_t* __restrict__ cent_col, int32_t* __restrict__ kseg, int32_t* __restrict__ kcols) {
+  __shared__ uint64_t m[kMaxCentroidsLossy * 2];
+  const int sg = blockIdx.x, lane = threadIdx.x;
+  const int64_t Bs = seglo[sg + 1] - seglo[sg];
+  const int P = Bs > 0 ? pool : 0;
+  const uint64_t* __restrict__ cs = close + (int64_t)sg * kMaxCentroidsLossy * 2;
+  for (int q = lane; q < 2 * P; q += kWave) m[q] = cs[q];
+  __syncthreads();
+  // present rows: bits 0..P-1 of (a0, a1)
+  uint64_t a0 = P >= 64 ? ~0ull : (1ull << P) - 1ull;
+  uint64_t a1 = P >= 128 ? ~0ull : P > 64 ? (1ull << (P - 64)) - 1ull : 0ull;
+  for (int i = 0; i < P; ++i) {
+    const bool present = ((i < 64 ? a0 >> i : a1 >> (i - 64)) & 1ull) != 0ull;
+    if (present) {
+      a0 &= ~m[2 * i];
+      a1 &= ~m[2 * i + 1];
+    }
+  }
+  if (lane == 0) {
     int k = 0;
     for (int y = 0; y < P; ++y)
-      if (alive[y]) cent_col[sg * stride + k++] = poolcol[sg * pool + y];
+      if ((y < 64 ? a0 >> y : a1 >> (y - 64)) & 1ull) cent_col[sg * stride + k++] = poolcol[sg * pool + y];
     kseg[sg] = k;
     if (k > 0) atomicMax(kcols, sg * stride + k);
   }
@@ -2043,12 +2069,16 @@ void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool,
 }
 
 void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
-                         int32_t pool, int32_t N, float eps, int32_t* cent_col, hipStream_t s) {
-  SNICIT_REQUIRE(pool <= sg.stride, "pool larger than the centroid stride");
+                         int32_t pool, int32_t N, float eps, uint64_t* close, int32_t* cent_col,
+                         hipStream_t s) {
+  SNICIT_REQUIRE(pool >= 1 && pool <= sg.stride && pool <= kMaxCentroidsLossy,
+                 "pool larger than the centroid stride");
   launch_fill_i32(sg.kcols, 0, 1, s);
-  hipLaunchKernelGGL(lossy_select_kernel, dim3((unsigned)sg.nseg), dim3(1024), 0, s, poolrows,
-                     poolcol, sg.seglo, sg.nseg, pool, N, eps, sg.stride, cent_col, sg.kseg,
-                     sg.kcols);
+  hipLaunchKernelGGL(lossy_close_kernel, dim3((unsigned)pool, (unsigned)sg.nseg), dim3(1024), 0,
+                     s, poolrows, sg.seglo, pool, N, eps, close);
+  check_launch();
+  hipLaunchKernelGGL(lossy_prune_kernel, dim3((unsigned)sg.nseg), dim3(kWave), 0, s, close,
+                     poolcol, sg.seglo, pool, sg.stride, cent_col, sg.kseg, sg.kcols);
   check_launch();
 }
 

@@ -114,8 +114,10 @@ void launch_argmax(const float* logits, int64_t ld, int32_t rows, const int32_t*
 // poolrows [nseg*pool][N], poolcol [nseg*pool] (column, -1 in an empty segment)
 void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool, int32_t N,
                        float* poolrows, int32_t* poolcol, hipStream_t s);
+// close: scratch of [nseg][128][2] words (each pool row's mask of close later rows)
 void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
-                         int32_t pool, int32_t N, float eps, int32_t* cent_col, hipStream_t s);
+                         int32_t pool, int32_t N, float eps, uint64_t* close, int32_t* cent_col,
+                         hipStream_t s);
 // dist: [max_cols][128] int32, zero on entry and on exit; centmark[i] = 1 for centroids
 void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max_cols,
                          const float* C, int32_t kcap, const int32_t* cent_col, int32_t N,

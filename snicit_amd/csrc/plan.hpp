@@ -63,6 +63,7 @@ struct snicit_plan {
   float* d_poolrows = nullptr;  // [nseg * pool][N]
   int32_t* d_poolcol = nullptr;
   int32_t* d_centmark = nullptr;
+  uint64_t* d_close = nullptr;  // [nseg][128][2] pool pruning masks
   // where the last run left its final state
   int last_orig = 0, last_mcar = 0;
   const float* last_D = nullptr;
