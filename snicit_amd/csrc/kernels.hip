@@ -1079,45 +1079,83 @@ __global__ __launch_bounds__(kWave) void lossy_prune_kernel(
 
 // Assignment (coarse_cluster, src/beyond/SNICIT/kernel.hpp:63-87): distance = #{n :
 // |C_c[n] - Y_i[n]| > eps} over the centroids of i's batch segment, first minimum; centroid
-// columns map to their own index.  Same wavefront walk over segments as the exact variant.
-__global__ __launch_bounds__(256) void lossy_dist_kernel(
+// columns map to their own index.  Workgroup (64 columns, block of eight centroids): lane =
+// column, wavefront w counts neurons [w N/16, (w+1) N/16) with eight neurons' loads issued
+// before their compares; the 16 partial counts of a (column, centroid) are summed in LDS and
+// stored once, centroid-major (dist[c][i], row stride ld; integers: the order does not
+// matter).  Lanes of different batch segments are
+// handled one segment at a time, as in the exact variant's walk.
+constexpr int kDistWaves = 16;
+
+__global__ __launch_bounds__(kDistWaves * kWave) void lossy_dist_kernel(
     const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ seglo, int32_t nseg,
     const float* __restrict__ C, int32_t kcap, int32_t stride, const int32_t* __restrict__ kseg,
-    int32_t N, int32_t nchunk, float eps, int32_t* __restrict__ dist) {
+    int32_t N, float eps, int32_t* __restrict__ dist) {
+  __shared__ int32_t part[kDistWaves][8][kWave];
   const int64_t count = seglo[nseg];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const int sg = nseg == 1 ? 0 : segment_of(seglo, nseg, i);
-  const int n0 = (int)((int64_t)N * blockIdx.y / nchunk);
-  const int n1 = (int)((int64_t)N * (blockIdx.y + 1) / nchunk);
-  uint64_t todo = __ballot(true);
+  if ((int64_t)blockIdx.x * kWave >= count) return;  // the whole workgroup
+  const int lane = lane_id(), wv = threadIdx.x / kWave;
+  const int64_t i = (int64_t)blockIdx.x * kWave + lane;  // < ld: ld is a multiple of 64
+  const bool valid = i < count;
+  const int sg = !valid ? -1 : nseg == 1 ? 0 : segment_of(seglo, nseg, i);
+  const int cb = 8 * (int)blockIdx.y;
+  const int n0 = (int)((int64_t)N * wv / kDistWaves);
+  const int n1 = (int)((int64_t)N * (wv + 1) / kDistWaves);
+  int tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t todo = __ballot(valid);
   while (todo) {
     const int lead = __ffsll((unsigned long long)todo) - 1;
     const int s0 = __builtin_amdgcn_readfirstlane(__shfl(sg, lead));
-    const bool mine = sg == s0;
+    const bool mine = valid && sg == s0;
     todo &= ~__ballot(mine);
-    const int k = kseg[s0];
-    const float* __restrict__ Cs = C + (int64_t)s0 * stride;
-    for (int cb = 0; cb < k; cb += 8) {
-      int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int n = n0; n < n1; ++n) {
-        const float y = Y[(int64_t)n * ld + i];
-        const float* __restrict__ cr = Cs + (int64_t)n * kcap + cb;
+    if (cb >= kseg[s0]) continue;
+    const float* __restrict__ Cs = C + (int64_t)s0 * stride + cb;
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int n = n0;
+    for (; n + 8 <= n1; n += 8) {
+      float y[8], c[8][8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) cnt[q] += fabsf(cr[q] - y) > eps;
-      }
+      for (int u = 0; u < 8; ++u) y[u] = Y[(int64_t)(n + u) * ld + i];
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (mine && cb + q < k && cnt[q])
-          atomicAdd(&dist[i * kMaxCentroidsLossy + cb + q], cnt[q]);
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) c[u][q] = Cs[(int64_t)(n + u) * kcap + q];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cnt[q] += fabsf(c[u][q] - y[u]) > eps;
+    }
+    for (; n < n1; ++n) {
+      const float y = Y[(int64_t)n * ld + i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) cnt[q] += fabsf(Cs[(int64_t)n * kcap + q] - y) > eps;
+    }
+    if (mine) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) tot[q] += cnt[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) part[wv][q][lane] = tot[q];
+  __syncthreads();
+  if (wv == 0 && valid) {
+    const int k = kseg[sg];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (cb + q >= k) break;
+      int sum = 0;
+#pragma unroll
+      for (int w = 0; w < kDistWaves; ++w) sum += part[w][q][lane];
+      dist[(int64_t)(cb + q) * ld + i] = sum;
     }
   }
 }
 
 __global__ __launch_bounds__(256) void lossy_final_kernel(
     const int32_t* __restrict__ seglo, int32_t nseg, const int32_t* __restrict__ kseg,
-    int32_t stride, const int32_t* __restrict__ cent_col, int32_t* __restrict__ dist,
-    int32_t* __restrict__ mcar, int32_t* __restrict__ centmark, int32_t* __restrict__ rec) {
+    int32_t stride, const int32_t* __restrict__ cent_col, const int32_t* __restrict__ dist,
+    int64_t ld, int32_t* __restrict__ mcar, int32_t* __restrict__ centmark,
+    int32_t* __restrict__ rec) {
   const int64_t count = seglo[nseg];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -1125,13 +1163,17 @@ __global__ __launch_bounds__(256) void lossy_final_kernel(
   const int k = kseg[sg];
   const int32_t* __restrict__ cs = cent_col + sg * stride;
   int best = 0x7fffffff, bestc = 0;
-  for (int c = 0; c < k; ++c) {
-    const int d = dist[i * kMaxCentroidsLossy + c];
-    dist[i * kMaxCentroidsLossy + c] = 0;
-    if (d < best) {
-      best = d;
-      bestc = c;
-    }
+  // sixteen loads in flight, then the first minimum in centroid order
+  for (int c0 = 0; c0 < k; c0 += 16) {
+    int d[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) d[u] = c0 + u < k ? dist[(int64_t)(c0 + u) * ld + i] : 0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (d[u] < best) {
+        best = d[u];
+        bestc = c0 + u;
+      }
   }
   int mark = 0;
   for (int c = 0; c < k; ++c)
@@ -1489,7 +1531,8 @@ __global__ __launch_bounds__(256) void assign_dist_kernel(
     todo &= ~__ballot(mine);
     const int k = kseg[s0];
     const float* __restrict__ Cs = C + (int64_t)s0 * stride;
-    for (int cb = 0; cb < k; cb += 8) {
+    // centroids cb..cb+7 of the segment: one block of eight per grid z index
+    for (int cb = 8 * (int)blockIdx.z; cb < k; cb += 8 * (int)gridDim.z) {
       int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       for (int n = n0; n < n1; ++n) {
         const float y = Y[(int64_t)n * ld + col];
@@ -2089,13 +2132,14 @@ void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max
   SNICIT_REQUIRE(sg.stride % 8 == 0 && sg.stride <= kMaxCentroidsLossy &&
                      kcap >= sg.stride * sg.nseg,
                  "bad centroid layout");
-  const int nchunk = std::max(1, std::min(64, N / 64));
-  dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)nchunk);
-  hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(256), 0, s, Y, ld, sg.seglo, sg.nseg, C, kcap,
-                     sg.stride, sg.kseg, N, nchunk, eps, dist);
+  // (64-column tiles, blocks of eight centroids; a block past a segment's k does nothing)
+  dim3 grid(grid1(max_cols, kWave, 1 << 30), (unsigned)(sg.stride / 8));
+  hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(kDistWaves * kWave), 0, s, Y, ld, sg.seglo,
+                     sg.nseg, C, kcap, sg.stride, sg.kseg, N, eps, dist);
   check_launch();
   hipLaunchKernelGGL(lossy_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
-                     sg.seglo, sg.nseg, sg.kseg, sg.stride, cent_col, dist, mcar, centmark, rec);
+                     sg.seglo, sg.nseg, sg.kseg, sg.stride, cent_col, dist, ld, mcar, centmark,
+                     rec);
   check_launch();
 }
 

@@ -118,7 +118,8 @@ void launch_lossy_pool(const float* Y, int64_t ld, const Segs& sg, int32_t pool,
 void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Segs& sg,
                          int32_t pool, int32_t N, float eps, uint64_t* close, int32_t* cent_col,
                          hipStream_t s);
-// dist: [max_cols][128] int32, zero on entry and on exit; centmark[i] = 1 for centroids
+// dist: [128][ld] int32 scratch (centroid-major, any contents on entry); centmark[i] = 1
+// for centroids
 void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max_cols,
                          const float* C, int32_t kcap, const int32_t* cent_col, int32_t N,
                          float eps, int32_t* dist, int32_t* mcar, int32_t* centmark, int32_t* rec,
