@@ -200,6 +200,7 @@ void destroy_net(Net* net) {
   dev_free(net->d_wint);
   dev_free(net->d_bin);
   dev_free(net->d_wout);
+  dev_free(net->d_woutt);
   dev_free(net->d_bout);
   dev_free(net->d_pcols);
   dev_free(net->d_pbeg);
@@ -348,6 +349,7 @@ void enqueue_input_layer(snicit_plan* p) {
   e.cols = nullptr;
   e.vals = net.d_win;
   e.wt = net.d_wint;
+  e.wt_ld = net.win_ld;
   e.K = net.Kin;
   e.rows = net.N;
   e.in_rows = net.Kin;
@@ -372,6 +374,8 @@ void enqueue_output_layer(snicit_plan* p, const float* y) {
   EllLaunch e{};
   e.cols = nullptr;
   e.vals = net.d_wout;
+  e.wt = net.d_woutt;
+  e.wt_ld = net.wout_ld;
   e.K = net.Kout;
   e.rows = net.Nout;
   e.in_rows = net.Kout;
@@ -1040,8 +1044,9 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
     dev_free(net->d_wint);
     dev_free(net->d_bin);
     dev_free(net->d_wout);
+    dev_free(net->d_woutt);
     dev_free(net->d_bout);
-    net->d_win = net->d_wint = net->d_bin = net->d_wout = net->d_bout = nullptr;
+    net->d_win = net->d_wint = net->d_bin = net->d_wout = net->d_woutt = net->d_bout = nullptr;
     net->Nin = Nin;
     net->Nout = Nout;
     net->Kin = (Nin + 15) / 16 * 16;
@@ -1051,9 +1056,11 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
       std::vector<float> w((size_t)N * net->Kin, 0.f);
       for (int32_t o = 0; o < N; ++o)
         std::memcpy(&w[(size_t)o * net->Kin], w_in + (size_t)o * Nin, (size_t)Nin * 4);
-      std::vector<float> wt((size_t)net->Kin * N, 0.f);
+      net->win_ld = (N + 63) / 64 * 64;
+      const size_t ldw = (size_t)net->win_ld;
+      std::vector<float> wt((size_t)net->Kin * ldw, 0.f);
       for (int32_t o = 0; o < N; ++o)
-        for (int32_t k = 0; k < Nin; ++k) wt[(size_t)k * N + o] = w_in[(size_t)o * Nin + k];
+        for (int32_t k = 0; k < Nin; ++k) wt[(size_t)k * ldw + o] = w_in[(size_t)o * Nin + k];
       net->d_win = dev_alloc<float>(w.size(), &net->device_bytes);
       net->d_wint = dev_alloc<float>(wt.size(), &net->device_bytes);
       net->d_bin = dev_alloc<float>(N, &net->device_bytes);
@@ -1065,9 +1072,16 @@ int snicit_net_set_io(snicit_net* net, int32_t Nin, const float* w_in, const flo
       std::vector<float> w((size_t)Nout * net->Kout, 0.f);
       for (int32_t o = 0; o < Nout; ++o)
         std::memcpy(&w[(size_t)o * net->Kout], w_out + (size_t)o * N, (size_t)N * 4);
+      net->wout_ld = (Nout + 31) / 32 * 32;
+      const size_t ldw = (size_t)net->wout_ld;
+      std::vector<float> wt((size_t)net->Kout * ldw, 0.f);
+      for (int32_t o = 0; o < Nout; ++o)
+        for (int32_t k = 0; k < N; ++k) wt[(size_t)k * ldw + o] = w_out[(size_t)o * N + k];
       net->d_wout = dev_alloc<float>(w.size(), &net->device_bytes);
+      net->d_woutt = dev_alloc<float>(wt.size(), &net->device_bytes);
       net->d_bout = dev_alloc<float>(Nout, &net->device_bytes);
       SNICIT_HIP(hipMemcpy(net->d_wout, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+      SNICIT_HIP(hipMemcpy(net->d_woutt, wt.data(), wt.size() * 4, hipMemcpyHostToDevice));
       SNICIT_HIP(hipMemcpy(net->d_bout, b_out, (size_t)Nout * 4, hipMemcpyHostToDevice));
     }
   });

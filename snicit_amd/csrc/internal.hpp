@@ -128,7 +128,11 @@ struct Net {
   int32_t Nin = 0, Kin = 0, Nout = 0, Kout = 0;
   float in_clip = 1.f;
   float* d_win = nullptr;   // [N][Kin]
-  float* d_wint = nullptr;  // [Kin][N], the same weights transposed (matrix-core operand)
+  float* d_wint = nullptr;  // [Kin][win_ld], the same weights transposed and zero-padded to
+                            //   whole 64-row tiles (matrix-core operand)
+  int32_t win_ld = 0;
+  float* d_woutt = nullptr; // [Kout][wout_ld], the output weights likewise (32-row tiles)
+  int32_t wout_ld = 0;
   float* d_bin = nullptr;   // [N]
   float* d_wout = nullptr;  // [Nout][Kout]
   float* d_bout = nullptr;  // [Nout]

@@ -1681,22 +1681,27 @@ __global__ __launch_bounds__(256) void scatter_delta_kernel(
 // ascending.  With K = 1 per step every output element takes one fused multiply-add per
 // input, so the result is exactly the chain acc = fma(w[K-1], x[K-1], ... fma(w[0], x[0],
 // 0)) of the vector kernels and the oracle (D layout as in panel_mfma_kernel).  Operands
-// come from the transposed weights wt[k][row] and the input rows, kDenseU steps ahead in a
-// second register set.  The four wavefronts of a workgroup take four row tiles of one column
+// come from the transposed weights wt[k][row] and the input rows, U steps ahead in a second
+// register set.  The four wavefronts of a workgroup take four row tiles of one column
 // tile, so the tile's inputs come from L1 after the first.  Columns past the live count read
 // the last live column; their outputs are dropped.  With at most `narrow` live columns every
 // thread of the grid takes (row, column) chains over the row-major weights instead.  A
 // launch handles live counts in [lo, hi) and returns at once otherwise, so two tile forms
 // can be launched back to back and the live count picks one on the device.
-constexpr int kDenseU = 8;
+// Input steps per register set: 16 for the one-accumulator form (its chain is bound by
+// the latency of its loads), 8 for the four-accumulator form (bound by the matrix cores).
+template <int RB, int CB>
+struct DenseSteps {
+  static constexpr int U = RB * CB == 1 ? 16 : 8;
+};
 
 template <int EPI, int RB, int CB>
 __global__ __launch_bounds__(256) void dense_mfma_kernel(
-    const float* __restrict__ wt, const float* __restrict__ vals, int32_t K, int32_t rows,
-    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    const float* __restrict__ wt, int32_t ldw, const float* __restrict__ vals, int32_t K,
+    int32_t rows, const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
     int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
     int32_t narrow, int64_t lo, int64_t hi) {
-  constexpr int U = kDenseU;
+  constexpr int U = DenseSteps<RB, CB>::U;
   const int64_t count = *count_ptr;
   if (count < lo || count >= hi) return;
   if (count <= narrow) {
@@ -1708,7 +1713,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int half = lane >> 5, j = lane & 31;
-  const int row_tiles = rows / (32 * RB);
+  const int row_tiles = ldw / (32 * RB);  // ldw: rows padded to whole tiles (zero weights)
   const int64_t items = (int64_t)row_tiles * ((count + 64 * CB - 1) / (64 * CB));
   for (int64_t it = (int64_t)blockIdx.x * 4 + wv; it < items; it += (int64_t)gridDim.x * 4) {
     const int r0 = (int)(it % row_tiles) * 32 * RB;
@@ -1729,7 +1734,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) a[u][rb] = ap[(int64_t)(k0 + u) * rows + 32 * rb];
+        for (int rb = 0; rb < RB; ++rb) a[u][rb] = ap[(int64_t)(k0 + u) * ldw + 32 * rb];
 #pragma unroll
         for (int cb = 0; cb < CB; ++cb) b[u][cb] = bp[cb][(int64_t)(k0 + u) * ldin];
       }
@@ -1744,13 +1749,27 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(
             acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x1f32(a[u][rb], b[u][cb], acc[rb][cb],
                                                                0, 0, 0);
     };
-    // K is a multiple of 16 = 2 U
-    load(a0, b0, 0);
-    for (int k0 = 0; k0 < K; k0 += 2 * U) {
+    // whole pairs of register sets, then the last K mod 2U inputs one at a time (k ascending)
+    const int Kmain = K - K % (2 * U);
+    if (Kmain > 0) load(a0, b0, 0);
+    for (int k0 = 0; k0 < Kmain; k0 += 2 * U) {
       load(a1, b1, k0 + U);
       mma(a0, b0);
-      if (k0 + 2 * U < K) load(a0, b0, k0 + 2 * U);
+      if (k0 + 2 * U < Kmain) load(a0, b0, k0 + 2 * U);
       mma(a1, b1);
+    }
+    // (K is a multiple of 16: with 2U = 16 there is no tail)
+    for (int k = Kmain; 2 * U > 16 && k < K; ++k) {
+      float a[RB], b[CB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) a[rb] = ap[(int64_t)k * ldw + 32 * rb];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) b[cb] = bp[cb][(int64_t)k * ldin];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x1f32(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
     }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -1758,7 +1777,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int o = r0 + 32 * rb + 8 * (q >> 2) + 4 * half + (q & 3);
-        bias[q] = bias_vec ? bias_vec[o] : bias_s;
+        bias[q] = bias_vec && o < rows ? bias_vec[o] : bias_s;
       }
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
@@ -1772,7 +1791,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(
             bool f = false;
             const float y = epilogue<EPI>(acc[rb][cb][16 * b + q], bias[q], clip, true, o, pos,
                                           0, nullptr, 0, 0.f, f);
-            yout[(int64_t)o * ldout + pos] = y;
+            if (o < rows) yout[(int64_t)o * ldout + pos] = y;
           }
         }
     }
@@ -2014,7 +2033,7 @@ constexpr int64_t kDenseWideCols = 2304;
 
 template <int RB, int CB>
 void launch_dense_mfma_t(const EllLaunch& a, int64_t lo, int64_t hi, hipStream_t s) {
-  const int64_t row_tiles = a.rows / (32 * RB);
+  const int64_t row_tiles = a.wt_ld / (32 * RB);
   const int64_t cols = std::min<int64_t>(a.max_cols, hi);
   const int64_t tiles = (cols + 64 * CB - 1) / (64 * CB);
   // four wavefronts per workgroup, or a thread per (row, column) pair when the count may be
@@ -2023,9 +2042,9 @@ void launch_dense_mfma_t(const EllLaunch& a, int64_t lo, int64_t hi, hipStream_t
   const int64_t most = std::max<int64_t>({1, (row_tiles * tiles + 3) / 4, (pairs + 255) / 256});
   const unsigned grid = (unsigned)std::min<int64_t>(most, (int64_t)8 * std::max(device_cus(), 1));
 #define SNICIT_DENSE(E)                                                                        \
-  hipLaunchKernelGGL((dense_mfma_kernel<E, RB, CB>), dim3(grid), dim3(256), 0, s, a.wt, a.vals, \
-                     a.K, a.rows, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin, a.yout, a.ldout, \
-                     a.count, a.shape.narrow, lo, hi)
+  hipLaunchKernelGGL((dense_mfma_kernel<E, RB, CB>), dim3(grid), dim3(256), 0, s, a.wt,         \
+                     a.wt_ld, a.vals, a.K, a.rows, a.bias_vec, a.bias_s, a.clip, a.yin, a.ldin,  \
+                     a.yout, a.ldout, a.count, a.shape.narrow, lo, hi)
   if (a.epilogue == EPI_PLAIN) SNICIT_DENSE(EPI_PLAIN);
   else SNICIT_DENSE(EPI_LINEAR);
 #undef SNICIT_DENSE
@@ -2033,10 +2052,11 @@ void launch_dense_mfma_t(const EllLaunch& a, int64_t lo, int64_t hi, hipStream_t
 }
 
 // 32 x 64 tiles, and from kDenseWideCols live columns 64 x 128 tiles (both launched when the
-// capacity reaches it; the (row, column) chains stay with the 32 x 64 form)
+// capacity reaches it and the padded rows are whole 64-row tiles; the (row, column) chains
+// stay with the 32 x 64 form)
 void launch_dense_mfma(const EllLaunch& a, hipStream_t s) {
   constexpr int64_t kAll = INT64_MAX;
-  if (a.max_cols >= kDenseWideCols && a.shape.narrow < kDenseWideCols) {
+  if (a.max_cols >= kDenseWideCols && a.shape.narrow < kDenseWideCols && a.wt_ld % 64 == 0) {
     launch_dense_mfma_t<1, 1>(a, 0, kDenseWideCols, s);
     launch_dense_mfma_t<2, 2>(a, kDenseWideCols, kAll, s);
   } else {
@@ -2053,8 +2073,8 @@ void launch_ell_spmm(const EllLaunch& a, hipStream_t s) {
   }
   // dense layers without per-column flags, gathers or centroid epilogues: matrix cores
   if (a.shape.ell_chunked == 2 && !a.cols && a.wt && !a.map && !a.flags &&
-      (a.epilogue == EPI_PLAIN || a.epilogue == EPI_LINEAR) && a.rows % 64 == 0 &&
-      a.in_rows == a.K) {
+      (a.epilogue == EPI_PLAIN || a.epilogue == EPI_LINEAR) && a.wt_ld % 32 == 0 &&
+      a.wt_ld >= a.rows && a.in_rows == a.K) {
     launch_dense_mfma(a, s);
     return;
   }

@@ -84,7 +84,8 @@ struct EllLaunch {
   const float* vals;    // [rows][K]
   const int32_t* rng;   // [rows][ell_chunks(in_rows)+1] ranges in ent, or null
   const int2* ent;      // padded (chunk tile byte offset, weight bits) stream, with rng
-  const float* wt;      // dense layers: [in_rows][rows] transposed weights, or null
+  const float* wt;      // dense layers: [in_rows][wt_ld] transposed weights, or null
+  int32_t wt_ld;        //   rows padded with zero weights to a multiple of 32
   int32_t K, rows;
   int32_t in_rows;      // input neurons the entries refer to (rows of yin read)
   const float* bias_vec;

@@ -88,15 +88,17 @@ def test_model_io_plain_bitexact(ctx, mode):
     np.testing.assert_array_equal(plan.categories(), cat)
 
 
-@pytest.mark.parametrize("B,cap", [(1, 1), (64, 4096), (65, 65), (777, 3000), (2303, 2400),
-                                   (2304, 2304), (8200, 8200)])
-def test_dense_input_layer_forms(ctx, B, cap):
-    """Dense input layer on the matrix cores: sample counts that take the (row, column)
-    chains (<= 64 columns), 32 x 64 wavefront tiles (< 2304 columns) and 64 x 128 tiles,
-    each with a partial last tile, in plans whose capacity launches one tile form or both;
-    300 inputs pad to 304, not a multiple of 32.  The vector path (ell_chunked = 1) gives
-    the same bits."""
-    Nin, N, L, K, Nout = 300, 256, 1, 40, 10
+@pytest.mark.parametrize("B,cap,N", [(1, 1, 256), (64, 4096, 256), (65, 65, 256),
+                                     (777, 3000, 256), (2303, 2400, 256), (2304, 2304, 256),
+                                     (8200, 8200, 256), (777, 3000, 160), (2304, 2304, 160)])
+def test_dense_input_layer_forms(ctx, B, cap, N):
+    """Dense layers on the matrix cores: sample counts that take the (row, column) chains
+    (<= 64 columns), 32 x 64 wavefront tiles (< 2304 columns) and 64 x 128 tiles, each with a
+    partial last tile, in plans whose capacity launches one tile form or both; 300 inputs pad
+    to 304, not a multiple of 32; 160 neurons pad to 192 rows (not whole 64-row tiles, so
+    only the 32 x 64 form runs); the 10-row output layer runs on one zero-padded 32-row
+    tile.  The vector path (ell_chunked = 1) gives the same bits."""
+    Nin, L, K, Nout = 300, 1, 40, 10
     m = _model(Nin, N, L, K, Nout, seed=11)
     x = sa.gen_features(Nin, 5, 0, B)
     net = sa.Net.from_ell(ctx, m["cols"], m["vals"], 0.0, clip=1.0)
