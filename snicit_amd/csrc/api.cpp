@@ -444,7 +444,7 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
     done_layer(p, l);
     if (cmp) {
       launch_compact(p->d_flags, p->d_counts + l, p->d_counts + l + 1, p->d_map, nullptr,
-                     p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, s);
+                     p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, p->cap, p->d_tilecnt, s);
       cur ^= 1;
     }
   }
@@ -515,7 +515,7 @@ void enqueue_cluster_lossy(snicit_plan* p) {
                      p->d_flags, s);
   // non-empty delta rows: positions in Da, with original id and centroid carried
   launch_compact(p->d_flags, p->d_counts, p->d_nerow + t, p->d_mapD, nullptr, p->d_orig[0],
-                 p->d_orig[1], p->d_mcar[0], p->d_mcar[1], s);
+                 p->d_orig[1], p->d_mcar[0], p->d_mcar[1], p->cap, p->d_tilecnt, s);
 }
 
 void enqueue_cluster(snicit_plan* p) {
@@ -536,7 +536,7 @@ void enqueue_cluster(snicit_plan* p) {
       launch_iota_i32(p->d_orig[0], p->cap, s);
       launch_row_any(p->d_in, p->ld, p->d_counts, p->cap, N, p->d_flags, s);
       launch_compact(p->d_flags, p->d_counts, p->d_counts + 1, p->d_map, nullptr, p->d_orig[0],
-                     p->d_orig[1], nullptr, nullptr, s);
+                     p->d_orig[1], nullptr, nullptr, p->cap, p->d_tilecnt, s);
       count = p->d_counts + 1;
     } else {
       count = p->d_counts + t;
@@ -559,7 +559,7 @@ void enqueue_cluster(snicit_plan* p) {
   // Non-empty delta rows: list of (column in Yt, centroid, original id).
   const int32_t* ident = nullptr;
   launch_compact(p->d_flags, count, p->d_nerow + t, p->d_mapD, map ? map : ident, p->d_orig[cur],
-                 p->d_orig[cur ^ 1], p->d_mcar[0], p->d_mcar[1], s);
+                 p->d_orig[cur ^ 1], p->d_mcar[0], p->d_mcar[1], p->cap, p->d_tilecnt, s);
   launch_make_delta(st.Yt, p->ld, p->d_mapD, p->d_C[0], p->kcap, p->d_mcar[1], p->d_nerow + t,
                     p->cap, N, st.Da, p->ld, s);
 }
@@ -616,7 +616,8 @@ void enqueue_post(snicit_plan* p) {
     }
     done_layer(p, l);
     launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
-                   p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1], s);
+                   p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1],
+                   p->cap, p->d_tilecnt, s);
     orig ^= 1;
     mc ^= 1;
   }
@@ -844,7 +845,8 @@ void plan_free(snicit_plan* p) {
     if (e) (void)hipEventDestroy(e);
   void* ptrs[] = {p->d_in,     p->d_y[0],    p->d_y[1],     p->d_flags,   p->d_map,
                   p->d_mapD,   p->d_orig[0], p->d_orig[1],  p->d_mcar[0], p->d_mcar[1],
-                  p->d_counts, p->d_nerow,   p->d_k,        p->d_F,       p->d_cent_slot,
+                  p->d_counts, p->d_tilecnt, p->d_nerow,   p->d_k,       p->d_F,
+                  p->d_cent_slot,
                   p->d_C[0],   p->d_C[1],    p->d_zc,       p->d_rec,     p->d_catc,
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
                   p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
@@ -1281,6 +1283,7 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         p->d_C[i] = dev_alloc<float>((size_t)N * p->kcap, acct);
       }
       p->d_counts = dev_alloc<int32_t>(L + 4, acct);
+      p->d_tilecnt = dev_alloc<int32_t>((size_t)compact_tiles(p->cap), acct);
       p->d_nerow = dev_alloc<int32_t>(L + 2, acct);
       p->d_k = dev_alloc<int32_t>(1, acct);
       p->d_F = dev_alloc<float>((size_t)32 * 32 * nseg, acct);

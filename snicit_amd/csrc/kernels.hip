@@ -1071,7 +1071,8 @@ __global__ __launch_bounds__(kWave) void lossy_prune_kernel(
   if (lane == 0) {
     int k = 0;
     for (int y = 0; y < P; ++y)
-      if ((y < 64 ? a0 >> y : a1 >> (y - 64)) & 1ull) cent_col[sg * stride + k++] = poolcol[sg * pool + y];
+      if ((y < 64 ? a0 >> y : a1 >> (y - 64)) & 1ull)
+        cent_col[sg * stride + k++] = poolcol[sg * pool + y];
     kseg[sg] = k;
     if (k > 0) atomicMax(kcols, sg * stride + k);
   }
@@ -1211,71 +1212,77 @@ __global__ __launch_bounds__(256) void lossy_delta_kernel(
 }
 
 // ---- stable compaction ----------------------------------------------------------------
-// One workgroup of 1024 threads walks the flags in rounds of 16 x 1024 columns: sixteen
-// coalesced loads per thread are in flight together, a ballot per 1024-column sub-round
-// gives per-wavefront counts, one wavefront scans the 256 counts, and every set flag gets
-// its output slot, so the new list keeps column order.  Flags are cleared as they are
-// consumed.  compose (optional) maps each flagged column to the value stored in map_out; it
-// must not alias map_out.  Replaces the per-layer host loops over managed memory
+// A workgroup of 1024 threads walks flags in rounds of SUB x 1024 columns: SUB coalesced
+// loads per thread are in flight together, a ballot per 1024-column sub-round gives
+// per-wavefront counts, one wavefront scans the 16 SUB counts, and every set flag gets its
+// output slot, so the new list keeps column order.  Flags are cleared as they are consumed.
+// compose (optional) maps each flagged column to the value stored in map_out; it must not
+// alias map_out.  Up to kCompactOneBlock columns, one workgroup walks them all (16 x 1024
+// per round).  Beyond that, compact_count_kernel counts the set flags of every 4096-column
+// tile, and workgroup b of compact_kernel places tile b's columns after the sum of the
+// earlier tiles' counts: the same order, with the tiles' rounds side by side and no waiting
+// between workgroups.  Replaces the per-layer host loops over managed memory
 // (src/SDGC/BF/BF.hpp:222-229, src/SDGC/SNICIT/SNICIT.hpp:556-615).
 constexpr int kCompactSub = 16;
-__global__ __launch_bounds__(1024) void compact_kernel(
-    int32_t* __restrict__ flags, const int32_t* __restrict__ count_in,
-    int32_t* __restrict__ count_out, int32_t* __restrict__ map_out,
+constexpr int kCompactTileSub = 4;
+constexpr int kCompactTile = 1024 * kCompactTileSub;
+constexpr int kCompactOneBlock = 1024 * kCompactSub;
+
+// Rounds over columns [begin, end), output slots from `running`; returns the slot after the
+// last one placed.  Every thread of the workgroup calls it with the same arguments.
+template <int SUB>
+__device__ __forceinline__ int compact_range(
+    int32_t* __restrict__ flags, int begin, int end, int running, int32_t* __restrict__ map_out,
     const int32_t* __restrict__ compose, const int32_t* __restrict__ c0_in,
     int32_t* __restrict__ c0_out, const int32_t* __restrict__ c1_in,
-    int32_t* __restrict__ c1_out) {
-  __shared__ int32_t s_cnt[kCompactSub * 16];
-  __shared__ int32_t s_off[kCompactSub * 16];
-  __shared__ int32_t s_tot;
-  const int n = *count_in;
+    int32_t* __restrict__ c1_out, int32_t* s_cnt, int32_t* s_off, int32_t* s_tot) {
+  constexpr int E = SUB * 16;   // (sub-round, wavefront) entries, in order e = 16 q + w
+  constexpr int EPL = E / 64;   // entries per lane of the scanning wavefront
   const int t = threadIdx.x, lane = lane_id(), wv = t / kWave;
-  int running = 0;
-  for (int base = 0; base < n; base += 1024 * kCompactSub) {
+  for (int base = begin; base < end; base += 1024 * SUB) {
     // every load of the round is issued before the scan: flags and the carried values
-    bool f[kCompactSub];
-    int32_t cm[kCompactSub], v0[kCompactSub], v1[kCompactSub];
+    bool f[SUB];
+    int32_t cm[SUB], v0[SUB], v1[SUB];
 #pragma unroll
-    for (int q = 0; q < kCompactSub; ++q) {
+    for (int q = 0; q < SUB; ++q) {
       const int i = base + q * 1024 + t;
-      const bool in = i < n;
+      const bool in = i < end;
       f[q] = in && flags[i] != 0;
       cm[q] = (in && compose) ? compose[i] : i;
       v0[q] = (in && c0_in) ? c0_in[i] : 0;
       v1[q] = (in && c1_in) ? c1_in[i] : 0;
     }
-    uint64_t m[kCompactSub];
+    uint64_t m[SUB];
 #pragma unroll
-    for (int q = 0; q < kCompactSub; ++q) {
+    for (int q = 0; q < SUB; ++q) {
       m[q] = __ballot(f[q]);
       if (lane == 0) s_cnt[q * 16 + wv] = __popcll(m[q]);
     }
     __syncthreads();
     if (wv == 0) {
-      // entries (sub-round q, wavefront w) in order e = 16 q + w; lane L scans e = 4L .. 4L+3
-      int v[4], tot4 = 0;
+      int v[EPL], tot = 0;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        v[e] = s_cnt[4 * lane + e];
-        tot4 += v[e];
+      for (int e = 0; e < EPL; ++e) {
+        v[e] = s_cnt[EPL * lane + e];
+        tot += v[e];
       }
-      int incl = tot4;
+      int incl = tot;
 #pragma unroll
       for (int d = 1; d < kWave; d <<= 1) {
         const int x = __shfl_up(incl, d, kWave);
         if (lane >= d) incl += x;
       }
-      int ex = incl - tot4;
+      int ex = incl - tot;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        s_off[4 * lane + e] = ex;
+      for (int e = 0; e < EPL; ++e) {
+        s_off[EPL * lane + e] = ex;
         ex += v[e];
       }
-      if (lane == kWave - 1) s_tot = incl;
+      if (lane == kWave - 1) *s_tot = incl;
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kCompactSub; ++q) {
+    for (int q = 0; q < SUB; ++q) {
       if (f[q]) {
         const int o = running + s_off[q * 16 + wv] + lane_prefix(m[q]);
         map_out[o] = cm[q];
@@ -1284,10 +1291,67 @@ __global__ __launch_bounds__(1024) void compact_kernel(
         flags[base + q * 1024 + t] = 0;
       }
     }
-    running += s_tot;
+    running += *s_tot;
     __syncthreads();
   }
-  if (t == 0) *count_out = running;
+  return running;
+}
+
+__global__ __launch_bounds__(1024) void compact_count_kernel(const int32_t* __restrict__ flags,
+                                                            const int32_t* __restrict__ count_in,
+                                                            int32_t* __restrict__ tile_cnt) {
+  __shared__ int32_t s_sum;
+  const int n = *count_in;
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  if (n <= kCompactOneBlock || base >= n) return;
+  const int t = threadIdx.x, lane = lane_id();
+  if (t == 0) s_sum = 0;
+  __syncthreads();
+  int c = 0;
+#pragma unroll
+  for (int q = 0; q < kCompactTileSub; ++q) {
+    const int64_t i = base + q * 1024 + t;
+    c += __popcll(__ballot(i < n && flags[i] != 0));
+  }
+  if (lane == 0 && c) atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (t == 0) tile_cnt[blockIdx.x] = s_sum;
+}
+
+__global__ __launch_bounds__(1024) void compact_kernel(
+    int32_t* __restrict__ flags, const int32_t* __restrict__ count_in,
+    int32_t* __restrict__ count_out, int32_t* __restrict__ map_out,
+    const int32_t* __restrict__ compose, const int32_t* __restrict__ c0_in,
+    int32_t* __restrict__ c0_out, const int32_t* __restrict__ c1_in,
+    int32_t* __restrict__ c1_out, const int32_t* __restrict__ tile_cnt) {
+  __shared__ int32_t s_cnt[kCompactSub * 16];
+  __shared__ int32_t s_off[kCompactSub * 16];
+  __shared__ int32_t s_tot;
+  __shared__ int32_t s_base;
+  const int n = *count_in;
+  const int t = threadIdx.x, lane = lane_id();
+  if (n <= kCompactOneBlock) {
+    if (blockIdx.x != 0) return;
+    const int placed = compact_range<kCompactSub>(flags, 0, n, 0, map_out, compose, c0_in, c0_out,
+                                                  c1_in, c1_out, s_cnt, s_off, &s_tot);
+    if (t == 0) *count_out = placed;
+    return;
+  }
+  const int64_t base = (int64_t)blockIdx.x * kCompactTile;
+  if (base >= n) return;
+  // output slots start after the earlier tiles' columns
+  if (t == 0) s_base = 0;
+  __syncthreads();
+  int part = 0;
+  for (int j = t; j < (int)blockIdx.x; j += 1024) part += tile_cnt[j];
+  for (int d = kWave / 2; d > 0; d >>= 1) part += __shfl_xor(part, d, kWave);
+  if (lane == 0 && part) atomicAdd(&s_base, part);
+  __syncthreads();
+  const int end = (int)min<int64_t>(base + kCompactTile, n);
+  const int placed = compact_range<kCompactTileSub>(flags, (int)base, end, s_base, map_out, compose,
+                                                    c0_in, c0_out, c1_in, c1_out, s_cnt, s_off,
+                                                    &s_tot);
+  if (end == n && t == 0) *count_out = placed;
 }
 
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int32_t v, int64_t n) {
@@ -1698,8 +1762,9 @@ struct DenseSteps {
 template <int EPI, int RB, int CB>
 __global__ __launch_bounds__(256) void dense_mfma_kernel(
     const float* __restrict__ wt, int32_t ldw, const float* __restrict__ vals, int32_t K,
-    int32_t rows, const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
-    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    int32_t rows, const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout, int64_t ldout,
+    const int32_t* __restrict__ count_ptr,
     int32_t narrow, int64_t lo, int64_t hi) {
   constexpr int U = DenseSteps<RB, CB>::U;
   const int64_t count = *count_ptr;
@@ -1969,8 +2034,8 @@ void launch_ell_chunk(const EllLaunch& a, int32_t rc, hipStream_t s) {
   // enough workgroups for the (row, column) chains when the count may be narrow
   const int64_t pairs = a.rows * std::min<int64_t>(a.max_cols, a.shape.narrow);
   const int64_t gy_pairs = std::min<int64_t>((pairs + 255) / 256, (int64_t)8 * device_cus());
-  const int64_t gy = std::min<int64_t>(
-      65535, std::max(grid_rows(a.shape.grid_target, groups, tiles), (gy_pairs + groups - 1) / groups));
+  const int64_t gy_tiles = grid_rows(a.shape.grid_target, groups, tiles);
+  const int64_t gy = std::min<int64_t>(65535, std::max(gy_tiles, (gy_pairs + groups - 1) / groups));
   dim3 grid((unsigned)groups, (unsigned)gy);
   const size_t lds = (size_t)rc * (TC + 4) * 4;
 #define SNICIT_ELL_CHUNK(E)                                                                    \
@@ -2175,9 +2240,21 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
 
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
-                    const int32_t* c1_in, int32_t* c1_out, hipStream_t s) {
-  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, s, flags, count_in, count_out,
-                     map_out, compose, c0_in, c0_out, c1_in, c1_out);
+                    const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
+                    hipStream_t s) {
+  SNICIT_REQUIRE(max_cols >= 0 && max_cols < (int64_t(1) << 31), "compaction size out of range");
+  // tiles of kCompactTile columns when the count may exceed one workgroup's walk
+  const unsigned tiles = max_cols > kCompactOneBlock
+                             ? (unsigned)((max_cols + kCompactTile - 1) / kCompactTile)
+                             : 1u;
+  if (tiles > 1) {
+    SNICIT_REQUIRE(tile_cnt != nullptr, "compaction of this size needs the tile-count scratch");
+    hipLaunchKernelGGL(compact_count_kernel, dim3(tiles), dim3(1024), 0, s, flags, count_in,
+                       tile_cnt);
+    check_launch();
+  }
+  hipLaunchKernelGGL(compact_kernel, dim3(tiles), dim3(1024), 0, s, flags, count_in, count_out,
+                     map_out, compose, c0_in, c0_out, c1_in, c1_out, tile_cnt);
   check_launch();
 }
 

@@ -132,9 +132,12 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
                         int64_t max_cols, int32_t N, float eps, bool positive_only, float* D,
                         int64_t ldD, int32_t* flags, hipStream_t s);
 
+// max_cols: upper bound of *count_in; tile_cnt: scratch of compact_tiles(max_cols) ints
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
-                    const int32_t* c1_in, int32_t* c1_out, hipStream_t s);
+                    const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
+                    hipStream_t s);
+inline int64_t compact_tiles(int64_t max_cols) { return (max_cols + 4095) / 4096; }
 
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s);

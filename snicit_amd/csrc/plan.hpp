@@ -38,6 +38,7 @@ struct snicit_plan {
   int32_t* d_orig[2] = {nullptr, nullptr};
   int32_t* d_mcar[2] = {nullptr, nullptr};
   int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
+  int32_t* d_tilecnt = nullptr; // [compact_tiles(cap)] per-tile counts of the compaction
   int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
   int32_t* d_k = nullptr;  // centroid columns in use (k with one segment)
   float* d_F = nullptr;

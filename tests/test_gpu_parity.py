@@ -128,6 +128,28 @@ def test_snicit_sdgc_matches_oracle(ctx, t):
     np.testing.assert_array_equal(plan.activations(), r["yL"])
 
 
+@pytest.mark.parametrize("mode,t", [("BF", 0), ("SNICIT", 4)])
+def test_compaction_over_many_tiles(ctx, mode, t):
+    """50000 columns (13 tiles of 4096): the live-column compaction runs as per-tile counts
+    and per-tile placement while more than 16384 columns live, and in one workgroup below;
+    the live counts per layer, activations and categories equal the oracle's."""
+    N, L, B = 1024, 12, 50000
+    cols, vals = nets.sdgc(N, L, seed=7)
+    y0 = nets.images(N, B, seed=11)
+    net = sa.Net.from_ell(ctx, cols, vals, -0.3)
+    plan, st = _run(ctx, net, sa.default_cfg(mode, threshold=t), y0)
+    if mode == "BF":
+        yL, cat, live = orc.plain_f32(cols, vals, -0.3, 32.0, y0)
+        assert live[0] > 16384 > live[-1]
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LIVE), live)
+    else:
+        r = orc.snicit_exact(cols, vals, -0.3, 32.0, y0, t)
+        yL, cat = r["yL"], r["cat"]
+        assert r["live"][0] > 16384 and st.live_at_threshold == r["live_at_t"]
+    np.testing.assert_array_equal(plan.activations(), yL)
+    np.testing.assert_array_equal(plan.categories(), cat)
+
+
 @pytest.mark.parametrize("t", [1, 2, 4])
 def test_snicit_random_net_delta_rows(ctx, t):
     """Unconverged nets: many non-empty delta rows through every post layer."""
