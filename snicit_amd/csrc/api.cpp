@@ -615,9 +615,10 @@ void enqueue_post(snicit_plan* p) {
       prof_mark(p, l, 1);
     }
     done_layer(p, l);
+    // delta rows: a short list (one launch, one workgroup)
     launch_compact(p->d_flags, p->d_nerow + l, p->d_nerow + l + 1, p->d_mapD, nullptr,
                    p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1],
-                   p->cap, p->d_tilecnt, s);
+                   p->cap, nullptr, s);
     orig ^= 1;
     mc ^= 1;
   }

@@ -1330,7 +1330,7 @@ __global__ __launch_bounds__(1024) void compact_kernel(
   __shared__ int32_t s_base;
   const int n = *count_in;
   const int t = threadIdx.x, lane = lane_id();
-  if (n <= kCompactOneBlock) {
+  if (n <= kCompactOneBlock || gridDim.x == 1) {  // one workgroup walks the whole list
     if (blockIdx.x != 0) return;
     const int placed = compact_range<kCompactSub>(flags, 0, n, 0, map_out, compose, c0_in, c0_out,
                                                   c1_in, c1_out, s_cnt, s_off, &s_tot);
@@ -2243,12 +2243,12 @@ void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out,
                     const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
                     hipStream_t s) {
   SNICIT_REQUIRE(max_cols >= 0 && max_cols < (int64_t(1) << 31), "compaction size out of range");
-  // tiles of kCompactTile columns when the count may exceed one workgroup's walk
-  const unsigned tiles = max_cols > kCompactOneBlock
+  // tiles of kCompactTile columns when the count may exceed one workgroup's walk; without
+  // the tile-count scratch (lists that stay short) one workgroup walks any count
+  const unsigned tiles = max_cols > kCompactOneBlock && tile_cnt
                              ? (unsigned)((max_cols + kCompactTile - 1) / kCompactTile)
                              : 1u;
   if (tiles > 1) {
-    SNICIT_REQUIRE(tile_cnt != nullptr, "compaction of this size needs the tile-count scratch");
     hipLaunchKernelGGL(compact_count_kernel, dim3(tiles), dim3(1024), 0, s, flags, count_in,
                        tile_cnt);
     check_launch();

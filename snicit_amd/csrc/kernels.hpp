@@ -132,7 +132,8 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
                         int64_t max_cols, int32_t N, float eps, bool positive_only, float* D,
                         int64_t ldD, int32_t* flags, hipStream_t s);
 
-// max_cols: upper bound of *count_in; tile_cnt: scratch of compact_tiles(max_cols) ints
+// max_cols: upper bound of *count_in; tile_cnt: scratch of compact_tiles(max_cols) ints, or
+// null for lists expected to stay short (one workgroup then walks any count, one launch)
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
                     const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
