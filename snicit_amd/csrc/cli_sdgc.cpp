@@ -6,7 +6,9 @@
 // (src/SDGC/SNICIT/SNICIT.hpp:659-664, src/SDGC/BF/BF.hpp:219, src/SDGC/SNIG/SNIG.hpp:304,
 // src/SDGC/XY/XY.hpp:500-501) and the identity-level golden check (utility.hpp:243-266).
 // Extra flags: --synthetic (seeded SDGC-like net and blob images instead of the files;
-// the golden categories then come from a plain BF run), --seed S, --inputs K, --dataset DIR.
+// the golden categories then come from a plain BF run), --seed S, --inputs K, --dataset DIR,
+// --hash-reorder (renumber the neurons of the weight and input files by the reference's
+// bucket hash before the run, as its SNICIT mode does: src/SDGC/SNICIT/SNICIT.hpp:99-139).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -65,6 +67,43 @@ void read_inputs(const std::string& path, int N, int64_t num, std::vector<float>
     if (b >= 1 && b <= num && n >= 1 && n <= N) y[(size_t)(b - 1) * N + (n - 1)] = v;
 }
 
+// The reference's neuron renumbering (src/SDGC/reorder/hash.h:14-21, buckets per N at
+// src/SDGC/SNICIT/SNICIT.hpp:99-104): neuron v = q * buckets + r moves to r * (N / buckets) + q,
+// so neurons that share a residue modulo the bucket count become neighbours.  It is a
+// permutation of [0, N) when buckets divides N, applied alike to the input and output index of
+// every weight and to the input features.  Categories do not depend on neuron order, so the run
+// must classify as without it (the golden check is on categories); the final activations are the
+// same values renumbered, up to the FP32 rounding of the row sums' changed input order.
+std::vector<int32_t> hash_permutation(int N) {
+  const std::map<int, int> buckets = {{65536, 4096}, {16384, 1024}, {4096, 256}, {1024, 64}};
+  const int nb = buckets.at(N), width = N / nb;
+  std::vector<int32_t> to(N);
+  for (int v = 0; v < N; ++v) to[v] = v / nb + (v % nb) * width;
+  return to;
+}
+
+// Renumber an ELL net ([L][N][K], row o holds output neuron o) and the row-major inputs.
+void apply_permutation(const std::vector<int32_t>& to, int L, int K, std::vector<int32_t>& cols,
+                       std::vector<float>& vals, std::vector<float>& y, int64_t num) {
+  const size_t N = to.size(), row = (size_t)K;
+  std::vector<int32_t> c2(cols.size());
+  std::vector<float> v2(vals.size());
+  for (int l = 0; l < L; ++l)
+    for (size_t o = 0; o < N; ++o) {
+      const size_t src = ((size_t)l * N + o) * row, dst = ((size_t)l * N + to[o]) * row;
+      for (size_t r = 0; r < row; ++r) {
+        c2[dst + r] = to[cols[src + r]];
+        v2[dst + r] = vals[src + r];
+      }
+    }
+  cols.swap(c2);
+  vals.swap(v2);
+  std::vector<float> y2(y.size());
+  for (int64_t b = 0; b < num; ++b)
+    for (size_t n = 0; n < N; ++n) y2[(size_t)b * N + to[n]] = y[(size_t)b * N + n];
+  y.swap(y2);
+}
+
 // Golden categories: one 1-based sample id per line (read_golden, utility.hpp:243-256).
 std::vector<int32_t> read_golden(const std::string& path, int64_t num) {
   std::ifstream f(path);
@@ -82,7 +121,7 @@ int main(int argc, char** argv) {
   std::string mode = "SNICIT", dataset = "../dataset/SDGC";
   int N = 1024, L = 120, t = 30;
   int64_t batch = 60000, inputs = 60000;
-  bool synthetic = false;
+  bool synthetic = false, hash_reorder = false;
   uint64_t seed = 2023;
   for (int a = 1; a < argc; ++a) {
     std::string s = argv[a];
@@ -99,9 +138,10 @@ int main(int argc, char** argv) {
     else if (s == "--synthetic") synthetic = true;
     else if (s == "--seed") seed = std::strtoull(next().c_str(), nullptr, 10);
     else if (s == "--dataset") dataset = next();
+    else if (s == "--hash-reorder") hash_reorder = true;
     else if (s == "-h" || s == "--help") {
       std::cout << "usage: snicit_sdgc [-m SNICIT|BF|SNIG|XY] [-n N] [-l L] [-t T] [-b B] "
-                   "[--synthetic] [--seed S] [--inputs K] [--dataset DIR]\n";
+                   "[--synthetic] [--seed S] [--inputs K] [--dataset DIR] [--hash-reorder]\n";
       return 0;
     } else die("unknown option " + s);
   }
@@ -121,24 +161,34 @@ int main(int argc, char** argv) {
   std::vector<float> y0;
   std::vector<int32_t> golden;
   const int K = 32;
-  if (synthetic) {
+  if (synthetic && !hash_reorder) {
     check(snicit_net_generate_sdgc(ctx, N, L, seed, 0.0625f, bias, 32.f, &net), "net");
     y0.resize((size_t)inputs * N);
     check(snicit_gen_images(N, seed + 1, 0, inputs, y0.data()), "images");
   } else {
-    const std::string wdir = dataset + "/tsv_weights/neuron" + std::to_string(N);
     std::vector<int32_t> cols((size_t)L * N * K);
-    std::vector<float> vals((size_t)L * N * K);
-    for (int l = 0; l < L; ++l)
-      read_layer(wdir + "/n" + std::to_string(N) + "-l" + std::to_string(l + 1) + ".tsv", N, cols,
-                 vals, l, K);
+    std::vector<float> vals((size_t)L * N * K, 0.0625f);
+    if (synthetic) {
+      check(snicit_gen_sdgc_ell(N, L, seed, cols.data()), "net");
+      y0.resize((size_t)inputs * N);
+      check(snicit_gen_images(N, seed + 1, 0, inputs, y0.data()), "images");
+    } else {
+      const std::string wdir = dataset + "/tsv_weights/neuron" + std::to_string(N);
+      for (int l = 0; l < L; ++l)
+        read_layer(wdir + "/n" + std::to_string(N) + "-l" + std::to_string(l + 1) + ".tsv", N,
+                   cols, vals, l, K);
+      read_inputs(dataset + "/MNIST/sparse-images-" + std::to_string(N) + ".tsv", N, inputs, y0);
+      golden = read_golden(dataset + "/MNIST/neuron" + std::to_string(N) + "-l" +
+                               std::to_string(L) + "-categories.tsv",
+                           inputs);
+    }
+    if (hash_reorder) {
+      apply_permutation(hash_permutation(N), L, K, cols, vals, y0, inputs);
+      std::cout << "Neurons renumbered by the bucket hash (" << N << " neurons)" << std::endl;
+    }
     check(snicit_net_create_ell(ctx, N, L, K, cols.data(), vals.data(), SNICIT_BIAS_SCALAR, &bias,
                                 32.f, &net),
           "net");
-    read_inputs(dataset + "/MNIST/sparse-images-" + std::to_string(N) + ".tsv", N, inputs, y0);
-    golden = read_golden(dataset + "/MNIST/neuron" + std::to_string(N) + "-l" + std::to_string(L) +
-                             "-categories.tsv",
-                         inputs);
   }
   snicit_run_cfg cfg;
   snicit_run_cfg_default(&cfg);

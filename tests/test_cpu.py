@@ -140,6 +140,32 @@ def test_permutation_net():
     np.testing.assert_array_equal(yL, expect)
 
 
+def test_neuron_renumbering_keeps_categories():
+    """The bucket-hash renumbering the SDGC command line applies with --hash-reorder
+    (src/SDGC/reorder/hash.h:14-21: v = q * buckets + r -> r * (N / buckets) + q, 64 buckets
+    at N = 1024) is a permutation; renumbering every weight index and input feature alike
+    leaves the categories unchanged and renumbers the final activations.  Rows sum in
+    ascending input-neuron order, which the renumbering changes, so the activations agree to
+    FP32 rounding (1e-4 relative), not bit for bit."""
+    N, L, B = 1024, 8, 300
+    v = np.arange(N)
+    to = v // 64 + (v % 64) * (N // 64)
+    assert np.array_equal(np.sort(to), v)
+    assert to[1] == 16 and to[64] == 1 and to[N - 1] == N - 1
+    cols = sa.gen_sdgc_cols(N, L, 5)
+    vals = np.full(cols.shape, 0.0625, np.float32)
+    y0 = sa.gen_images(N, 6, 0, B)
+    pc = np.empty_like(cols)
+    pc[:, to] = to[cols]
+    py = np.empty_like(y0)
+    py[:, to] = y0
+    yL, cat, _ = orc.plain_f32(cols, vals, sa.SDGC_BIAS[N], 32.0, y0)
+    pyL, pcat, _ = orc.plain_f32(pc, vals, sa.SDGC_BIAS[N], 32.0, py)
+    assert 0 < cat.sum() < B
+    np.testing.assert_array_equal(pcat, cat)
+    np.testing.assert_allclose(pyL[:, to], yL, rtol=1e-4, atol=1e-4)
+
+
 def test_dense_brute_force():
     """Small net against a dense float64 matmul forward (clip and bias included)."""
     N, L, B = 128, 6, 40

@@ -62,6 +62,26 @@ def test_sdgc_cli_on_graph_challenge_files(tmp_path, mode):
     assert re.search(rf"{mode} info: runtime [0-9.e+-]+ ?ms", r.stdout)
 
 
+@pytest.mark.parametrize("mode", ["SNICIT", "BF"])
+def test_sdgc_cli_hash_reorder_keeps_categories(tmp_path, mode):
+    """--hash-reorder renumbers every neuron of the weight and input files by the reference's
+    bucket hash (src/SDGC/reorder/hash.h:14-21) before the run; the golden categories file is
+    written for the original numbering, so passing shows the renumbered run classifies alike."""
+    N, L, B = 1024, 24, 2000
+    data = tmp_path / "dataset" / "SDGC"
+    _write_sdgc_dataset(str(data), N, L, 4, B)
+    base = [os.path.join(PKG, "snicit_sdgc"), "-m", mode, "-n", str(N), "-l", str(L), "-t", "6",
+            "-b", "1000", "--inputs", str(B), "--dataset", str(data)]
+    r = _run(base + ["--hash-reorder"], cwd=str(tmp_path))
+    assert r.returncode == 0
+    assert "renumbered by the bucket hash" in r.stdout
+    assert "Number of different categories: 0" in r.stdout and "CHALLENGE PASSED" in r.stdout
+    r = _run([os.path.join(PKG, "snicit_sdgc"), "--synthetic", "--hash-reorder", "-m", mode,
+              "-n", "4096", "-l", "30", "-t", "8", "-b", "2000", "--inputs", "4000"],
+             cwd=str(tmp_path))
+    assert r.returncode == 0 and "CHALLENGE PASSED" in r.stdout
+
+
 def test_sdgc_cli_synthetic(tmp_path):
     r = _run([os.path.join(PKG, "snicit_sdgc"), "--synthetic", "-n", "4096", "-l", "40", "-t", "10",
               "-b", "3000", "--inputs", "6000"], cwd=str(tmp_path))
