@@ -52,7 +52,7 @@ def spmm_layer_bytes(N, P, rows):
 def measured_traffic(wl):
     """HBM bytes per pre-threshold launch from the committed counter summary of this
     workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by a known-size copy), or None."""
-    path = os.path.join(ROOT, "profiles", "panel_spmm_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "panel_plain_mfma_c2_traffic.json")
     try:
         rec = json.load(open(path))
     except (OSError, ValueError):
@@ -506,7 +506,7 @@ def main():
     b_pre = sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre)
     t_pre = sum(ms_l[l] for l in pre) / 1000.0
     n_pre = max(1, len(pre))
-    roof = {"kernel": "panel_spmm_kernel<plain> (layers < t)", "bound": "hbm",
+    roof = {"kernel": "panel_plain_mfma_kernel (layers < t)", "bound": "hbm",
             "achieved": round(b_pre / t_pre / 1e9, 1) if t_pre > 0 else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(b_pre / t_pre / 1e9 / HBM_PEAK_GBS, 4) if t_pre > 0 else None,

@@ -21,22 +21,22 @@ using namespace snicit;
 
 namespace {
 
-// Upload one layer's host panels into the net's concatenated arrays.
-// Layers whose panels average >= kMfmaMinAvgRows rows get slot-major weights (matrix-core
-// operands); the others keep row-major weights for the vector-ALU kernel.
+// Upload one layer's host panels into the net's concatenated arrays.  Row-major weights
+// serve every panel kernel; SNICIT_KERNEL=mfma stores them slot-major for panel_mfma_kernel.
 int32_t max_rows(const HostLayerPanels& hp) {
   int32_t m = 0;
   for (size_t p = 0; p + 1 < hp.pbeg.size(); ++p) m = std::max(m, hp.pbeg[p + 1] - hp.pbeg[p]);
   return m;
 }
 
-// SNICIT_KERNEL=valu|mfma|auto selects the layer kernel family when a net is built.
-// auto = vector ALU: on the 32-nnz nets it runs at ~80% of device copy bandwidth with 8
-// wavefronts per SIMD, while the matrix-core form needs ~170 VGPRs (2 wavefronts per SIMD)
-// and measured slower; mfma stays available (bit-identical) for nets where it wins.
+// SNICIT_KERNEL=auto|valu|mfma selects the panel kernel family when a net is built (all
+// three give the same bits).  auto (0): plain-epilogue layers on the matrix cores with the
+// weights staged in LDS (panel_plain_mfma_kernel), centroid and delta epilogues on the
+// vector ALU.  valu (1): the vector-ALU kernel for every epilogue.  mfma (2): slot-major
+// weights held in registers (panel_mfma_kernel, ~170 VGPRs, measured slower).
 int kernel_choice_from_env() {
   const char* e = std::getenv("SNICIT_KERNEL");
-  if (!e || !*e || std::strcmp(e, "auto") == 0) return 1;
+  if (!e || !*e || std::strcmp(e, "auto") == 0) return 0;
   if (std::strcmp(e, "valu") == 0) return 1;
   if (std::strcmp(e, "mfma") == 0) return 2;
   throw Error(SNICIT_E_INVALID, std::string("SNICIT_KERNEL must be auto, valu or mfma, got ") + e);
@@ -117,9 +117,7 @@ struct NetBuilder {
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
     outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());
-    const bool mfma = kernel_choice != 1 && P > 0 &&
-                      (kernel_choice == 2 || (int64_t)net->N >= (int64_t)kMfmaMinAvgRows * P) &&
-                      max_rows(hp) <= 32;
+    const bool mfma = kernel_choice == 2 && P > 0 && max_rows(hp) <= 32;
     net->mfma.push_back(mfma ? 1 : 0);
     net->w_off.push_back(w_floats);
     const int64_t nf = mfma ? P * kSlots * 32 : (int64_t)net->N * kSlots;
@@ -141,6 +139,7 @@ struct NetBuilder {
 
 void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
   hipStream_t s = net->ctx->stream;
+  net->plain_mfma = nb.kernel_choice == 0;
   const int64_t Ptot = net->panel_off.back();
   net->d_pcols = dev_alloc<int32_t>((size_t)Ptot * kSlots, &net->device_bytes);
   net->d_pbeg = dev_alloc<int32_t>((size_t)Ptot + net->L, &net->device_bytes);
@@ -1719,7 +1718,10 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
 namespace {
 std::string tune_key(const snicit_plan* p) {
   const Net& n = *p->net;
-  const char* kind = n.is_ell(0) ? "ell" : (n.mfma[0] ? "panel-mfma" : "panel-valu");
+  const char* kind = n.is_ell(0)       ? "ell"
+                     : n.mfma[0]       ? "panel-mfma"
+                     : n.plain_mfma    ? "panel-mx"
+                                       : "panel-valu";
   const int32_t K = n.is_ell(0) ? n.ell_K[0] : kSlots;
   const int64_t rows = n.is_ell(0) ? 1 : n.N / std::max<int64_t>(1, n.panels(0));
   int64_t bucket = 1;

@@ -39,8 +39,6 @@ constexpr int kSlots = 32;
 // Maximum output rows per panel (one 32-row matrix-core tile); larger sharing groups are
 // split.
 constexpr int kMaxPanelRows = 32;
-// Layers whose panels average at least this many rows run on the matrix cores.
-constexpr int kMfmaMinAvgRows = 16;
 // Centroid capacity of the exact variant (pool <= 32 rows => k <= 32).
 constexpr int kMaxCentroidsExact = 32;
 // Centroid capacity of the lossy variant (pool of 128 strided rows).
@@ -107,6 +105,8 @@ struct Net {
   float* d_w = nullptr;            // per layer: row-major [N][kSlots] or slot-major [P][kSlots][32]
   std::vector<int64_t> w_off;      // [L] first float of each layer in d_w
   std::vector<uint8_t> mfma;       // [L] layer runs on the matrix cores (slot-major weights)
+  // plain-epilogue launches of row-major panel layers run panel_plain_mfma_kernel
+  bool plain_mfma = true;
   float* d_bias = nullptr;         // [L][N] when bias_kind == PER_NEURON
   // wide-ELL layers (rows with more than 32 nonzeros): rows padded to ell_K (multiple of 16)
   std::vector<int32_t> ell_K;      // [L] padded row length, 0 for panel layers

@@ -58,6 +58,9 @@ __device__ __forceinline__ float epilogue(float acc, float b, float clip, bool v
   return y;
 }
 
+// Accumulator of one v_mfma_f32_32x32x1_2b_f32 (two 32 x 32 blocks, 32 values per lane).
+typedef float floatx32 __attribute__((ext_vector_type(32)));
+
 // ---- panel SpMM ---------------------------------------------------------------------
 // Block (panel p, column tile): each wavefront gathers the panel's 32 inputs of its 64
 // columns into registers, then produces every output row of the panel with a fixed-order
@@ -72,6 +75,45 @@ __device__ __forceinline__ float epilogue(float acc, float b, float clip, bool v
 // With at most `narrow` columns the wavefront-per-64-columns layout would leave one
 // wavefront walking the panel's rows behind dependent scalar loads, so those launches run
 // one thread per (row, column) pair instead, with the same chain and epilogue.
+// (row, column) pairs of one panel over every workgroup of the panel: each pair's 32 inputs
+// are loaded together, then the chain runs.
+template <int EPI>
+__device__ __forceinline__ void panel_pairs(
+    int beg, int end, const int32_t* __restrict__ pc, const int32_t* __restrict__ outs,
+    const float* __restrict__ w, const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ yin, int64_t ldin, float* __restrict__ yout, int64_t ldout,
+    int64_t count, const int32_t* __restrict__ map, int32_t* __restrict__ flags,
+    float* __restrict__ zc, int32_t kcap, const int32_t* __restrict__ mcent, float eps_post) {
+  const int G = end - beg;
+  const int64_t pairs = (int64_t)G * count;
+  for (int64_t q = (int64_t)blockIdx.y * kBlock + threadIdx.x; q < pairs;
+       q += (int64_t)gridDim.y * kBlock) {
+    const int j = beg + (int)(q % G);
+    const int64_t pos = q / G;
+    const int64_t src = map ? (int64_t)map[pos] : pos;
+    const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
+    float x[kSlots];
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) x[r] = yin[(int64_t)pc[r] * ldin + src];
+    float acc = 0.f;
+#pragma unroll
+    for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+      const float4 wv = w4[r4];
+      acc = __builtin_fmaf(wv.x, x[4 * r4 + 0], acc);
+      acc = __builtin_fmaf(wv.y, x[4 * r4 + 1], acc);
+      acc = __builtin_fmaf(wv.z, x[4 * r4 + 2], acc);
+      acc = __builtin_fmaf(wv.w, x[4 * r4 + 3], acc);
+    }
+    const int o = outs[j];
+    const float b = bias_vec ? bias_vec[o] : bias_s;
+    const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
+    bool flag = false;
+    const float y = epilogue<EPI>(acc, b, clip, true, o, pos, mi, zc, kcap, eps_post, flag);
+    yout[(int64_t)o * ldout + pos] = y;
+    if (flags && flag) flags[pos] = 1;
+  }
+}
+
 template <int EPI>
 __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
@@ -85,36 +127,8 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   const int beg = pbeg[p], end = pbeg[p + 1];
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
   if (count <= narrow) {
-    // (row, column) pairs over every workgroup of the panel; each pair's 32 inputs are
-    // loaded together, then the chain runs
-    const int G = end - beg;
-    const int64_t pairs = (int64_t)G * count;
-    for (int64_t q = (int64_t)blockIdx.y * kBlock + threadIdx.x; q < pairs;
-         q += (int64_t)gridDim.y * kBlock) {
-      const int j = beg + (int)(q % G);
-      const int64_t pos = q / G;
-      const int64_t src = map ? (int64_t)map[pos] : pos;
-      const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)j * kSlots);
-      float x[kSlots];
-#pragma unroll
-      for (int r = 0; r < kSlots; ++r) x[r] = yin[(int64_t)pc[r] * ldin + src];
-      float acc = 0.f;
-#pragma unroll
-      for (int r4 = 0; r4 < kSlots / 4; ++r4) {
-        const float4 wv = w4[r4];
-        acc = __builtin_fmaf(wv.x, x[4 * r4 + 0], acc);
-        acc = __builtin_fmaf(wv.y, x[4 * r4 + 1], acc);
-        acc = __builtin_fmaf(wv.z, x[4 * r4 + 2], acc);
-        acc = __builtin_fmaf(wv.w, x[4 * r4 + 3], acc);
-      }
-      const int o = outs[j];
-      const float b = bias_vec ? bias_vec[o] : bias_s;
-      const int32_t mi = (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) ? mcent[pos] : 0;
-      bool flag = false;
-      const float y = epilogue<EPI>(acc, b, clip, true, o, pos, mi, zc, kcap, eps_post, flag);
-      yout[(int64_t)o * ldout + pos] = y;
-      if (flags && flag) flags[pos] = 1;
-    }
+    panel_pairs<EPI>(beg, end, pc, outs, w, bias_vec, bias_s, clip, yin, ldin, yout, ldout,
+                     count, map, flags, zc, kcap, mcent, eps_post);
     return;
   }
   const int lane = lane_id();
@@ -153,6 +167,99 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
       if (valid) yout[(int64_t)o * ldout + pos] = y;
     }
     if (flags && valid && flag) flags[pos] = 1;
+  }
+}
+
+// ---- plain panel layer on the matrix cores ----------------------------------------------
+// Same contract as panel_spmm_kernel<EPI_PLAIN> for row-major panel weights.  The workgroup
+// stages its panel's weights in LDS (rows past the panel's row count as zeros), then each
+// wavefront owns 64 columns per tile: it gathers the 32 inputs of its columns into registers
+// and runs slot r as one v_mfma_f32_32x32x1_2b_f32 step, A = the 32 row weights of slot r
+// (both 32-lane halves), B = the 64 columns' input of slot r (block 0 = columns 0..31,
+// block 1 = 32..63).  With K = 1 every element of a step is one fused multiply-add, so the
+// 32 steps in slot order are exactly the vector kernel's chain
+// acc = fma(w[31], x[31], ... fma(w[0], x[0], 0)) (src/SDGC/BF/kernel.hpp:57-88).  The
+// matrix cores take the 32 multiply-adds per output off the vector ALU, which otherwise
+// holds the layer below the memory rate; 96 registers give 5 wavefronts per SIMD.
+// D element (block b, row i, column j) sits in lane 32 (i / 4 % 2) + j, register
+// 16 b + 4 (i / 8) + i % 4.  Up to `narrow` columns the (row, column) pair path runs.
+__global__ __launch_bounds__(kBlock, 5) void panel_plain_mfma_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, int32_t narrow) {
+  __shared__ float w_s[kMaxPanelRows * (kSlots + 1)];  // row i, slot r at i * 33 + r
+  __shared__ int32_t o_s[kMaxPanelRows];
+  __shared__ float b_s[kMaxPanelRows];
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int beg = pbeg[p], end = pbeg[p + 1];
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  if (count <= narrow) {
+    panel_pairs<EPI_PLAIN>(beg, end, pc, outs, w, bias_vec, bias_s, clip, yin, ldin, yout,
+                           ldout, count, map, flags, nullptr, 0, nullptr, 0.f);
+    return;
+  }
+  const int G = end - beg;
+  for (int q = threadIdx.x; q < kMaxPanelRows * kSlots; q += kBlock) {
+    const int i = q / kSlots, r = q % kSlots;
+    w_s[i * (kSlots + 1) + r] = i < G ? w[(size_t)(beg + i) * kSlots + r] : 0.f;
+  }
+  if (threadIdx.x < kMaxPanelRows) {
+    const int i = threadIdx.x;
+    const int o = i < G ? outs[beg + i] : 0;
+    o_s[i] = o;
+    b_s[i] = bias_vec ? bias_vec[o] : bias_s;
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int half = lane >> 5, jl = lane & 31;
+  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
+       base += (int64_t)gridDim.y * kBlock) {
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    // columns past the count read column 0 (count > 0 here) and are zeroed
+    const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
+    float x[kSlots];
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) x[r] = yin[(int64_t)pc[r] * ldin + src];
+    bool nz = false;
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) {
+      x[r] = valid ? x[r] : 0.f;
+      nz |= x[r] != 0.f;
+    }
+    floatx32 acc;
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.f;
+    // all 64 x 32 inputs zero: acc = +0 exactly, what the steps would give
+    if (__ballot(nz) != 0ull) {
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r)
+        acc = __builtin_amdgcn_mfma_f32_32x32x1f32(w_s[jl * (kSlots + 1) + r], x[r], acc, 0, 0,
+                                                   0);
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int64_t pb = base + 32 * b + jl;
+      const bool vb = pb < count;
+      bool flag = false;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int i = 8 * (q >> 2) + 4 * half + (q & 3);
+        if (i < G) {
+          const float y = act_clip(acc[16 * b + q] + b_s[i], clip);
+          flag |= y > 0.f;
+          if (vb) yout[(int64_t)o_s[i] * ldout + pb] = y;
+        }
+      }
+      // the column's rows are split over lanes j and j + 32
+      const uint64_t m = __ballot(flag);
+      const bool any = ((m >> jl) & 1ull) | ((m >> (jl + 32)) & 1ull);
+      if (flags && half == 0 && vb && any) flags[pb] = 1;
+    }
   }
 }
 
@@ -343,7 +450,6 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
 // acc = fma(w[31], x[31], ... fma(w[0], x[0], 0)) of the vector kernel and the oracle.
 // D element (block b, row i, column j) sits in lane 32*(i/4 % 2) + j, register
 // 16 b + 4 (i / 8) + i % 4: each store writes two 128-byte runs (rows i, i+4 of 32 columns).
-typedef float floatx32 __attribute__((ext_vector_type(32)));
 
 template <int EPI>
 __global__ __launch_bounds__(kBlock, 2) void panel_mfma_kernel(
@@ -1906,6 +2012,13 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const float* bv = net.bias_vec(l);
   const float bs = net.bias_scalar(l);
   const bool mfma = net.mfma[l] != 0;
+  if (!mfma && a.epilogue == EPI_PLAIN && net.plain_mfma) {
+    hipLaunchKernelGGL(panel_plain_mfma_kernel, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
+                       bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags,
+                       a.shape.narrow);
+    check_launch();
+    return;
+  }
 #define SNICIT_SPMM(E)                                                                         \
   do {                                                                                         \
     if (mfma)                                                                                  \

@@ -263,15 +263,16 @@ def test_infer_rounds(ctx):
         np.testing.assert_array_equal(cls(net).infer(y0, batch=300), cat)
 
 
-@pytest.mark.parametrize("share", [16, 32])
+@pytest.mark.parametrize("share", [1, 13, 16, 32])
 def test_matrix_core_and_vector_kernels_agree(ctx, share, monkeypatch):
-    """The matrix-core layer kernel (K=1 fused steps) and the vector-ALU kernel give the same
-    bits, and both equal the oracle."""
+    """The matrix-core layer kernels (K=1 fused steps: the default plain-layer kernel with
+    weights in LDS, and the slot-major one) and the vector-ALU kernel give the same bits, and
+    all equal the oracle."""
     N, L, B = 1024, 8, 700
     cols, vals = nets.random_ell(N, L, K=32, seed=11, share=share)
     y0 = nets.random_inputs(N, B, seed=4, density=0.25)
     outs = {}
-    for kind in ("valu", "mfma"):
+    for kind in ("auto", "valu", "mfma"):
         monkeypatch.setenv("SNICIT_KERNEL", kind)
         net = sa.Net.from_ell(ctx, cols, vals, -0.01)
         for mode, t in (("BF", 0), ("SNICIT", 3)):
@@ -279,6 +280,6 @@ def test_matrix_core_and_vector_kernels_agree(ctx, share, monkeypatch):
             outs[(kind, mode)] = plan.activations()
     yL, _, _ = orc.plain_f32(cols, vals, -0.01, 32.0, y0)
     r = orc.snicit_exact(cols, vals, -0.01, 32.0, y0, 3)
-    for kind in ("valu", "mfma"):
+    for kind in ("auto", "valu", "mfma"):
         np.testing.assert_array_equal(outs[(kind, "BF")], yL)
         np.testing.assert_array_equal(outs[(kind, "SNICIT")], r["yL"])
