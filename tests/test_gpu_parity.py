@@ -263,6 +263,30 @@ def test_infer_rounds(ctx):
         np.testing.assert_array_equal(cls(net).infer(y0, batch=300), cat)
 
 
+@pytest.mark.parametrize("K,share", [(8, 1), (12, 3), (5, 2)])
+def test_merged_short_row_panels(ctx, K, share, monkeypatch):
+    """Rows with fewer than 32 inputs share panels when their input sets fit 32 slots
+    (inspector-style grouping): fewer panels, and BF and exact SNICIT equal the oracle bit for
+    bit, as does the layout without merging.  One row lists an input twice; its group keeps
+    its own panel."""
+    N, L, B = 1024, 8, 700
+    cols, vals = nets.random_ell(N, L, K=K, seed=13, share=share)
+    cols[2, 5, 1] = cols[2, 5, 0]
+    y0 = nets.random_inputs(N, B, seed=4, density=0.3)
+    yL, _, _ = orc.plain_f32(cols, vals, -0.01, 32.0, y0)
+    r = orc.snicit_exact(cols, vals, -0.01, 32.0, y0, 3)
+    panels = {}
+    for merge in ("1", "0"):
+        print(f"K={K} share={share} merge={merge}", flush=True)
+        monkeypatch.setenv("SNICIT_PANEL_MERGE", merge)
+        net = sa.Net.from_ell(ctx, cols, vals, -0.01)
+        panels[merge] = net.panels
+        for mode, t in (("BF", 0), ("SNICIT", 3)):
+            plan, _ = _run(ctx, net, sa.default_cfg(mode, threshold=t), y0)
+            np.testing.assert_array_equal(plan.activations(), yL if mode == "BF" else r["yL"])
+    assert panels["1"] < panels["0"]
+
+
 @pytest.mark.parametrize("share", [1, 13, 16, 32])
 def test_matrix_core_and_vector_kernels_agree(ctx, share, monkeypatch):
     """The matrix-core layer kernels (K=1 fused steps: the default plain-layer kernel with
