@@ -219,6 +219,37 @@ def bench_batch(args, wl):
     print(json.dumps(out), flush=True)
 
 
+def short_rows(K):
+    """Panel grouping on a random net whose rows have K < 32 inputs (N=16384, 60 layers,
+    60000 samples at density 0.3): plain BF and SNICIT (t=20) ms/step with rows grouped into
+    shared panels (the default) and with one input set per panel (SNICIT_PANEL_MERGE=0);
+    categories must agree."""
+    import snicit_amd as sa
+
+    N, L, B = 16384, 60, 60000
+    rng = np.random.default_rng(5)
+    cols = rng.integers(0, N, (L, N, K)).astype(np.int32)
+    vals = (rng.standard_normal((L, N, K)) * np.sqrt(2.0 / K)).astype(np.float32)
+    y0 = (rng.random((B, N)) < 0.3).astype(np.float32)
+    ctx = sa.Context(0)
+    cats = {}
+    for merge in ("1", "0"):
+        os.environ["SNICIT_PANEL_MERGE"] = merge
+        net = sa.Net.from_ell(ctx, cols, vals, -0.01)
+        for mode in ("BF", "SNICIT"):
+            plan = sa.Plan(net, sa.default_cfg(mode, threshold=20), B)
+            plan.set_input(y0)
+            ms, _, _ = time_steps(plan, 3, 1)
+            cats[(merge, mode)] = plan.categories().copy()
+            log(f"K={K} grouping={'on' if merge == '1' else 'off'} panels={net.panels} "
+                f"{mode}: {ms:.2f} ms/step")
+            plan.close()
+        net.close()
+    os.environ.pop("SNICIT_PANEL_MERGE", None)
+    same = all(np.array_equal(cats[("1", m)], cats[("0", m)]) for m in ("BF", "SNICIT"))
+    log(f"categories identical with and without grouping: {same}")
+
+
 def stream_ceiling():
     """Device-memory rates of plain streams (best of 5): a sum reads 2 GiB, a fill writes
     2 GiB, a copy reads and writes 2 GiB each.  The ceiling the layer kernels' HBM rates are
@@ -408,9 +439,14 @@ def main():
                          "them through SLOTS device buffers (c1-c3)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="read, write and copy rates of 2 GiB device buffers (torch)")
+    ap.add_argument("--short-rows", type=int, default=0, metavar="K",
+                    help="panel grouping with and without, on a random net of K-input rows")
     args = ap.parse_args()
     if args.stream_ceiling:
         stream_ceiling()
+        return
+    if args.short_rows:
+        short_rows(args.short_rows)
         return
     if args.copy_calibration:
         import torch
