@@ -43,6 +43,9 @@ def lib():
             _I32P, _F32P, ctypes.c_int, _F32P, ctypes.c_float, ctypes.c_int64, _F32P,
             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _I32P, _F32P, _I32P, _F32P,
             _I32P, _I32P, _I64P, _I64P, _I64P, ctypes.c_int]
+        ex = list(L.oracle_snicit_exact.argtypes)
+        L.oracle_snicit_exact_ex.restype = ctypes.c_int
+        L.oracle_snicit_exact_ex.argtypes = ex[:14] + [ctypes.c_int] + ex[14:]
         model = [ctypes.c_int, _F32P, _F32P, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                  ctypes.c_int, _I32P, _F32P, ctypes.c_int, _F32P, ctypes.c_float, ctypes.c_int,
                  _F32P, _F32P, ctypes.c_int64, _F32P]
@@ -120,9 +123,11 @@ def layer_f64(cols_l, vals_l, bias_l, clip, yin):
 
 
 def snicit_exact(cols, vals, bias, clip, y0, t, sample_size=32, segments=16, eps=0.03,
-                 nthreads=0, want_y=True):
+                 nthreads=0, want_y=True, ref_segments=False):
     """Exact SNICIT restatement.  Returns a dict with cat, yL, pool, F, centroids, assign,
-    live (per pre layer), nerow (per post layer) and info (B', pool, k, nerow at t)."""
+    live (per pre layer), nerow (per post layer) and info (B', pool, k, nerow at t).
+    ref_segments=True sums the pool rows over the reference's overlapping segments
+    (quirk Q2, src/SDGC/SNICIT/kernel.hpp:147-148) instead of N/segments-neuron ones."""
     cols = np.ascontiguousarray(cols, np.int32)
     vals = np.ascontiguousarray(vals, np.float32)
     y0 = np.ascontiguousarray(y0, np.float32)
@@ -138,11 +143,13 @@ def snicit_exact(cols, vals, bias, clip, y0, t, sample_size=32, segments=16, eps
     live = np.full(L, -1, np.int64)
     nerow = np.zeros(L, np.int64)
     info = np.zeros(4, np.int64)
-    rc = lib().oracle_snicit_exact(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind, _p(b, _F32P),
-                                   clip, B, _p(y0, _F32P), t, sample_size, segments, eps,
-                                   _p(cat, _I32P), _p(yL, _F32P), _p(pool, _I32P), _p(F, _F32P),
-                                   _p(cent, _I32P), _p(assign, _I32P), _p(live, _I64P),
-                                   _p(nerow, _I64P), _p(info, _I64P), nthreads)
+    rc = lib().oracle_snicit_exact_ex(N, L, K, _p(cols, _I32P), _p(vals, _F32P), kind,
+                                      _p(b, _F32P), clip, B, _p(y0, _F32P), t, sample_size,
+                                      segments, eps, int(bool(ref_segments)),
+                                      _p(cat, _I32P), _p(yL, _F32P), _p(pool, _I32P),
+                                      _p(F, _F32P), _p(cent, _I32P), _p(assign, _I32P),
+                                      _p(live, _I64P), _p(nerow, _I64P), _p(info, _I64P),
+                                      nthreads)
     assert rc == 0, rc
     Bp, P, k, ne0 = (int(x) for x in info)
     return {"cat": cat, "yL": yL, "pool": pool[:P], "F": F[:P * segments].reshape(P, segments),

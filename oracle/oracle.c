@@ -19,14 +19,23 @@
  * (src/SDGC/SNICIT/kernel.hpp:94-98); any fixed order is a faithful restatement, and the
  * ascending order is the one both sides here use.
  *
- * Parity pins (tests/test_oracle.py): closed forms (layer 0 on binary inputs, dead-bias
- * nets, a permutation net, dense brute force), invariants (SNICIT categories == plain
- * categories for any centroid choice; yhat_i + yhat_m(i) == y_i), FP64 agreement, and the
- * committed fixtures under tests/golden/.
+ * Tolerances: the reference writes them as double literals (0.03, 0.04, 0.05) and compares
+ * float magnitudes against them in double (src/SDGC/SNICIT/kernel.hpp:178,184;
+ * src/beyond/SNICIT/kernel.hpp:31,38,90,161).  They reach this library as floats; the value
+ * compared against is the shortest decimal that rounds to the given float (0.05f -> 0.05),
+ * so |x| = 0.05f counts as above 0.05, as in the reference.
+ *
+ * Parity pins (tests/test_cpu.py, tests/test_cpu_cluster.py, tests/test_cpu_beyond.py):
+ * closed forms (layer 0 on binary inputs, dead-bias nets, a permutation net, dense brute
+ * force), invariants (SNICIT categories == plain categories for any centroid choice;
+ * yhat_i + yhat_m(i) == y_i), numpy recomputation of the clustering stages (segment sums,
+ * the Alg. 1 survivor characterisation, first-argmin assignment), hand-built clustering
+ * cases with known answers, FP64 agreement, and the fixtures under tests/golden/.
  */
 #include <math.h>
 #include <omp.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -37,6 +46,16 @@
 /* f(z) = min(max(z,0),clip): src/SDGC/XY/kernel.hpp:5-7 (clip 32),
  * src/beyond/general.hpp:70 (clip 1). */
 static inline float act(float z, float clip) { return fminf(fmaxf(z, 0.f), clip); }
+
+/* Decimal value of a tolerance given as a float: the shortest decimal that rounds to it. */
+static double decimal_of(float e) {
+  char buf[32];
+  for (int p = 1; p <= 9; ++p) {
+    snprintf(buf, sizeof buf, "%.*g", p, (double)e);
+    if (strtof(buf, NULL) == e) return strtod(buf, NULL);
+  }
+  return (double)e;
+}
 static inline double actd(double z, double clip) { return fmin(fmax(z, 0.0), clip); }
 
 static float bias_of(int bias_kind, const float* bias, int N, int l, int o) {
@@ -259,15 +278,40 @@ int oracle_layer_f64(int N, int K, const int32_t* cols_l, const float* vals_l, i
  * before t are 0.  yL (nullable) gets yhat_i + yhat_m(i) (or yhat_m(i) for empty rows).
  * Trace outputs (all nullable): pool_ids[sample_size], F[sample_size*segments], cent_ids[32],
  * assign[B] (centroid index, -1 dead before t), live_per_layer[L] (layers < t),
- * nerow_per_layer[L] (after each post layer), info[4] = {B', npool, k, nerow at t}. */
+ * nerow_per_layer[L] (after each post layer), info[4] = {B', npool, k, nerow at t}.
+ * seg_layout 0: segment g = neurons [g*N/S, (g+1)*N/S) (paper 3.2.1).  seg_layout 1
+ * reproduces the reference's indexing (quirk Q2, kernel.hpp:147-148, launched with
+ * blockDim (1024/S, S) at SNICIT.hpp:434-437): segment g = neurons [g*1024/S, g*1024/S + N/S),
+ * which overlap for N > 1024; it needs 1024 % S == 0 and (S-1)*1024/S + N/S <= N. */
+int oracle_snicit_exact_ex(int N, int L, int K, const int32_t* cols, const float* vals,
+                           int bias_kind, const float* bias, float clip, int64_t B, const float* y0,
+                           int t, int sample_size, int segments, float eps, int seg_layout,
+                           int32_t* cat, float* yL, int32_t* pool_ids, float* F_out,
+                           int32_t* cent_ids, int32_t* assign, int64_t* live_per_layer,
+                           int64_t* nerow_per_layer, int64_t* info, int nthreads);
+
 int oracle_snicit_exact(int N, int L, int K, const int32_t* cols, const float* vals,
                         int bias_kind, const float* bias, float clip, int64_t B, const float* y0,
                         int t, int sample_size, int segments, float eps, int32_t* cat, float* yL,
                         int32_t* pool_ids, float* F_out, int32_t* cent_ids, int32_t* assign,
                         int64_t* live_per_layer, int64_t* nerow_per_layer, int64_t* info,
                         int nthreads) {
+  return oracle_snicit_exact_ex(N, L, K, cols, vals, bias_kind, bias, clip, B, y0, t, sample_size,
+                                segments, eps, 0, cat, yL, pool_ids, F_out, cent_ids, assign,
+                                live_per_layer, nerow_per_layer, info, nthreads);
+}
+
+int oracle_snicit_exact_ex(int N, int L, int K, const int32_t* cols, const float* vals,
+                           int bias_kind, const float* bias, float clip, int64_t B, const float* y0,
+                           int t, int sample_size, int segments, float eps, int seg_layout,
+                           int32_t* cat, float* yL, int32_t* pool_ids, float* F_out,
+                           int32_t* cent_ids, int32_t* assign, int64_t* live_per_layer,
+                           int64_t* nerow_per_layer, int64_t* info, int nthreads) {
   if (nthreads > 0) omp_set_num_threads(nthreads);
   if (t < 0 || t >= L || sample_size < 1 || sample_size > 64 || segments < 1 || N % segments) return -2;
+  if (seg_layout != 0 && (seg_layout != 1 || 1024 % segments ||
+                          (int64_t)(segments - 1) * (1024 / segments) + N / segments > N))
+    return -2;
   sorted_net s;
   if (sort_net(N, L, K, cols, vals, &s)) return -1;
   const int skip = all_bias_nonpositive(bias_kind, bias, N, L);
@@ -313,20 +357,23 @@ int oracle_snicit_exact(int N, int L, int K, const int32_t* cols, const float* v
   float* F = (float*)calloc((size_t)(P > 0 ? P : 1) * segments, sizeof(float));
   /* F: segment g split into S2 = min(32, seglen) contiguous sub-chunks, each summed in
    * ascending neuron order, partials added in ascending order (a fixed order for the
-   * reference's unordered shared-memory atomics, kernel.hpp:147-150). */
+   * reference's unordered shared-memory atomics, kernel.hpp:147-150).  The segment starts
+   * at g*seglen, or at g*(1024/segments) with the reference's indexing. */
   const int S2 = seglen < 32 ? seglen : 32;
+  const int seg_step = seg_layout == 1 ? 1024 / segments : seglen;
   for (int q = 0; q < P; ++q) {
     const float* y = a + (size_t)lid[q] * N;
     for (int g = 0; g < segments; ++g) {
       float sum = 0.f;
       for (int c = 0; c < S2; ++c) {
         float part = 0.f;
-        for (int n = c * seglen / S2; n < (c + 1) * seglen / S2; ++n) part += y[g * seglen + n];
+        for (int n = c * seglen / S2; n < (c + 1) * seglen / S2; ++n) part += y[g * seg_step + n];
         sum += part;
       }
       F[q * segments + g] = sum;
     }
   }
+  const double eps_d = decimal_of(eps);
   int alive[64];
   for (int q = 0; q < 64; ++q) alive[q] = q < P;
   for (int c = 0; c < P; ++c) {
@@ -334,8 +381,9 @@ int oracle_snicit_exact(int N, int L, int K, const int32_t* cols, const float* v
     for (int y = 0; y < P; ++y) {
       if (!alive[y] || y == c) continue;
       int cnt = 0;
-      for (int g = 0; g < segments; ++g) cnt += fabsf(F[y * segments + g] - F[c * segments + g]) > eps;
-      if ((double)cnt < (double)segments * (double)eps) alive[y] = 0;
+      for (int g = 0; g < segments; ++g)
+        cnt += (double)fabsf(F[y * segments + g] - F[c * segments + g]) > eps_d;
+      if ((double)cnt < (double)segments * eps_d) alive[y] = 0;
     }
   }
   int cent[64], k = 0;
@@ -596,7 +644,9 @@ int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, flo
     pcol[y] = (int32_t)((int64_t)y * B / pool);
     alive[y] = 1;
   }
-  const double limit = (double)N * (double)eps_pool;
+  const double eps_pool_d = decimal_of(eps_pool), eps_assign_d = decimal_of(eps_assign);
+  const double eps_post_d = decimal_of(eps_post);
+  const double limit = (double)N * eps_pool_d;
   for (int i = 0; i < P; ++i) {
     if (!alive[i]) continue;
     const float* pi = Y + (size_t)pcol[i] * N;
@@ -604,7 +654,7 @@ int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, flo
       if (y == i || !alive[y]) continue;
       const float* py = Y + (size_t)pcol[y] * N;
       int cnt = 0;
-      for (int n = 0; n < N; ++n) cnt += fabsf(py[n] - pi[n]) > eps_pool;
+      for (int n = 0; n < N; ++n) cnt += (double)fabsf(py[n] - pi[n]) > eps_pool_d;
       if ((double)cnt < limit) alive[y] = 0;
     }
   }
@@ -624,7 +674,7 @@ int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, flo
     for (int c = 0; c < k; ++c) {
       const float* yc = C + (size_t)c * N;
       int d = 0;
-      for (int n = 0; n < N; ++n) d += fabsf(yc[n] - y[n]) > eps_assign;
+      for (int n = 0; n < N; ++n) d += (double)fabsf(yc[n] - y[n]) > eps_assign_d;
       if (d < best) {
         best = d;
         bestc = c;
@@ -642,7 +692,7 @@ int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, flo
     int any = 0;
     for (int n = 0; n < N; ++n) {
       const float d = y[n] - yc[n];
-      const float v = fabsf(d) > eps_assign ? d : 0.f;
+      const float v = (double)fabsf(d) > eps_assign_d ? d : 0.f;
       y[n] = v;
       any |= positive_only ? (v > 0.f) : (v != 0.f);
     }
@@ -683,7 +733,7 @@ int oracle_model_snicit_lossy(int Nin, const float* w_in, const float* b_in, flo
         for (int o = 0; o < N; ++o) {
           const float z = row_dot(&s, l, o, y);
           float v = act(zc[o] + z, clip) - act(zc[o], clip);
-          v = fabsf(v) > eps_post ? v : 0.f;
+          v = (double)fabsf(v) > eps_post_d ? v : 0.f;
           outrow[o] = v;
           any |= v != 0.f;
         }

@@ -73,3 +73,96 @@ def prototype_features(Nin, B, seed):
     m = (r.random((B, Nin)) < 0.05) & (r.random(B) < 0.5)[:, None]
     x[m] += r.standard_normal(m.sum()).astype(np.float32) * 0.1
     return np.clip(x, 0, 1).astype(np.float32)
+
+
+def crafted_cluster_case(N=1024, seed=0):
+    """Threshold-layer input (use t = 0, so Y_t is the input itself) whose exact-clustering
+    result is fixed by construction (16 segments of N/16 neurons, eps 0.03; every value a
+    multiple of 1/64, so every segment sum is exact in FP32).
+
+    Rows 1 and 13 are all zero (dropped before pooling).  In the pool (the first 32 live rows):
+    2 = row 0 + 1/64 at one neuron and 4 = row 0 with two values of one segment swapped (both
+    pruned by 0); 3 = row 0 + 3/64 (kept: one sum differs by more than eps); 6 = a copy of 5;
+    8 = 7 + 1/64 (pruned by 7) and 9 = 7 + 2/64 (kept: 2/64 > eps from 7, and the row within
+    eps of it, 8, was itself pruned, so it prunes nothing); 11 = row 10 - 1/64 (pruned by 10),
+    12 = row 10 - 2/64 (kept); 14..33 are random.  After the pool: a copy of row 3 (distance
+    0), rows equidistant from two or three centroids (the first of them wins), random rows.
+    Rows 8 and 11 are also equidistant (distance 1) from the two centroids next to them.
+    Returns (y0, expect) with expect = {pool, centroids, assign (per sample: centroid index or
+    -1), ne (per sample)}."""
+    rng = np.random.default_rng(seed)
+    vals = np.array([0.0, 0.5, 1.0, 2.0], np.float32)
+
+    def pattern():
+        return rng.choice(vals, size=N, p=[0.55, 0.15, 0.15, 0.15]).astype(np.float32)
+
+    seg = N // 16
+    rows = {}
+    A, Bp, Cp, D = pattern(), pattern(), pattern(), pattern()
+    n1, n2, n3, n4 = 5, seg + 7, 3 * seg + 1, 9 * seg + 2
+    a, b = 2 * seg, 2 * seg + 1
+    A[a], A[b] = 1.0, 2.0
+    rows[0] = A
+    rows[1] = np.zeros(N, np.float32)
+    rows[2] = A.copy()
+    rows[2][n1] += 1 / 64
+    rows[3] = A.copy()
+    rows[3][n2] += 3 / 64
+    rows[4] = A.copy()
+    rows[4][a], rows[4][b] = 2.0, 1.0  # two values of one segment swapped: same sums
+    rows[5] = Bp
+    rows[6] = Bp.copy()
+    rows[7] = Cp
+    rows[8] = Cp.copy()
+    rows[8][n3] += 1 / 64
+    rows[9] = Cp.copy()
+    rows[9][n3] += 2 / 64
+    D[n4] = 1.0
+    rows[10] = D
+    rows[11] = D.copy()
+    rows[11][n4] -= 1 / 64
+    rows[12] = D.copy()
+    rows[12][n4] -= 2 / 64
+    rows[13] = np.zeros(N, np.float32)
+    for i in range(14, 34):
+        rows[i] = pattern()
+    cent = [0, 3, 5, 7, 9, 10, 12] + list(range(14, 34))
+    cidx = {s: c for c, s in enumerate(cent)}
+    assign = {s: cidx[s] for s in cent}
+    assign.update({2: cidx[0], 4: cidx[0], 6: cidx[5], 8: cidx[7], 11: cidx[10]})
+
+    def between(P, Q, third):
+        """A row at equal L0 distance from P and Q, equal to both where they agree."""
+        x = P.copy()
+        d = np.flatnonzero(P != Q)
+        h = len(d) // 2
+        x[d[:h]] = Q[d[:h]]
+        if len(d) % 2:
+            x[d[-1]] = third  # differs from both
+        return x
+
+    rows[34] = rows[3].copy()
+    assign[34] = cidx[3]
+    rows[35] = between(A, Bp, 0.25)
+    assign[35] = cidx[0]
+    rows[36] = between(rows[20], rows[15], 0.25)
+    assign[36] = cidx[15]
+    rows[37] = np.zeros(N, np.float32)
+    rows[38] = between(rows[12], rows[16], 0.25)
+    assign[38] = cidx[12]
+    for i in range(39, 48):
+        rows[i] = pattern()
+    y0 = np.stack([rows[i] for i in range(48)]).astype(np.float32)
+    live = [i for i in range(48) if y0[i].any()]
+    # samples after the pool: nearest centroid by brute force (first minimum); the rows
+    # built to be equidistant were placed so that the expected winner is the first one
+    C = y0[cent]
+    for i in range(39, 48):
+        assign[i] = int(np.argmin((y0[i][None, :] != C).sum(axis=1)))
+    assign_arr = np.full(48, -1, np.int32)
+    ne = np.zeros(48, bool)
+    for i, c in assign.items():
+        assign_arr[i] = c
+        ne[i] = i not in cidx and bool((y0[i] != C[c]).any())
+    return y0, {"pool": np.array(live[:32]), "centroids": np.array(cent), "assign": assign_arr,
+                "ne": ne}

@@ -221,21 +221,6 @@ def test_snicit_exact_equals_plain(t):
     assert r["centroids"][0] == r["pool"][0]
 
 
-def test_snicit_alg1_prunes_identical_rows():
-    """Identical pool rows collapse to one centroid; each row maps to its own copy."""
-    N, L, B = 256, 6, 64
-    cols, vals = nets.random_ell(N, L, K=32, seed=1, share=4)
-    base = nets.random_inputs(N, 4, seed=5, density=0.3)
-    y0 = np.repeat(base, 16, axis=0)  # 4 distinct rows, 16 copies each
-    r = orc.snicit_exact(cols, vals, -0.01, 32.0, y0, 2)
-    live = r["live_at_t"]
-    distinct_live = len({tuple(row) for row in y0[np.flatnonzero(r["assign"] >= 0)]})
-    assert r["k"] <= distinct_live and r["nerow_at_t"] == 0 or r["k"] < 32
-    # every row's delta is empty when its twin is a centroid
-    _, cat, _ = orc.plain_f32(cols, vals, -0.01, 32.0, y0)
-    np.testing.assert_array_equal(r["cat"], cat)
-
-
 def test_snicit_dead_rows():
     N, L = 1024, 20
     cols, vals = nets.sdgc(N, L, seed=7)
