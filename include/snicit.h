@@ -61,7 +61,9 @@ typedef struct snicit_plan snicit_plan;
 /* Run configuration.  snicit_run_cfg_default() fills the reference's values:
  * sample_size 32 and segments 16 (SNICIT.hpp:228-229), eps 0.03 (kernel.hpp:178,184),
  * lossy: pool 128 (main/beyond.cu:38), assign/delta tolerance 0.04 and post prune
- * 0.05 (src/beyond/SNICIT/kernel.hpp:71,90,161). */
+ * 0.05 (src/beyond/SNICIT/kernel.hpp:71,90,161).  A tolerance e counts |x| > e with e
+ * taken as the shortest decimal that rounds to the given float, as the reference compares
+ * float magnitudes with its double literals (0.05f: |x| = 0.05f is above 0.05). */
 typedef struct snicit_run_cfg {
   int32_t mode;            /* SNICIT_MODE_*                                            */
   int32_t threshold;       /* layer t at which clustering runs, 0 <= t < L (SNICIT)   */
@@ -165,7 +167,9 @@ int snicit_plan_create(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
                        snicit_plan** out);
 int snicit_plan_destroy(snicit_plan* plan);
 /* Upload y0 (B rows of N floats, sample-major like the reference's input[b][n],
- * utility.hpp:268-293) into the plan's resident input buffer. */
+ * utility.hpp:268-293) into the plan's resident input buffer.  Values must be finite:
+ * SNICIT_E_INVALID when one is inf or NaN (padded and merged panel slots multiply the
+ * inputs a row does not use by 0, which is exact only for finite inputs). */
 int snicit_plan_set_input(snicit_plan* plan, const float* y0, int64_t B);
 /* Generate B seeded binary blob images (samples first_sample..first_sample+B-1) on the
  * device into the resident input buffer; equals snicit_gen_images on the host. */
@@ -315,7 +319,8 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
                          int32_t n_nets, const snicit_server_cfg* cfg, snicit_server** out);
 /* Stops the worker after the groups in flight; pending requests are dropped. */
 int snicit_server_destroy(snicit_server* server);
-/* samples x (Nin or N) host floats, copied before return. */
+/* samples x (Nin or N) host floats, copied before return; values must be finite (not
+ * checked here, unlike snicit_plan_set_input). */
 int snicit_server_submit(snicit_server* server, int32_t net, const float* x, int64_t samples,
                          uint64_t* request_id);
 /* Device-rendered blob images of samples first_sample.. (nets without an input layer). */

@@ -404,7 +404,7 @@ SpmmLaunch spmm_base(snicit_plan* p, int l, int epi) {
   a.la = plan_layer_arrays(p, l);
   a.epilogue = epi;
   a.kcap = p->kcap;
-  a.eps_post = p->cfg.lossy_eps_post;
+  a.eps_post = tol_cut(p->cfg.lossy_eps_post);
   a.max_cols = p->cap;
   a.shape = p->shape;
   return a;
@@ -583,7 +583,7 @@ void enqueue_post(snicit_plan* p) {
                         p->kcap, p->d_k,
                         q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
                         p->d_nerow + l, map_in, p->d_mcar[mc], p->d_flags,
-                        p->cfg.lossy_eps_post, p->cap, p->shape, s);
+                        tol_cut(p->cfg.lossy_eps_post), p->cap, p->shape, s);
       prof_mark(p, l, 1);
       prof_mark(p, l, 2);
       prof_mark(p, l, 3);
@@ -851,7 +851,7 @@ void plan_free(snicit_plan* p) {
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
                   p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
-                  p->d_stage};
+                  p->d_stage,  p->d_nonfinite};
   for (void* q : ptrs) dev_free(q);
 }
 
@@ -1356,6 +1356,7 @@ void ensure_stage(snicit_plan* p) {
   const Net& n = *p->net;
   const int64_t w = std::max<int64_t>(n.N, std::max<int64_t>(n.Nin, n.Nout));
   if (!p->d_stage) p->d_stage = dev_alloc<float>((size_t)p->cap * w, &p->device_bytes);
+  if (!p->d_nonfinite) p->d_nonfinite = dev_alloc<int32_t>(1, &p->device_bytes);
 }
 void load_host_input(snicit_plan* p, const float* y0, int64_t B, int64_t col_off);
 void load_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B,
@@ -1432,13 +1433,25 @@ void load_images(snicit_plan* p, uint64_t seed, int64_t first_sample, int64_t B,
 }
 
 // Copies B host samples (B x width, sample-major) into columns col_off.. of the plan's input.
+// Inputs must be finite: rows shorter than a panel's 32 slots and merged short-row panels
+// add value-0 steps fma(0, x, acc), which equal acc only for finite x, and the reference
+// never reads those inputs for such a row.  A non-finite value is refused here.
 void load_host_input(snicit_plan* p, const float* y0, int64_t B, int64_t col_off) {
   if (B <= 0) return;
   const int width = p->net->Nin > 0 ? p->net->Nin : p->net->N;
   hipStream_t s = p->stream;
   ensure_stage(p);
   SNICIT_HIP(hipMemcpyAsync(p->d_stage, y0, (size_t)B * width * 4, hipMemcpyHostToDevice, s));
-  launch_transpose_in(p->d_stage, B, width, p->d_in + col_off, p->ld, s);
+  launch_fill_i32(p->d_nonfinite, 0, 1, s);
+  launch_transpose_in(p->d_stage, B, width, p->d_in + col_off, p->ld, p->d_nonfinite, s);
+  int32_t bad = 0;
+  SNICIT_HIP(hipMemcpyAsync(&bad, p->d_nonfinite, 4, hipMemcpyDeviceToHost, s));
+  SNICIT_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    p->have_input = false;
+    p->have_run = false;
+    throw Error(SNICIT_E_INVALID, "input holds a non-finite value (inf or NaN)");
+  }
 }
 }  // namespace snicit
 

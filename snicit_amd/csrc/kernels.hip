@@ -3,9 +3,15 @@
 // Layout: activations are feature-major, Y[n * ld + pos] (neuron n of the sample held in
 // column pos).  A wavefront owns 64 consecutive columns, so each neuron access is one
 // contiguous 256-byte run; weights, panel inputs and output ids are wavefront-uniform and
-// are fetched with scalar loads.  The bound is HBM bandwidth (one read and one write of
-// every live activation per layer); there is no MFMA.
+// are fetched with scalar loads.  The sparse layers are bound by HBM bandwidth (one read and
+// one write of every live activation per layer).  The plain panel layer and the dense input
+// and output layers run their fixed-order multiply-add chains on the matrix cores
+// (v_mfma_f32_32x32x1_2b_f32: with K = 1 each element is one fused multiply-add, so the
+// chain's bits are those of the vector form); everything else runs on the vector ALU.
 #include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "kernels.hpp"
 
@@ -1122,7 +1128,8 @@ __global__ __launch_bounds__(256) void lossy_pool_kernel(const float* __restrict
 // the only ones the walk can remove, and the survivors are those of the sequential rule.
 __global__ __launch_bounds__(1024) void lossy_close_kernel(const float* __restrict__ poolrows,
                                                           const int32_t* __restrict__ seglo,
-                                                          int32_t pool, int32_t N, float eps,
+                                                          int32_t pool, int32_t N, float cut,
+                                                          double limit,
                                                           uint64_t* __restrict__ close) {
   __shared__ uint64_t bits[2];
   const int i = blockIdx.x, sg = blockIdx.y;
@@ -1134,7 +1141,6 @@ __global__ __launch_bounds__(1024) void lossy_close_kernel(const float* __restri
   if (i < P) {
     const float* __restrict__ rows = poolrows + (int64_t)sg * pool * N;
     const float* __restrict__ pi = rows + (int64_t)i * N;
-    const double limit = (double)N * (double)eps;
     for (int y = i + 1 + wv; y < P; y += 16) {
       const float* __restrict__ py = rows + (int64_t)y * N;
       int cnt = 0;
@@ -1142,7 +1148,7 @@ __global__ __launch_bounds__(1024) void lossy_close_kernel(const float* __restri
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int n = n0 + u * kWave + lane;
-          const bool d = n < N && fabsf(py[n] - pi[n]) > eps;
+          const bool d = n < N && fabsf(py[n] - pi[n]) > cut;
           cnt += __popcll(__ballot(d));
         }
       }
@@ -1481,14 +1487,19 @@ __global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
                                                          int64_t rows, int64_t cols,
                                                          int64_t ldi, float* __restrict__ out,
-                                                         int64_t ldo) {
+                                                         int64_t ldo,
+                                                         int32_t* __restrict__ nonfinite) {
   __shared__ float tile[64][65];
   const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  bool bad = false;
   for (int k = ty; k < 64; k += 4) {
     const int64_t r = r0 + k, c = c0 + tx;
-    tile[k][tx] = (r < rows && c < cols) ? in[r * ldi + c] : 0.f;
+    const float v = (r < rows && c < cols) ? in[r * ldi + c] : 0.f;
+    bad |= !__builtin_isfinite(v);
+    tile[k][tx] = v;
   }
+  if (nonfinite && bad) *nonfinite = 1;
   __syncthreads();
   for (int k = ty; k < 64; k += 4) {
     const int64_t c = c0 + k, r = r0 + tx;
@@ -1626,8 +1637,8 @@ __global__ __launch_bounds__(1024) void seg_sums_kernel(const float* __restrict_
 __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restrict__ F,
                                                            const int32_t* __restrict__ seglo,
                                                            int32_t nseg, int32_t pool,
-                                                           int32_t segments, float eps,
-                                                           int32_t stride,
+                                                           int32_t segments, float cut,
+                                                           double limit, int32_t stride,
                                                            int32_t* __restrict__ cent_slot,
                                                            int32_t* __restrict__ kseg,
                                                            int32_t* __restrict__ kcols) {
@@ -1642,8 +1653,8 @@ __global__ __launch_bounds__(64) void select_exact_kernel(const float* __restric
     if (alive && lane != c) {
       int cnt = 0;
       for (int g = 0; g < segments; ++g)
-        cnt += fabsf(Fs[lane * segments + g] - Fs[c * segments + g]) > eps;
-      if ((double)cnt < (double)segments * (double)eps) alive = false;
+        cnt += fabsf(Fs[lane * segments + g] - Fs[c * segments + g]) > cut;
+      if ((double)cnt < limit) alive = false;
     }
   }
   const uint64_t am = __ballot(alive);
@@ -1996,6 +2007,22 @@ inline void check_launch() {
 
 }  // namespace
 
+double tol_decimal(float e) {
+  char buf[32];
+  for (int p = 1; p <= 9; ++p) {
+    snprintf(buf, sizeof buf, "%.*g", p, (double)e);
+    if (strtof(buf, nullptr) == e) return strtod(buf, nullptr);
+  }
+  return (double)e;
+}
+
+float tol_cut(float e) {
+  const double d = tol_decimal(e);
+  float c = (float)d;
+  if ((double)c > d) c = nextafterf(c, -INFINITY);
+  return c;
+}
+
 void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const Net& net = *a.net;
   const int l = a.layer;
@@ -2316,7 +2343,8 @@ void launch_lossy_select(const float* poolrows, const int32_t* poolcol, const Se
                  "pool larger than the centroid stride");
   launch_fill_i32(sg.kcols, 0, 1, s);
   hipLaunchKernelGGL(lossy_close_kernel, dim3((unsigned)pool, (unsigned)sg.nseg), dim3(1024), 0,
-                     s, poolrows, sg.seglo, pool, N, eps, close);
+                     s, poolrows, sg.seglo, pool, N, tol_cut(eps), (double)N * tol_decimal(eps),
+                     close);
   check_launch();
   hipLaunchKernelGGL(lossy_prune_kernel, dim3((unsigned)sg.nseg), dim3(kWave), 0, s, close,
                      poolcol, sg.seglo, pool, sg.stride, cent_col, sg.kseg, sg.kcols);
@@ -2333,7 +2361,7 @@ void launch_lossy_assign(const float* Y, int64_t ld, const Segs& sg, int64_t max
   // (64-column tiles, blocks of eight centroids; a block past a segment's k does nothing)
   dim3 grid(grid1(max_cols, kWave, 1 << 30), (unsigned)(sg.stride / 8));
   hipLaunchKernelGGL(lossy_dist_kernel, grid, dim3(kDistWaves * kWave), 0, s, Y, ld, sg.seglo,
-                     sg.nseg, C, kcap, sg.stride, sg.kseg, N, eps, dist);
+                     sg.nseg, C, kcap, sg.stride, sg.kseg, N, tol_cut(eps), dist);
   check_launch();
   hipLaunchKernelGGL(lossy_final_kernel, dim3(grid1(max_cols, 256, 1 << 30)), dim3(256), 0, s,
                      sg.seglo, sg.nseg, sg.kseg, sg.stride, cent_col, dist, ld, mcar, centmark,
@@ -2347,7 +2375,7 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
                         int64_t ldD, int32_t* flags, hipStream_t s) {
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(lossy_delta_kernel, grid, dim3(256), 0, s, Y, ld, C, kcap, mcar, centmark, B,
-                     N, eps, positive_only ? 1 : 0, D, ldD, flags);
+                     N, tol_cut(eps), positive_only ? 1 : 0, D, ldD, flags);
   check_launch();
 }
 
@@ -2385,17 +2413,17 @@ void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
 }
 
 void launch_transpose_in(const float* src, int64_t B, int32_t N, float* dst, int64_t ld,
-                         hipStream_t s) {
+                         int32_t* nonfinite, hipStream_t s) {
   dim3 grid((unsigned)((N + 63) / 64), (unsigned)((B + 63) / 64));
   hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, src, B, (int64_t)N, (int64_t)N,
-                     dst, ld);
+                     dst, ld, nonfinite);
   check_launch();
 }
 void launch_transpose_out(const float* src, int64_t ld, int64_t B, int32_t N, float* dst,
                           hipStream_t s) {
   dim3 grid((unsigned)((B + 63) / 64), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, src, (int64_t)N, B, ld, dst,
-                     (int64_t)N);
+                     (int64_t)N, (int32_t*)nullptr);
   check_launch();
 }
 void launch_scatter_cols(const float* src, int64_t ldsrc, const int32_t* map,
@@ -2453,7 +2481,8 @@ void launch_select_exact(const float* F, const Segs& sg, int32_t pool, int32_t s
   SNICIT_REQUIRE(pool <= sg.stride && sg.stride <= 64, "bad exact centroid layout");
   launch_fill_i32(sg.kcols, 0, 1, s);
   hipLaunchKernelGGL(select_exact_kernel, dim3((unsigned)sg.nseg), dim3(64), 0, s, F, sg.seglo,
-                     sg.nseg, pool, segments, eps, sg.stride, cent_slot, sg.kseg, sg.kcols);
+                     sg.nseg, pool, segments, tol_cut(eps), (double)segments * tol_decimal(eps),
+                     sg.stride, cent_slot, sg.kseg, sg.kcols);
   check_launch();
 }
 void launch_gather_centroids(const float* Y, int64_t ld, const int32_t* map,

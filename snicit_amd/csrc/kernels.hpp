@@ -26,6 +26,15 @@ struct Segs {
   int32_t* kcols;        // device [1]
 };
 
+// Tolerances.  The reference writes them as double literals (0.03, 0.04, 0.05) and
+// compares float magnitudes with them in double (src/SDGC/SNICIT/kernel.hpp:178,184;
+// src/beyond/SNICIT/kernel.hpp:31,38,90,161).  A tolerance arrives as a float; its value is
+// the shortest decimal that rounds to that float (tol_decimal: 0.05f -> 0.05), and the
+// kernels compare |x| > tol_cut(e), the largest float not above that decimal, which for
+// every float x equals (double)|x| > tol_decimal(e).
+double tol_decimal(float e);
+float tol_cut(float e);
+
 // Launch shape of the layer kernels (chosen per net, batch size and density by the autotuner):
 // grid_target = workgroups a layer launch aims for (panels x column tiles), narrow = column
 // count up to which the thread-per-(row, column) path runs.
@@ -145,8 +154,9 @@ void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
 
 // [B][N] row-major (device) -> Y[n*ld + b]
+// nonfinite (nullable): set to 1 when any value read is inf or NaN.
 void launch_transpose_in(const float* src, int64_t B, int32_t N, float* dst, int64_t ld,
-                         hipStream_t s);
+                         int32_t* nonfinite, hipStream_t s);
 // Y[n*ld + b] -> [B][N] row-major (device), b < B
 void launch_transpose_out(const float* src, int64_t ld, int64_t B, int32_t N, float* dst,
                           hipStream_t s);

@@ -55,6 +55,7 @@ struct snicit_plan {
   int32_t* d_img_blobs = nullptr;
   int32_t* d_img_proto = nullptr;
   float* d_stage = nullptr;  // [cap][max(N, Nin, Nout)] staging for host transfers
+  int32_t* d_nonfinite = nullptr;  // [1] set when a host input holds inf or NaN
   // nets with dense input / output layers
   float* d_h0 = nullptr;      // [N][ld] input-layer output (hidden loop input), later the
                               //   recovered hidden activations

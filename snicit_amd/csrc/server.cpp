@@ -200,7 +200,7 @@ void launch_group(snicit_server* S, Slot* sl) {
   }
   flush(off);
   // rows of image requests in the staging buffer are overwritten by their renders below
-  if (any_host) launch_transpose_in(p->d_stage, total, width, p->d_in, p->ld, s);
+  if (any_host) launch_transpose_in(p->d_stage, total, width, p->d_in, p->ld, nullptr, s);
   off = 0;
   for (const Request* r : sl->reqs) {
     if (r->images) load_images(p, r->seed, r->first, r->samples, off);
@@ -328,6 +328,11 @@ void worker_main(snicit_server* S) {
         } catch (const std::exception& e) {
           rc = SNICIT_E_INVALID;
           err = e.what();
+        }
+        if (rc != SNICIT_OK) {
+          // copies from the arena blocks and the staging rows may already be queued on the
+          // slot's stream: let them end before the blocks and the slot are released
+          (void)hipStreamSynchronize(sl->stream);
         }
         lk.lock();
         if (rc != SNICIT_OK) {

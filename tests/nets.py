@@ -166,3 +166,22 @@ def crafted_cluster_case(N=1024, seed=0):
         ne[i] = i not in cidx and bool((y0[i] != C[c]).any())
     return y0, {"pool": np.array(live[:32]), "centroids": np.array(cent), "assign": assign_arr,
                 "ne": ne}
+
+
+def post_tolerance_case():
+    """One hidden layer (N = 64, one entry per row, clip 1, bias 0) and two samples: sample 0
+    is all zero and becomes the only centroid; sample 1 is one-hot at neuron 7, so its delta
+    at clustering is 1 at neuron 7.  The layer's delta is then exactly w for the rows that read
+    neuron 7: w = 0.05f (just above 0.05, so kept by the reference's |v| > 0.05 test in
+    double, src/beyond/SNICIT/kernel.hpp:161) on rows 0 and 2, and the float just below
+    0.05f (pruned) on row 1.  Returns (cols, vals, y0, kept_rows)."""
+    N = 64
+    cols = np.arange(N, dtype=np.int32).reshape(1, N, 1).copy()
+    vals = np.zeros((1, N, 1), np.float32)
+    for o, w in ((0, np.float32(0.05)), (1, np.nextafter(np.float32(0.05), np.float32(0))),
+                 (2, np.float32(0.05))):
+        cols[0, o, 0] = 7
+        vals[0, o, 0] = w
+    y0 = np.zeros((2, N), np.float32)
+    y0[1, 7] = 1.0
+    return cols, vals, y0, [0, 2]

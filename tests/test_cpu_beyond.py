@@ -191,3 +191,15 @@ def test_real_net_a_formats_against_float64():
     np.testing.assert_allclose(yL, h, rtol=0, atol=1e-4)
     np.testing.assert_allclose(lg, ref_lg, rtol=0, atol=1e-3)
     assert (cat == np.argmax(ref_lg, axis=1)).mean() > 0.99
+
+
+def test_post_tolerance_is_the_decimal_value():
+    """A post-layer delta of exactly 0.05f is kept (0.05f > 0.05 in double, as the reference
+    compares) and the float below it is pruned."""
+    cols, vals, y0, kept = nets.post_tolerance_case()
+    r = orc.model_snicit_lossy(cols, vals, 0.0, 1.0, y0, 0)
+    assert r["k"] == 1 and r["nerow_at_t"] == 1
+    expect = np.zeros(y0.shape[1], np.float32)
+    expect[kept] = np.float32(0.05)
+    np.testing.assert_array_equal(r["yL"][1], expect)
+    assert r["cat"][1] == 1 and r["cat"][0] == 0

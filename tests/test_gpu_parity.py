@@ -225,6 +225,16 @@ def test_invalid_arguments(ctx):
         plan.run()  # no input yet
     with pytest.raises(sa.SnicitError):
         plan.set_input(np.zeros((65, N), np.float32))
+    # non-finite inputs are refused, and the plan needs a new input before it runs
+    for bad in (np.inf, -np.inf, np.nan):
+        y = np.zeros((64, N), np.float32)
+        y[37, 1000] = bad
+        with pytest.raises(sa.SnicitError):
+            plan.set_input(y)
+        with pytest.raises(sa.SnicitError):
+            plan.run()
+    plan.set_input(np.zeros((64, N), np.float32))
+    plan.run()
 
 
 def test_graph_replay_deterministic(ctx):
