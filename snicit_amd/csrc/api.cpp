@@ -221,6 +221,32 @@ bool compacting(const snicit_plan* p) {
   return p->net->skip_dead && p->cfg.mode != SNICIT_MODE_PLAIN_DENSE && !p->net->has_io();
 }
 
+// Whether a plan's post-threshold layers use the compact layout: exact clustering in one
+// batch segment, with every post layer run by the fused panel kernel.
+bool compact_post_layout(const snicit_plan* p) {
+  if (p->cfg.mode != SNICIT_MODE_SNICIT || p->cfg.cluster != SNICIT_CLUSTER_EXACT) return false;
+  if (p->kcap != kMaxCentroidsExact) return false;
+  for (int l = p->cfg.threshold; l < p->net->L; ++l)
+    if (!post_layer_fused(*p->net, l) || p->net->is_ell(l)) return false;
+  return true;
+}
+
+// Compact post layout: the delta list is compacted after post layer j (0-based, of n_post):
+// after each of the first four, then at doubling gaps up to every kPostRing layers, and
+// after the last.  In between, rows that became empty stay in the list as zero rows (their
+// deltas stay exactly zero), and every layer reads and writes the same positions.
+bool post_compacts_after(int j, int n_post) {
+  if (j == n_post - 1 || j < 4) return true;
+  if (((j + 1) & j) == 0) return true;  // j + 1 is a power of two
+  return (j + 1) % kPostRing == 0;
+}
+
+int post_compactions(int n_post) {
+  int n = 0;
+  for (int j = 0; j < n_post; ++j) n += post_compacts_after(j, n_post) ? 1 : 0;
+  return n;
+}
+
 // Input of hidden layer 0: the input-layer output when the net has one.
 const float* hidden_in(const snicit_plan* p) { return p->net->Nin > 0 ? p->d_h0 : p->d_in; }
 
@@ -560,12 +586,65 @@ void enqueue_cluster(snicit_plan* p) {
   launch_compact(p->d_flags, count, p->d_nerow + t, p->d_mapD, map ? map : ident, p->d_orig[cur],
                  p->d_orig[cur ^ 1], p->d_mcar[0], p->d_mcar[1], p->cap, p->d_tilecnt, s);
   launch_make_delta(st.Yt, p->ld, p->d_mapD, p->d_C[0], p->kcap, p->d_mcar[1], p->d_nerow + t,
-                    p->cap, N, st.Da, p->ld, s);
+                    p->cap, N, st.Da, p->ld, p->compact_post, s);
+}
+
+// List positions post layer j of the compact layout processes (device counter).
+const int32_t* post_count(const snicit_plan* p, int j) {
+  const int t = p->cfg.threshold;
+  return j == 0 ? p->d_nerow + t : p->d_nlist + t + j;
+}
+
+// Stage "post" in the compact layout: one fused launch per layer, reading and writing the
+// delta rows at fixed list positions with stride compact_ld(positions); at the layers
+// post_compacts_after names, the per-layer flags since the last compaction are counted
+// (non-empty rows after each layer) and the list is compacted.  Every layer's centroid
+// rows, delta rows and counts equal those of the per-layer compaction.
+void enqueue_post_compact(snicit_plan* p) {
+  hipStream_t s = p->stream;
+  const int t = p->cfg.threshold, L = p->net->L, n_post = L - t;
+  SnicitState st = snicit_state(p);
+  int orig = cluster_cur(p) ^ 1, mc = 1;
+  int j0 = 0;  // first layer since the last compaction
+  weights_begin(p);
+  for (int j = 0; j < n_post; ++j) {
+    const int l = t + j;
+    use_layer(p, l, L);
+    const int q = j & 1;
+    const bool after_compaction = j > 0 && post_compacts_after(j - 1, n_post);
+    const bool compact_next = post_compacts_after(j, n_post);
+    int32_t* ring = p->d_ring + (int64_t)(j % kPostRing) * p->ld;
+    prof_mark(p, l, 0);
+    launch_panel_post(*p->net, l, plan_layer_arrays(p, l), false, p->d_C[q], p->d_C[q ^ 1],
+                      p->kcap, p->d_k, q == 0 ? st.Da : st.Db, p->ld,
+                      post_count(p, j == 0 ? 0 : j - 1), q == 0 ? st.Db : st.Da, p->ld, true,
+                      post_count(p, j), compact_next ? nullptr : p->d_nlist + l + 1,
+                      after_compaction ? p->d_mapD : nullptr, p->d_mcar[mc], ring, 0.f, p->cap,
+                      p->shape, s);
+    prof_mark(p, l, 1);
+    prof_mark(p, l, 2);
+    prof_mark(p, l, 3);
+    done_layer(p, l);
+    if (compact_next) {
+      launch_ring_count(p->d_ring, p->ld, j0, j - j0 + 1, post_count(p, j),
+                        p->d_nerow + t + j0 + 1, s);
+      launch_compact(ring, post_count(p, j), p->d_nlist + l + 1, p->d_mapD, nullptr,
+                     p->d_orig[orig], p->d_orig[orig ^ 1], p->d_mcar[mc], p->d_mcar[mc ^ 1],
+                     p->cap, nullptr, s);
+      orig ^= 1;
+      mc ^= 1;
+      j0 = j + 1;
+    }
+  }
 }
 
 // Stage "post" (src/SDGC/SNICIT/SNICIT.hpp:526-538,603-614): per layer the centroid rows,
 // then the delta rows with the fused residue epilogue, then the device-side recompaction.
 void enqueue_post(snicit_plan* p) {
+  if (p->compact_post) {
+    enqueue_post_compact(p);
+    return;
+  }
   hipStream_t s = p->stream;
   const int t = p->cfg.threshold, L = p->net->L;
   SnicitState st = snicit_state(p);
@@ -580,10 +659,10 @@ void enqueue_post(snicit_plan* p) {
     if (post_layer_fused(*p->net, l) && !p->net->is_ell(l) && p->kcap == kMaxCentroidsExact) {
       prof_mark(p, l, 0);
       launch_panel_post(*p->net, l, plan_layer_arrays(p, l), lossy, p->d_C[q], p->d_C[q ^ 1],
-                        p->kcap, p->d_k,
-                        q == 0 ? st.Da : st.Db, p->ld, q == 0 ? st.Db : st.Da, p->ld,
-                        p->d_nerow + l, map_in, p->d_mcar[mc], p->d_flags,
-                        tol_cut(p->cfg.lossy_eps_post), p->cap, p->shape, s);
+                        p->kcap, p->d_k, q == 0 ? st.Da : st.Db, p->ld, nullptr,
+                        q == 0 ? st.Db : st.Da, p->ld, false, p->d_nerow + l, nullptr, map_in,
+                        p->d_mcar[mc], p->d_flags, tol_cut(p->cfg.lossy_eps_post), p->cap,
+                        p->shape, s);
       prof_mark(p, l, 1);
       prof_mark(p, l, 2);
       prof_mark(p, l, 3);
@@ -623,6 +702,22 @@ void enqueue_post(snicit_plan* p) {
   }
 }
 
+// Delta-list positions each post layer processed in the last run ([L + 2], index = layer):
+// the non-empty rows entering it, or in the compact layout the list length, which keeps
+// rows that became empty until the next compaction.
+std::vector<int32_t> post_list_lengths(const snicit_plan* p, const std::vector<int32_t>& nerow) {
+  if (!p->compact_post) return nerow;
+  std::vector<int32_t> list(p->net->L + 2);
+  SNICIT_HIP(hipMemcpy(list.data(), p->d_nlist, list.size() * 4, hipMemcpyDeviceToHost));
+  list[p->cfg.threshold] = nerow[p->cfg.threshold];
+  return list;
+}
+
+// Positions the last post layer wrote (the stride of last_D in the compact layout), or null.
+const int32_t* last_D_count(const snicit_plan* p) {
+  return p->compact_post ? post_count(p, p->net->L - p->cfg.threshold - 1) : nullptr;
+}
+
 void enqueue_recover(snicit_plan* p) {
   hipStream_t s = p->stream;
   const int L = p->net->L, N = p->net->N;
@@ -634,7 +729,7 @@ void enqueue_recover(snicit_plan* p) {
     // layer and argmax
     launch_expand_centroids(p->d_rec, p->d_counts, p->cap, p->last_C, p->kcap, N, p->d_h0, p->ld,
                             s);
-    launch_scatter_delta(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar],
+    launch_scatter_delta(p->last_D, p->ld, last_D_count(p), p->d_mapD, p->d_mcar[p->last_mcar],
                          p->d_orig[p->last_orig], p->d_nerow + L, p->cap, p->last_C, p->kcap, N,
                          p->d_h0, p->ld, s);
     enqueue_output_layer(p, p->d_h0);
@@ -642,8 +737,9 @@ void enqueue_recover(snicit_plan* p) {
   }
   launch_centroid_cats(p->last_C, p->kcap, p->d_k, N, p->d_catc, s);
   launch_recover_base(p->d_rec, p->d_counts, p->cap, p->d_catc, p->d_cat, s);
-  launch_recover_rows(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar], p->d_orig[p->last_orig],
-                      p->d_nerow + L, p->cap, p->last_C, p->kcap, N, p->d_act, p->d_cat, s);
+  launch_recover_rows(p->last_D, p->ld, last_D_count(p), p->d_mapD, p->d_mcar[p->last_mcar],
+                      p->d_orig[p->last_orig], p->d_nerow + L, p->cap, p->last_C, p->kcap, N,
+                      p->d_act, p->d_cat, s);
 }
 
 // Runs `body` directly or through a captured graph cached in slot `g`.  Plans on streamed
@@ -696,12 +792,14 @@ void plan_bookkeeping(snicit_plan* p, int32_t stop) {
     p->last_list = compacting(p) && l_end > 0;
     p->stop_layer = l_end;
   } else {
-    // each post layer flips every slot
+    // each compaction of the delta list flips the original-id and centroid slots: one per
+    // post layer, or post_compactions() of them in the compact layout
     const int t = p->cfg.threshold;
     SnicitState st = snicit_state(p);
     const int n_post = L - t;
-    p->last_orig = (cluster_cur(p) ^ 1) ^ (n_post & 1);
-    p->last_mcar = 1 ^ (n_post & 1);
+    const int flips = p->compact_post ? post_compactions(n_post) : n_post;
+    p->last_orig = (cluster_cur(p) ^ 1) ^ (flips & 1);
+    p->last_mcar = 1 ^ (flips & 1);
     p->last_C = p->d_C[n_post & 1];
     p->last_D = (((n_post - 1) & 1) == 0) ? st.Db : st.Da;
     p->stop_layer = L;
@@ -817,7 +915,8 @@ void plan_stats(snicit_plan* p, snicit_stats* stats, const float* stage_ms) {
       stats->live_at_threshold = compacting(p) ? counts[t == 0 ? 1 : t] : counts[0];
       stats->nerow_first = nerow[t];
       stats->nerow_last = nerow[L];
-      for (int l = t; l < L; ++l) rows += nerow[l] + k;
+      const std::vector<int32_t> list = post_list_lengths(p, nerow);
+      for (int l = t; l < L; ++l) rows += list[l] + k;
     }
     stats->rows_processed = rows;
     std::vector<int32_t> cat(p->B);
@@ -851,7 +950,7 @@ void plan_free(snicit_plan* p) {
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
                   p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
-                  p->d_stage,  p->d_nonfinite};
+                  p->d_stage,  p->d_nonfinite, p->d_nlist, p->d_ring};
   for (void* q : ptrs) dev_free(q);
 }
 
@@ -1303,6 +1402,13 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_HIP(hipMemset(p->d_C[0], 0, (size_t)N * p->kcap * 4));
       SNICIT_HIP(hipMemset(p->d_C[1], 0, (size_t)N * p->kcap * 4));
       SNICIT_HIP(hipMemset(p->d_zc, 0, (size_t)N * p->kcap * 4));
+      p->compact_post = compact_post_layout(p.get());
+      if (p->compact_post) {
+        p->d_nlist = dev_alloc<int32_t>(L + 2, acct);
+        p->d_ring = dev_alloc<int32_t>((size_t)kPostRing * p->ld, acct);
+        SNICIT_HIP(hipMemset(p->d_nlist, 0, (L + 2) * 4));
+        SNICIT_HIP(hipMemset(p->d_ring, 0, (size_t)kPostRing * p->ld * 4));
+      }
       for (auto& e : p->ev) SNICIT_HIP(hipEventCreate(&e));
       if (net->streamed()) {
         const int nb = net->stream_buffers;
@@ -1518,9 +1624,9 @@ int snicit_plan_activations(snicit_plan* p, float* y) {
       } else {
         launch_expand_centroids(p->d_rec, p->d_counts, p->cap, p->last_C, p->kcap, N, full, p->ld,
                                 s);
-        launch_scatter_delta(p->last_D, p->ld, p->d_mapD, p->d_mcar[p->last_mcar],
-                             p->d_orig[p->last_orig], p->d_nerow + net->L, p->cap, p->last_C,
-                             p->kcap, N, full, p->ld, s);
+        launch_scatter_delta(p->last_D, p->ld, last_D_count(p), p->d_mapD,
+                             p->d_mcar[p->last_mcar], p->d_orig[p->last_orig], p->d_nerow + net->L,
+                             p->cap, p->last_C, p->kcap, N, full, p->ld, s);
       }
       launch_transpose_out(full, p->ld, p->B, N, p->d_stage, s);
       SNICIT_HIP(hipMemcpyAsync(y, p->d_stage, (size_t)p->B * N * 4, hipMemcpyDeviceToHost, s));
@@ -1674,6 +1780,7 @@ int snicit_plan_profile(snicit_plan* p, double* ms, int64_t* rows, double* cent_
     SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
     const bool snic = !plain_mode(p->cfg.mode);
     const int t = snic ? p->cfg.threshold : L;
+    const std::vector<int32_t> list = snic ? post_list_lengths(p, nerow) : nerow;
     for (int l = 0; l < std::min<int64_t>(L, cap); ++l) {
       float a = 0.f, c = 0.f;
       SNICIT_HIP(hipEventElapsedTime(&a, p->prof_ev[4 * l], p->prof_ev[4 * l + 1]));
@@ -1681,7 +1788,7 @@ int snicit_plan_profile(snicit_plan* p, double* ms, int64_t* rows, double* cent_
         SNICIT_HIP(hipEventElapsedTime(&c, p->prof_ev[4 * l + 2], p->prof_ev[4 * l + 3]));
       if (ms) ms[l] = a;
       if (cent_ms) cent_ms[l] = c;
-      if (rows) rows[l] = (snic && l >= t) ? nerow[l] : (compacting(p) ? counts[l] : counts[0]);
+      if (rows) rows[l] = (snic && l >= t) ? list[l] : (compacting(p) ? counts[l] : counts[0]);
     }
   });
 }

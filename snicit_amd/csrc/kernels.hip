@@ -321,16 +321,22 @@ __global__ __launch_bounds__(kBlock) void panel_centroid_kernel(
 // `narrow` use one thread per (row, column) pair in block row 0; otherwise each
 // wavefront owns 64 columns and every row of the panel.  Same fused-multiply-add chain as
 // panel_spmm_kernel.
+// Compact post layout (ldin_cnt non-null, compact_out): the delta buffers' row stride is
+// compact_ld of the positions they hold, read from the device counters, so a layer's delta
+// rows fill one dense region; nlist_next (nullable) receives the list length for the next
+// layer when no compaction runs in between.  The counters, the centroid count and the
+// panel bounds are loaded together before the early exit.
 template <bool LOSSY>
 __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
     const int32_t* __restrict__ outs, const float* __restrict__ w,
     const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ cin,
     float* __restrict__ cout, int32_t kcap, const int32_t* __restrict__ k_ptr,
-    const float* __restrict__ din, int64_t ldin, float* __restrict__ dout, int64_t ldout,
-    const int32_t* __restrict__ count_ptr, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post,
-    int32_t narrow) {
+    const float* __restrict__ din, int64_t ldin_arg, const int32_t* __restrict__ ldin_cnt,
+    float* __restrict__ dout, int64_t ldout_arg, int32_t compact_out,
+    const int32_t* __restrict__ count_ptr, int32_t* __restrict__ nlist_next,
+    const int32_t* __restrict__ map, const int32_t* __restrict__ mcent,
+    int32_t* __restrict__ flags, float eps_post, int32_t narrow) {
   constexpr int kC = kMaxCentroidsExact + 1;  // padded centroid stride in LDS
   __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
   __shared__ float x_s[kSlots * kC];
@@ -338,14 +344,16 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   __shared__ int32_t o_s[kMaxPanelRows];
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
-  const int beg0 = pbeg[p], G0 = pbeg[p + 1] - beg0;
-  // narrow: every workgroup of the panel takes (row, column) pairs; wide: 256 columns each.
-  // Workgroup row 0 always runs: it writes the centroid rows, delta rows or not.
-  if (blockIdx.y != 0 && (count <= narrow ? (int64_t)blockIdx.y * kBlock >= (int64_t)G0 * count
-                                          : (int64_t)blockIdx.y * kBlock >= count))
-    return;
   const int k = *k_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int64_t ldin = ldin_cnt ? compact_ld(*ldin_cnt) : ldin_arg;
+  const int64_t ldout = compact_out ? compact_ld(count) : ldout_arg;
+  if (nlist_next && p == 0 && blockIdx.y == 0 && threadIdx.x == 0) *nlist_next = (int32_t)count;
+  // narrow: every workgroup of the panel takes (row, column) pairs; wide: 256 columns each.
+  // Workgroup row 0 always runs: it writes the centroid rows, delta rows or not.
+  if (blockIdx.y != 0 && (count <= narrow ? (int64_t)blockIdx.y * kBlock >= (int64_t)G * count
+                                          : (int64_t)blockIdx.y * kBlock >= count))
+    return;
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
   {
     const float4* __restrict__ w4 = reinterpret_cast<const float4*>(w + (size_t)beg * kSlots);
@@ -1466,6 +1474,33 @@ __global__ __launch_bounds__(1024) void compact_kernel(
   if (end == n && t == 0) *count_out = placed;
 }
 
+// Non-empty delta rows of the compact post layout: workgroup i counts the set flags of ring
+// slot (j0 + i) % kPostRing over the list's *count_ptr positions into nerow[i] and clears
+// them, except in the last slot of the range, which the compaction that follows consumes.
+__global__ __launch_bounds__(1024) void ring_count_kernel(int32_t* __restrict__ ring,
+                                                         int64_t ring_ld, int32_t j0,
+                                                         int32_t nslots,
+                                                         const int32_t* __restrict__ count_ptr,
+                                                         int32_t* __restrict__ nerow) {
+  __shared__ int32_t s_sum;
+  const int i = blockIdx.x;
+  int32_t* __restrict__ f = ring + (int64_t)((j0 + i) % kPostRing) * ring_ld;
+  const bool clear = i + 1 < nslots;
+  const int n = *count_ptr;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  int c = 0;
+  for (int q = threadIdx.x; q < n; q += 1024) {
+    const int32_t v = f[q];
+    c += v != 0;
+    if (clear && v) f[q] = 0;
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+  if (lane_id() == 0 && c) atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (threadIdx.x == 0) nerow[i] = s_sum;
+}
+
 __global__ void fill_i32_kernel(int32_t* __restrict__ p, int32_t v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -1763,8 +1798,10 @@ __global__ __launch_bounds__(256) void assign_final_kernel(
 __global__ __launch_bounds__(256) void make_delta_kernel(
     const float* __restrict__ Y, int64_t ld, const int32_t* __restrict__ map,
     const float* __restrict__ C, int32_t kcap, const int32_t* __restrict__ m,
-    const int32_t* __restrict__ count_ptr, int32_t N, float* __restrict__ D, int64_t ldD) {
+    const int32_t* __restrict__ count_ptr, int32_t N, float* __restrict__ D, int64_t ldD_arg,
+    int32_t compact) {
   const int64_t count = *count_ptr;
+  const int64_t ldD = compact ? compact_ld(count) : ldD_arg;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int64_t col = map[j];
@@ -1801,10 +1838,12 @@ __global__ void recover_base_kernel(const int32_t* __restrict__ rec,
 
 // active(i) = any(Yhat_i + Yhat_m(i) > 0) over a 256-neuron slice per block row.
 __global__ __launch_bounds__(256) void recover_rows_kernel(
-    const float* __restrict__ D, int64_t ldD, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ m, const int32_t* __restrict__ count_ptr,
-    const float* __restrict__ C, int32_t kcap, int32_t N, int32_t* __restrict__ act) {
+    const float* __restrict__ D, int64_t ldD_arg, const int32_t* __restrict__ ldD_cnt,
+    const int32_t* __restrict__ map, const int32_t* __restrict__ m,
+    const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap, int32_t N,
+    int32_t* __restrict__ act) {
   const int64_t count = *count_ptr;
+  const int64_t ldD = ldD_cnt ? compact_ld(*ldD_cnt) : ldD_arg;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int64_t col = map[j];
@@ -1840,11 +1879,12 @@ __global__ __launch_bounds__(256) void expand_centroids_kernel(
 }
 
 __global__ __launch_bounds__(256) void scatter_delta_kernel(
-    const float* __restrict__ D, int64_t ldD, const int32_t* __restrict__ map,
-    const int32_t* __restrict__ m, const int32_t* __restrict__ orig,
-    const int32_t* __restrict__ count_ptr, const float* __restrict__ C, int32_t kcap, int32_t N,
-    float* __restrict__ Y, int64_t ld) {
+    const float* __restrict__ D, int64_t ldD_arg, const int32_t* __restrict__ ldD_cnt,
+    const int32_t* __restrict__ map, const int32_t* __restrict__ m,
+    const int32_t* __restrict__ orig, const int32_t* __restrict__ count_ptr,
+    const float* __restrict__ C, int32_t kcap, int32_t N, float* __restrict__ Y, int64_t ld) {
   const int64_t count = *count_ptr;
+  const int64_t ldD = ldD_cnt ? compact_ld(*ldD_cnt) : ldD_arg;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const int64_t col = map[j], dst = orig[j];
@@ -2086,11 +2126,11 @@ void launch_panel_centroid(const Net& net, int l, const LayerArrays& la, const f
 }
 
 void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
-                       const float* cin, float* cout,
-                       int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
-                       int64_t ldout, const int32_t* count, const int32_t* map,
-                       const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
-                       const LaunchShape& shape, hipStream_t s) {
+                       const float* cin, float* cout, int32_t kcap, const int32_t* k,
+                       const float* din, int64_t ldin, const int32_t* ldin_cnt, float* dout,
+                       int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
+                       const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
+                       int64_t max_cols, const LaunchShape& shape, hipStream_t s) {
   SNICIT_REQUIRE(net.mfma[l] == 0, "post kernel needs row-major weights");
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
@@ -2104,11 +2144,13 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
   if (lossy)
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
-                       ldin, dout, ldout, count, map, mcent, flags, eps_post, shape.narrow);
+                       ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
+                       mcent, flags, eps_post, shape.narrow);
   else
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
-                       ldin, dout, ldout, count, map, mcent, flags, eps_post, shape.narrow);
+                       ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
+                       mcent, flags, eps_post, shape.narrow);
   check_launch();
 }
 
@@ -2399,6 +2441,13 @@ void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out,
   check_launch();
 }
 
+void launch_ring_count(int32_t* ring, int64_t ring_ld, int32_t j0, int32_t nslots,
+                       const int32_t* count, int32_t* nerow, hipStream_t s) {
+  SNICIT_REQUIRE(nslots >= 1 && nslots <= kPostRing, "ring range");
+  hipLaunchKernelGGL(ring_count_kernel, dim3((unsigned)nslots), dim3(1024), 0, s, ring, ring_ld,
+                     j0, nslots, count, nerow);
+  check_launch();
+}
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_i32_kernel, dim3(grid1(n, 256, 4096)), dim3(256), 0, s, p, v, n);
   check_launch();
@@ -2511,10 +2560,10 @@ void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const S
 }
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
                        int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
-                       int32_t N, float* D, int64_t ldD, hipStream_t s) {
+                       int32_t N, float* D, int64_t ldD, bool compact, hipStream_t s) {
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
   hipLaunchKernelGGL(make_delta_kernel, grid, dim3(256), 0, s, Y, ld, map, C, kcap, m, count, N,
-                     D, ldD);
+                     D, ldD, compact ? 1 : 0);
   check_launch();
 }
 void launch_centroid_cats(const float* C, int32_t kcap, const int32_t* k, int32_t N,
@@ -2529,13 +2578,13 @@ void launch_recover_base(const int32_t* rec, const int32_t* B, int64_t max_cols,
                      B, catc, cat);
   check_launch();
 }
-void launch_recover_rows(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
-                         const int32_t* orig, const int32_t* count, int64_t max_cols,
-                         const float* C, int32_t kcap, int32_t N, int32_t* act, int32_t* cat,
-                         hipStream_t s) {
+void launch_recover_rows(const float* D, int64_t ldD, const int32_t* ldD_cnt, const int32_t* map,
+                         const int32_t* m, const int32_t* orig, const int32_t* count,
+                         int64_t max_cols, const float* C, int32_t kcap, int32_t N, int32_t* act,
+                         int32_t* cat, hipStream_t s) {
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
-  hipLaunchKernelGGL(recover_rows_kernel, grid, dim3(256), 0, s, D, ldD, map, m, count, C, kcap,
-                     N, act);
+  hipLaunchKernelGGL(recover_rows_kernel, grid, dim3(256), 0, s, D, ldD, ldD_cnt, map, m, count,
+                     C, kcap, N, act);
   check_launch();
   hipLaunchKernelGGL(recover_write_kernel, dim3(grid1(max_cols, 256, 4096)), dim3(256), 0, s,
                      orig, count, act, cat);
@@ -2548,13 +2597,13 @@ void launch_expand_centroids(const int32_t* rec, const int32_t* B, int64_t max_c
   hipLaunchKernelGGL(expand_centroids_kernel, grid, dim3(256), 0, s, rec, B, C, kcap, N, Y, ld);
   check_launch();
 }
-void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
-                          const int32_t* orig, const int32_t* count, int64_t max_cols,
-                          const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
-                          hipStream_t s) {
+void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* ldD_cnt, const int32_t* map,
+                          const int32_t* m, const int32_t* orig, const int32_t* count,
+                          int64_t max_cols, const float* C, int32_t kcap, int32_t N, float* Y,
+                          int64_t ld, hipStream_t s) {
   dim3 grid(grid1(max_cols, 256, 1 << 30), (unsigned)((N + 63) / 64));
-  hipLaunchKernelGGL(scatter_delta_kernel, grid, dim3(256), 0, s, D, ldD, map, m, orig, count, C,
-                     kcap, N, Y, ld);
+  hipLaunchKernelGGL(scatter_delta_kernel, grid, dim3(256), 0, s, D, ldD, ldD_cnt, map, m, orig,
+                     count, C, kcap, N, Y, ld);
   check_launch();
 }
 

@@ -35,6 +35,16 @@ struct Segs {
 double tol_decimal(float e);
 float tol_cut(float e);
 
+// Row stride of a post-threshold delta buffer that holds c list positions (compact post
+// layout): neuron n of position j sits at n * compact_ld(c) + j, rows 128-byte aligned.
+// Against the plan's batch-capacity stride this keeps a layer's delta rows in one dense
+// region (65536 rows of 2 KB instead of rows 240 KB apart at N = 65536, 485 rows).
+__host__ __device__ inline int64_t compact_ld(int64_t c) { return ((c > 0 ? c : 1) + 31) / 32 * 32; }
+
+// Ring of per-layer non-empty flags of the compact post layout: post layer j writes slot
+// j % kPostRing, and a compaction at most every kPostRing layers counts and clears them.
+constexpr int kPostRing = 64;
+
 // Launch shape of the layer kernels (chosen per net, batch size and density by the autotuner):
 // grid_target = workgroups a layer launch aims for (panels x column tiles), narrow = column
 // count up to which the thread-per-(row, column) path runs.
@@ -74,13 +84,16 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
 // One post-threshold layer in one launch: centroid rows (cin -> cout, k columns of ld kcap)
 // and delta rows (din -> dout through map, centroid index mcent).  Needs row-major weights
 // (post_layer_fused); otherwise the centroid and delta epilogues of launch_panel_spmm run.
+// Compact post layout: ldin_cnt (nullable) gives din's stride compact_ld(*ldin_cnt),
+// compact_out writes dout with stride compact_ld(*count), and nlist_next (nullable) receives
+// *count for the next layer.
 bool post_layer_fused(const Net& net, int l);
 void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
-                       const float* cin, float* cout,
-                       int32_t kcap, const int32_t* k, const float* din, int64_t ldin, float* dout,
-                       int64_t ldout, const int32_t* count, const int32_t* map,
-                       const int32_t* mcent, int32_t* flags, float eps_post, int64_t max_cols,
-                       const LaunchShape& shape, hipStream_t s);
+                       const float* cin, float* cout, int32_t kcap, const int32_t* k,
+                       const float* din, int64_t ldin, const int32_t* ldin_cnt, float* dout,
+                       int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
+                       const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
+                       int64_t max_cols, const LaunchShape& shape, hipStream_t s);
 // Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
 void launch_panel_centroid(const Net& net, int l, const LayerArrays& la, const float* cin,
                            float* cout, float* zc, int32_t kcap, const int32_t* k,
@@ -149,6 +162,10 @@ void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out,
                     hipStream_t s);
 inline int64_t compact_tiles(int64_t max_cols) { return (max_cols + 4095) / 4096; }
 
+// nerow[i] = set flags of ring slot (j0 + i) % kPostRing over *count positions, i < nslots;
+// every slot but the last is cleared
+void launch_ring_count(int32_t* ring, int64_t ring_ld, int32_t j0, int32_t nslots,
+                       const int32_t* count, int32_t* nerow, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
@@ -197,28 +214,29 @@ void launch_assign_exact(const float* Y, int64_t ld, const int32_t* map, const S
                          int64_t max_cols, const float* C, int32_t kcap, const int32_t* cent_slot,
                          int32_t N, int32_t* dist, int32_t* flags, int32_t* mcar,
                          const int32_t* orig, int32_t* rec, hipStream_t s);
-// D[n*ldD + j] = Y[n*ld + map[j]] - C[n*kcap + m[j]]
+// D[n*ldD + j] = Y[n*ld + map[j]] - C[n*kcap + m[j]]; compact: ldD = compact_ld(*count)
 void launch_make_delta(const float* Y, int64_t ld, const int32_t* map, const float* C,
                        int32_t kcap, const int32_t* m, const int32_t* count, int64_t max_cols,
-                       int32_t N, float* D, int64_t ldD, hipStream_t s);
+                       int32_t N, float* D, int64_t ldD, bool compact, hipStream_t s);
 
 // ---- recovery (src/SDGC/SNICIT/kernel.hpp:197-218) ---------------------------------
 void launch_centroid_cats(const float* C, int32_t kcap, const int32_t* k, int32_t N,
                           int32_t* catc, hipStream_t s);
 void launch_recover_base(const int32_t* rec, const int32_t* B, int64_t max_cols,
                          const int32_t* catc, int32_t* cat, hipStream_t s);
-void launch_recover_rows(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
-                         const int32_t* orig, const int32_t* count, int64_t max_cols,
-                         const float* C, int32_t kcap, int32_t N, int32_t* act, int32_t* cat,
-                         hipStream_t s);
+// ldD_cnt (nullable): D has the compact post layout, stride compact_ld(*ldD_cnt)
+void launch_recover_rows(const float* D, int64_t ldD, const int32_t* ldD_cnt, const int32_t* map,
+                         const int32_t* m, const int32_t* orig, const int32_t* count,
+                         int64_t max_cols, const float* C, int32_t kcap, int32_t N, int32_t* act,
+                         int32_t* cat, hipStream_t s);
 // Yfull[n*ld + b] = rec[b] < 0 ? 0 : C[n*kcap + rec[b]]
 void launch_expand_centroids(const int32_t* rec, const int32_t* B, int64_t max_cols,
                              const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
                              hipStream_t s);
-// Y[n*ld + orig[j]] = D[n*ldD + map[j]] + C[n*kcap + m[j]]
-void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* map, const int32_t* m,
-                          const int32_t* orig, const int32_t* count, int64_t max_cols,
-                          const float* C, int32_t kcap, int32_t N, float* Y, int64_t ld,
-                          hipStream_t s);
+// Y[n*ld + orig[j]] = D[n*ldD + map[j]] + C[n*kcap + m[j]] (ldD_cnt as in recover_rows)
+void launch_scatter_delta(const float* D, int64_t ldD, const int32_t* ldD_cnt, const int32_t* map,
+                          const int32_t* m, const int32_t* orig, const int32_t* count,
+                          int64_t max_cols, const float* C, int32_t kcap, int32_t N, float* Y,
+                          int64_t ld, hipStream_t s);
 
 }  // namespace snicit

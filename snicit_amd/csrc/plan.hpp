@@ -40,6 +40,12 @@ struct snicit_plan {
   int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
   int32_t* d_tilecnt = nullptr; // [compact_tiles(cap)] per-tile counts of the compaction
   int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
+  // compact post layout (exact clustering, one segment, every post layer fused): the delta
+  // list keeps its positions between compactions, which run at the layers post_compacts_after
+  // names; d_nlist[l] = list positions layer l processes (l > t), d_ring = per-layer flags
+  bool compact_post = false;
+  int32_t* d_nlist = nullptr;   // [L + 2]
+  int32_t* d_ring = nullptr;    // [kPostRing][ld]
   int32_t* d_k = nullptr;  // centroid columns in use (k with one segment)
   float* d_F = nullptr;
   int32_t* d_cent_slot = nullptr;

@@ -174,7 +174,9 @@ def post_tolerance_case():
     at clustering is 1 at neuron 7.  The layer's delta is then exactly w for the rows that read
     neuron 7: w = 0.05f (just above 0.05, so kept by the reference's |v| > 0.05 test in
     double, src/beyond/SNICIT/kernel.hpp:161) on rows 0 and 2, and the float just below
-    0.05f (pruned) on row 1.  Returns (cols, vals, y0, kept_rows)."""
+    0.05f (pruned) on row 1.  Rows 3.. have bias 0.5 (equal for both samples, so their
+    deltas are 0), which makes the net one without dead-row skipping, as lossy clustering
+    needs.  Returns (cols, vals, bias [1, N], y0, kept_rows)."""
     N = 64
     cols = np.arange(N, dtype=np.int32).reshape(1, N, 1).copy()
     vals = np.zeros((1, N, 1), np.float32)
@@ -184,4 +186,6 @@ def post_tolerance_case():
         vals[0, o, 0] = w
     y0 = np.zeros((2, N), np.float32)
     y0[1, 7] = 1.0
-    return cols, vals, y0, [0, 2]
+    bias = np.full((1, N), 0.5, np.float32)
+    bias[0, :3] = 0.0
+    return cols, vals, bias, y0, [0, 2]

@@ -196,10 +196,10 @@ def test_real_net_a_formats_against_float64():
 def test_post_tolerance_is_the_decimal_value():
     """A post-layer delta of exactly 0.05f is kept (0.05f > 0.05 in double, as the reference
     compares) and the float below it is pruned."""
-    cols, vals, y0, kept = nets.post_tolerance_case()
-    r = orc.model_snicit_lossy(cols, vals, 0.0, 1.0, y0, 0)
+    cols, vals, bias, y0, kept = nets.post_tolerance_case()
+    r = orc.model_snicit_lossy(cols, vals, bias, 1.0, y0, 0)
     assert r["k"] == 1 and r["nerow_at_t"] == 1
-    expect = np.zeros(y0.shape[1], np.float32)
+    expect = np.where(np.arange(y0.shape[1]) >= 3, np.float32(0.5), np.float32(0))
+    np.testing.assert_array_equal(r["yL"][0], expect)  # the centroid
     expect[kept] = np.float32(0.05)
     np.testing.assert_array_equal(r["yL"][1], expect)
-    assert r["cat"][1] == 1 and r["cat"][0] == 0

@@ -230,11 +230,11 @@ def test_real_net_a_from_csr(ctx, mode, t):
 def test_post_tolerance_is_the_decimal_value(ctx):
     """A post-layer delta of exactly 0.05f is kept, as the reference's double comparison
     keeps it, and the float below it is pruned; equal to the oracle."""
-    cols, vals, y0, kept = nets.post_tolerance_case()
-    net = sa.Net.from_ell(ctx, cols, vals, 0.0, clip=1.0)
+    cols, vals, bias, y0, kept = nets.post_tolerance_case()
+    net = sa.Net.from_ell(ctx, cols, vals, bias, clip=1.0)
     cfg = sa.default_cfg("SNICIT", threshold=0, cluster=sa.CLUSTER_LOSSY)
     plan, st = _run(net, cfg, y0)
-    r = orc.model_snicit_lossy(cols, vals, 0.0, 1.0, y0, 0)
+    r = orc.model_snicit_lossy(cols, vals, bias, 1.0, y0, 0)
     assert st.centroids == r["k"] == 1
     y = plan.activations()
     np.testing.assert_array_equal(y, r["yL"])
