@@ -42,11 +42,43 @@ int kernel_choice_from_env() {
   throw Error(SNICIT_E_INVALID, std::string("SNICIT_KERNEL must be auto, valu or mfma, got ") + e);
 }
 
+// The layer with each panel's rows reordered so that rows with equal weights (bit for bit)
+// are adjacent, first occurrences in their original order; returns `out`.
+const HostLayerPanels& group_equal_rows(const HostLayerPanels& hp, HostLayerPanels& out) {
+  out.pcols = hp.pcols;
+  out.pbeg = hp.pbeg;
+  out.outs.resize(hp.outs.size());
+  out.w.resize(hp.w.size());
+  const int64_t P = (int64_t)hp.pbeg.size() - 1;
+  std::vector<int32_t> order;
+  for (int64_t p = 0; p < P; ++p) {
+    const int32_t b = hp.pbeg[p], G = hp.pbeg[p + 1] - b;
+    order.clear();
+    std::vector<bool> used(G, false);
+    for (int32_t g = 0; g < G; ++g) {
+      if (used[g]) continue;
+      for (int32_t h = g; h < G; ++h)
+        if (!used[h] && std::memcmp(&hp.w[(size_t)(b + h) * kSlots],
+                                    &hp.w[(size_t)(b + g) * kSlots], kSlots * 4) == 0) {
+          used[h] = true;
+          order.push_back(h);
+        }
+    }
+    for (int32_t i = 0; i < G; ++i) {
+      out.outs[b + i] = hp.outs[b + order[i]];
+      std::memcpy(&out.w[(size_t)(b + i) * kSlots], &hp.w[(size_t)(b + order[i]) * kSlots],
+                  kSlots * 4);
+    }
+  }
+  return out;
+}
+
 struct NetBuilder {
   Net* net;
   int kernel_choice = kernel_choice_from_env();
   std::vector<int32_t> pcols, pbeg, outs;
   std::vector<float> w;
+  std::vector<uint32_t> psame;
   bool weights_given = true;
   int64_t w_floats = 0;
   // wide-ELL layers
@@ -107,13 +139,27 @@ struct NetBuilder {
       erng.push_back((int32_t)((int64_t)eent.size() - stream0));
     }
   }
-  void add(const HostLayerPanels& hp) {
-    const int64_t P = (int64_t)hp.pbeg.size() - 1;
+  void add(const HostLayerPanels& hp_in) {
+    const int64_t P = (int64_t)hp_in.pbeg.size() - 1;
     net->ell_K.push_back(0);
     net->ell_off.push_back((int64_t)ecols.size());
     net->erng_off.push_back((int64_t)erng.size());
     net->eent_off.push_back((int64_t)eent.size());
     net->panel_off.push_back(net->panel_off.back() + P);
+    HostLayerPanels grouped;
+    const HostLayerPanels& hp = weights_given ? group_equal_rows(hp_in, grouped) : hp_in;
+    for (int64_t p = 0; p < P; ++p) {
+      const int32_t b = hp.pbeg[p], G = hp.pbeg[p + 1] - b;
+      uint32_t m = 0;
+      for (int32_t g = 1; g < G && g < 32; ++g) {
+        // generated nets hold one weight value everywhere
+        const bool same = !weights_given ||
+                          std::memcmp(&hp.w[(size_t)(b + g) * kSlots],
+                                      &hp.w[(size_t)(b + g - 1) * kSlots], kSlots * 4) == 0;
+        if (same) m |= 1u << g;
+      }
+      psame.push_back(m);
+    }
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
     outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());
@@ -145,6 +191,10 @@ void upload_net(Net* net, NetBuilder& nb, float fill_weight) {
   net->d_pbeg = dev_alloc<int32_t>((size_t)Ptot + net->L, &net->device_bytes);
   net->d_outs = dev_alloc<int32_t>((size_t)net->L * net->N, &net->device_bytes);
   net->d_w = dev_alloc<float>((size_t)nb.w_floats, &net->device_bytes);
+  net->d_psame = dev_alloc<uint32_t>(std::max<size_t>(1, nb.psame.size()), &net->device_bytes);
+  if (!nb.psame.empty())
+    SNICIT_HIP(hipMemcpy(net->d_psame, nb.psame.data(), nb.psame.size() * 4,
+                         hipMemcpyHostToDevice));
   SNICIT_HIP(hipMemcpy(net->d_pcols, nb.pcols.data(), nb.pcols.size() * 4, hipMemcpyHostToDevice));
   SNICIT_HIP(hipMemcpy(net->d_pbeg, nb.pbeg.data(), nb.pbeg.size() * 4, hipMemcpyHostToDevice));
   SNICIT_HIP(hipMemcpy(net->d_outs, nb.outs.data(), nb.outs.size() * 4, hipMemcpyHostToDevice));
@@ -205,6 +255,7 @@ void destroy_net(Net* net) {
   dev_free(net->d_pbeg);
   dev_free(net->d_outs);
   dev_free(net->d_w);
+  dev_free(net->d_psame);
   dev_free(net->d_bias);
 }
 
@@ -259,6 +310,7 @@ LayerArrays layer_arrays(const Net& net, int l, const char* slot) {
     a.pbeg = reinterpret_cast<const int32_t*>(at(b.pbeg));
     a.outs = reinterpret_cast<const int32_t*>(at(b.outs));
     a.w = reinterpret_cast<const float*>(at(b.w));
+    a.psame = reinterpret_cast<const uint32_t*>(at(b.psame));
     a.ecols = reinterpret_cast<const int32_t*>(at(b.ecols));
     a.evals = reinterpret_cast<const float*>(at(b.evals));
     a.erng = reinterpret_cast<const int32_t*>(at(b.erng));
@@ -275,6 +327,7 @@ LayerArrays layer_arrays(const Net& net, int l, const char* slot) {
     a.pbeg = net.d_pbeg + net.panel_off[l] + l;
     a.outs = net.d_outs + (size_t)l * net.N;
     a.w = net.d_w + net.w_off[l];
+    a.psame = net.d_psame + net.panel_off[l];
   }
   return a;
 }
@@ -1232,7 +1285,9 @@ int snicit_net_stream_weights(snicit_net* net, int32_t buffers) {
         at = align(at + N * 4);
         b.w = at;
         at = align(at + wf * 4);
-        freed += P * kSlots * 4 + (P + 1) * 4 + N * 4 + wf * 4;
+        b.psame = at;
+        at = align(at + P * 4);
+        freed += P * kSlots * 4 + (P + 1) * 4 + N * 4 + wf * 4 + P * 4;
       }
       bytes[l] = at;
     }
@@ -1260,6 +1315,8 @@ int snicit_net_stream_weights(snicit_net* net, int32_t buffers) {
           SNICIT_HIP(hipMemcpy(base + b.pbeg, a.pbeg, (P + 1) * 4, hipMemcpyDeviceToHost));
           SNICIT_HIP(hipMemcpy(base + b.outs, a.outs, N * 4, hipMemcpyDeviceToHost));
           SNICIT_HIP(hipMemcpy(base + b.w, a.w, wf * 4, hipMemcpyDeviceToHost));
+          if (P > 0)
+            SNICIT_HIP(hipMemcpy(base + b.psame, a.psame, P * 4, hipMemcpyDeviceToHost));
         }
       }
     } catch (...) {
@@ -1267,9 +1324,10 @@ int snicit_net_stream_weights(snicit_net* net, int32_t buffers) {
       throw;
     }
     for (void* q : {(void*)net->d_pcols, (void*)net->d_pbeg, (void*)net->d_outs,
-                    (void*)net->d_w, (void*)net->d_ecols, (void*)net->d_evals,
-                    (void*)net->d_erng, (void*)net->d_eent})
+                    (void*)net->d_w, (void*)net->d_psame, (void*)net->d_ecols,
+                    (void*)net->d_evals, (void*)net->d_erng, (void*)net->d_eent})
       dev_free(q);
+    net->d_psame = nullptr;
     net->d_pcols = net->d_pbeg = net->d_outs = net->d_ecols = net->d_erng = nullptr;
     net->d_w = net->d_evals = nullptr;
     net->d_eent = nullptr;

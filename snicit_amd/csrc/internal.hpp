@@ -72,6 +72,7 @@ struct LayerArrays {
   const int32_t* pbeg = nullptr;   //   [P + 1]
   const int32_t* outs = nullptr;   //   [N]
   const float* w = nullptr;        //   row-major or slot-major weights
+  const uint32_t* psame = nullptr; //   [P] bit g: row g has the weights of row g - 1
   const int32_t* ecols = nullptr;  // wide-row layers: [N][K]
   const float* evals = nullptr;    //   [N][K]
   const int32_t* erng = nullptr;   //   [N][chunks + 1]
@@ -80,8 +81,8 @@ struct LayerArrays {
 
 // Byte offsets of a layer's arrays inside its streamed block (-1: the layer has none).
 struct BlockLayout {
-  int64_t pcols = -1, pbeg = -1, outs = -1, w = -1, ecols = -1, evals = -1, erng = -1,
-          eent = -1;
+  int64_t pcols = -1, pbeg = -1, outs = -1, w = -1, psame = -1, ecols = -1, evals = -1,
+          erng = -1, eent = -1;
 };
 
 struct Ctx {
@@ -103,6 +104,11 @@ struct Net {
   int32_t* d_pbeg = nullptr;       // [Ptot + L] (P_l + 1 entries per layer)
   int32_t* d_outs = nullptr;       // [L][N]
   float* d_w = nullptr;            // per layer: row-major [N][kSlots] or slot-major [P][kSlots][32]
+  // [Ptot] duplicate rows per panel: bit g set when panel row g has exactly (bit for bit) the
+  // weights of row g - 1.  Rows of a panel share their inputs, so such a row's multiply-add
+  // chain gives the bits of the row before it; the packer orders each panel's rows so that
+  // rows with equal weights are adjacent.
+  uint32_t* d_psame = nullptr;
   std::vector<int64_t> w_off;      // [L] first float of each layer in d_w
   std::vector<uint8_t> mfma;       // [L] layer runs on the matrix cores (slot-major weights)
   // plain-epilogue launches of row-major panel layers run panel_plain_mfma_kernel

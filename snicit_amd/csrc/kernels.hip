@@ -325,7 +325,9 @@ __global__ __launch_bounds__(kBlock) void panel_centroid_kernel(
 // compact_ld of the positions they hold, read from the device counters, so a layer's delta
 // rows fill one dense region; nlist_next (nullable) receives the list length for the next
 // layer when no compaction runs in between.  The counters, the centroid count and the
-// panel bounds are loaded together before the early exit.
+// panel bounds are loaded together before the early exit.  psame (nullable) marks rows
+// whose weights equal the previous row's: their chain is the previous row's, bit for bit,
+// so the delta rows reuse it (SDGC panels: one chain for all 32 rows).
 template <bool LOSSY>
 __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
@@ -336,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     float* __restrict__ dout, int64_t ldout_arg, int32_t compact_out,
     const int32_t* __restrict__ count_ptr, int32_t* __restrict__ nlist_next,
     const int32_t* __restrict__ map, const int32_t* __restrict__ mcent,
-    int32_t* __restrict__ flags, float eps_post, int32_t narrow) {
+    int32_t* __restrict__ flags, float eps_post, int32_t narrow,
+    const uint32_t* __restrict__ psame) {
   constexpr int kC = kMaxCentroidsExact + 1;  // padded centroid stride in LDS
   __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
   __shared__ float x_s[kSlots * kC];
@@ -346,6 +349,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   const int64_t count = *count_ptr;
   const int k = *k_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const uint32_t same = psame ? psame[p] : 0u;
   const int64_t ldin = ldin_cnt ? compact_ld(*ldin_cnt) : ldin_arg;
   const int64_t ldout = compact_out ? compact_ld(count) : ldout_arg;
   if (nlist_next && p == 0 && blockIdx.y == 0 && threadIdx.x == 0) *nlist_next = (int32_t)count;
@@ -435,16 +439,19 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     }
     const bool wave_nz = __ballot(nz) != 0ull;
     bool flag = false;
+    float acc = 0.f;
     for (int jj = 0; jj < G; ++jj) {
-      float acc = 0.f;
-      if (wave_nz) {
+      if (!((same >> jj) & 1u)) {
+        acc = 0.f;
+        if (wave_nz) {
 #pragma unroll
-        for (int r4 = 0; r4 < kSlots / 4; ++r4) {
-          const float4 wv4 = w_s[jj * (kSlots / 4) + r4];
-          acc = __builtin_fmaf(wv4.x, x[4 * r4 + 0], acc);
-          acc = __builtin_fmaf(wv4.y, x[4 * r4 + 1], acc);
-          acc = __builtin_fmaf(wv4.z, x[4 * r4 + 2], acc);
-          acc = __builtin_fmaf(wv4.w, x[4 * r4 + 3], acc);
+          for (int r4 = 0; r4 < kSlots / 4; ++r4) {
+            const float4 wv4 = w_s[jj * (kSlots / 4) + r4];
+            acc = __builtin_fmaf(wv4.x, x[4 * r4 + 0], acc);
+            acc = __builtin_fmaf(wv4.y, x[4 * r4 + 1], acc);
+            acc = __builtin_fmaf(wv4.z, x[4 * r4 + 2], acc);
+            acc = __builtin_fmaf(wv4.w, x[4 * r4 + 3], acc);
+          }
         }
       }
       const float y = residue(acc, zc_s[jj * kC + mi]);
@@ -2145,12 +2152,12 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.narrow);
+                       mcent, flags, eps_post, shape.narrow, la.psame);
   else
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.narrow);
+                       mcent, flags, eps_post, shape.narrow, la.psame);
   check_launch();
 }
 

@@ -128,6 +128,31 @@ def test_snicit_sdgc_matches_oracle(ctx, t):
     np.testing.assert_array_equal(plan.activations(), r["yL"])
 
 
+@pytest.mark.parametrize("t", [2, 9])
+def test_panels_with_repeated_weight_rows(ctx, t):
+    """Panels whose rows repeat other rows' weights, not next to each other in row order
+    (rows 8g..8g+7 share inputs; rows 8g+5 and 8g+7 copy row 8g+1, row 8g+6 copies 8g+3, so
+    the packer groups them and the post kernel runs one chain per distinct row), with a
+    per-neuron bias that differs between the copies: exact SNICIT equals the oracle."""
+    N, L, B = 512, 24, 600
+    cols, vals = nets.random_ell(N, L, K=32, seed=21, share=8)
+    v = vals.reshape(L, N // 8, 8, 32)
+    v[:, :, 5] = v[:, :, 1]
+    v[:, :, 7] = v[:, :, 1]
+    v[:, :, 6] = v[:, :, 3]
+    bias = (np.random.default_rng(2).standard_normal((L, N)) * 0.02 - 0.03).astype(np.float32)
+    bias = np.minimum(bias, 0)
+    y0 = nets.random_inputs(N, B, seed=4, density=0.3)
+    y0[300:400] = y0[7]
+    net = sa.Net.from_ell(ctx, cols, vals, bias)
+    plan, st = _run(ctx, net, sa.default_cfg("SNICIT", threshold=t), y0)
+    r = orc.snicit_exact(cols, vals, bias, 32.0, y0, t)
+    assert r["nerow_at_t"] > 0 and r["nerow"][-1] > 0
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+    np.testing.assert_array_equal(plan.activations(), r["yL"])
+    np.testing.assert_array_equal(plan.categories(), r["cat"])
+
+
 @pytest.mark.parametrize("mode,t", [("BF", 0), ("SNICIT", 4)])
 def test_compaction_over_many_tiles(ctx, mode, t):
     """50000 columns (13 tiles of 4096): the live-column compaction runs as per-tile counts
