@@ -92,6 +92,7 @@ struct NetBuilder {
     net->panel_off.push_back(net->panel_off.back());
     pbeg.push_back(0);
     net->mfma.push_back(0);
+    net->dup_rows.push_back(0);
     net->w_off.push_back(w_floats);
     net->ell_K.push_back(Kp);
     net->ell_off.push_back((int64_t)ecols.size());
@@ -148,6 +149,7 @@ struct NetBuilder {
     net->panel_off.push_back(net->panel_off.back() + P);
     HostLayerPanels grouped;
     const HostLayerPanels& hp = weights_given ? group_equal_rows(hp_in, grouped) : hp_in;
+    bool all_dup = P > 0;
     for (int64_t p = 0; p < P; ++p) {
       const int32_t b = hp.pbeg[p], G = hp.pbeg[p + 1] - b;
       uint32_t m = 0;
@@ -159,7 +161,10 @@ struct NetBuilder {
         if (same) m |= 1u << g;
       }
       psame.push_back(m);
+      const uint32_t full = G >= 32 ? 0xFFFFFFFEu : ((1u << G) - 1u) & ~1u;
+      all_dup = all_dup && m == full;
     }
+    net->dup_rows.push_back(all_dup ? 1 : 0);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
     outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());

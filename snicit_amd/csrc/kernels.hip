@@ -127,10 +127,12 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
     const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
     int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
     const int32_t* __restrict__ map, int32_t* __restrict__ flags, float* __restrict__ zc,
-    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t narrow) {
+    int32_t kcap, const int32_t* __restrict__ mcent, float eps_post, int32_t narrow,
+    const uint32_t* __restrict__ psame) {
   const int p = blockIdx.x;
   const int64_t count = *count_ptr;
   const int beg = pbeg[p], end = pbeg[p + 1];
+  const uint32_t same = psame ? psame[p] : 0u;
   const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
   if (count <= narrow) {
     panel_pairs<EPI>(beg, end, pc, outs, w, bias_vec, bias_s, clip, yin, ldin, yout, ldout,
@@ -160,12 +162,16 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
     int32_t mi = 0;
     if (EPI == EPI_DELTA || EPI == EPI_DELTA_LOSSY) mi = valid ? mcent[pos] : 0;
     bool flag = false;
+    float acc = 0.f;
     for (int j = beg; j < end; ++j) {
-      const float* __restrict__ wj = w + (size_t)j * kSlots;
-      float acc = 0.f;
-      if (wave_nz) {
+      // a row repeating the previous row's weights has its chain (same inputs, same order)
+      if (!((same >> (j - beg)) & 1u)) {
+        const float* __restrict__ wj = w + (size_t)j * kSlots;
+        acc = 0.f;
+        if (wave_nz) {
 #pragma unroll
-        for (int r = 0; r < kSlots; ++r) acc = __builtin_fmaf(wj[r], x[r], acc);
+          for (int r = 0; r < kSlots; ++r) acc = __builtin_fmaf(wj[r], x[r], acc);
+        }
       }
       const int o = outs[j];
       const float b = bias_vec ? bias_vec[o] : bias_s;
@@ -2086,7 +2092,9 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
   const float* bv = net.bias_vec(l);
   const float bs = net.bias_scalar(l);
   const bool mfma = net.mfma[l] != 0;
-  if (!mfma && a.epilogue == EPI_PLAIN && net.plain_mfma) {
+  // panels whose rows all repeat one weight row need one chain per column: the vector
+  // kernel runs it; otherwise plain layers take the 32 chains on the matrix cores
+  if (!mfma && a.epilogue == EPI_PLAIN && net.plain_mfma && !net.dup_rows[l]) {
     hipLaunchKernelGGL(panel_plain_mfma_kernel, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map, a.flags,
                        a.shape.narrow);
@@ -2102,7 +2110,8 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
     else                                                                                       \
       hipLaunchKernelGGL(panel_spmm_kernel<E>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w, \
                          bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,     \
-                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post, a.shape.narrow);          \
+                         a.flags, a.zc, a.kcap, a.mcent, a.eps_post, a.shape.narrow,           \
+                         a.la.psame);                                                          \
   } while (0)
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_SPMM(EPI_PLAIN); break;
