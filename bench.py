@@ -1,9 +1,10 @@
 """Benchmark: SNICIT vs plain SpMM layer loop on one MI355X, edges/s and ms per batch.
 
 A step is one full forward (all L layers, clustering, recovery) over one batch of
-synthetic inputs already resident in HBM.  Default workload: the SDGC-like net
-N=16384, L=120, 32 nnz/row of 0.0625, bias -0.4, clip 32, batch 60000 binary blob
-images at ~15% density, threshold layer 30.
+synthetic inputs already resident in HBM.  Default workload: the largest one-GPU shape,
+the SDGC-like net N=65536, L=1920, 32 nnz/row of 0.0625, bias -0.45, clip 32, batch
+60000 binary blob images at ~15% density, threshold layer 30 (--config c2 for N=16384,
+L=120, bias -0.4).
 
 Prints diagnostics on stderr and ONE JSON line on stdout.
 """
@@ -49,19 +50,23 @@ def spmm_layer_bytes(N, P, rows):
     return 8.0 * N * rows + weights
 
 
-def measured_traffic(wl):
-    """HBM bytes per pre-threshold launch from the committed counter summary of this
-    workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by a known-size copy), or None."""
-    path = os.path.join(ROOT, "profiles", "panel_plain_mfma_c2_traffic.json")
-    try:
-        rec = json.load(open(path))
-    except (OSError, ValueError):
-        return None
-    w = rec.get("workload", {})
-    if (w.get("N"), w.get("L"), w.get("batch"), w.get("threshold")) != (
-            wl["N"], wl["L"], wl["B"], wl["t"]):
-        return None
-    return rec.get("pre_threshold_traffic_bytes_per_launch")
+def measured_traffic(wl, kernel):
+    """HBM bytes per launch of `kernel` on this workload from a committed counter summary
+    (profiles/*_traffic.json: FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+    counter reads for a copy of known size), or None when none matches."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
+        try:
+            rec = json.load(open(path))
+        except (OSError, ValueError):
+            continue
+        w = rec.get("workload", {})
+        if rec.get("kernel") != kernel or (w.get("N"), w.get("L"), w.get("batch"),
+                                           w.get("threshold")) != (wl["N"], wl["L"], wl["B"],
+                                                                   wl["t"]):
+            continue
+        return rec.get("traffic_bytes_per_launch")
+    return None
 
 
 def time_steps(plan, steps, warmup):
@@ -79,7 +84,7 @@ def cpu_baseline(wl, gpu_cat, target_s):
     import snicit_amd as sa
 
     N, L = wl["N"], wl["L"]
-    cores = min(16, os.cpu_count() or 1)
+    cores = os.cpu_count() or 1
     cols = sa.gen_sdgc_cols(N, L, NET_SEED)
     vals = np.full(cols.shape, 0.0625, np.float32)
     # grow the sample (first n samples of the batch) until the timed loop takes about
@@ -95,8 +100,9 @@ def cpu_baseline(wl, gpu_cat, target_s):
     edges = float(n) * N * 32 * L
     match = bool(np.array_equal(cat, gpu_cat[:n]))
     return {"value": edges / dt, "unit": "edges/s", "cores": cores, "kind": "port",
-            "sample": f"first {n} of the {wl['B']} samples x all {L} layers "
-                      f"(plain FP32 forward with dead-row skipping, OpenMP); {dt:.2f} s",
+            "sample": f"first {n} of the {wl['B']} samples x all {L} layers (plain FP32 "
+                      f"forward with dead-row skipping, OpenMP, {cores} threads) in {dt:.2f} s; "
+                      f"edges/s extrapolated to the full batch at that rate",
             "seconds": dt, "categories_match_gpu": match, "cpu_model": cpu_model(),
             "host_cpus": os.cpu_count()}
 
@@ -424,7 +430,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--bf-steps", type=int, default=5,
+                    help="timed steps of the plain-BF comparison (at most --steps)")
     ap.add_argument("--threshold", type=int, default=None)
     ap.add_argument("--dense", action="store_true", help="also time the plain-dense form")
     ap.add_argument("--no-cpu", action="store_true")
@@ -522,7 +530,8 @@ def main():
 
     bf = sa.Plan(net, sa.default_cfg("BF"), B)
     bf.set_input_images(IMG_SEED, 0, B)
-    ms_b, ev_b, st_b = time_steps(bf, args.steps, args.warmup)
+    ms_b, ev_b, st_b = time_steps(bf, max(1, min(args.steps, args.bf_steps)),
+                                  min(args.warmup, 1))
     cat_b = bf.categories()
     same = bool(np.array_equal(cat_s, cat_b))
     log(f"plain BF: {ms_b:.3f} ms/step (events {ev_b:.3f}); categories identical: {same}")
@@ -536,29 +545,42 @@ def main():
         log(f"plain dense: {ms_d:.3f} ms/step")
         del dn
 
-    # live per-launch timing of the dominant kernel (plain-epilogue panel SpMM before t)
+    # Live per-launch timing (HIP events on the plan's stream around every layer launch) of
+    # the two layer kernels: the plain panel layer before t and the fused post layer.  The
+    # one with the larger total is the roofline kernel of this workload.
     ms_l, rows_l, cms_l = snic.profile()
-    pre = list(range(t))
-    b_pre = sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre)
-    t_pre = sum(ms_l[l] for l in pre) / 1000.0
-    n_pre = max(1, len(pre))
-    roof = {"kernel": "panel_plain_mfma_kernel (layers < t)", "bound": "hbm",
-            "achieved": round(b_pre / t_pre / 1e9, 1) if t_pre > 0 else None,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(b_pre / t_pre / 1e9 / HBM_PEAK_GBS, 4) if t_pre > 0 else None,
-            "traffic": measured_traffic(wl),
-            "bytes_per_launch": b_pre / n_pre, "avg_launch_ms": 1000.0 * t_pre / n_pre}
-    post = list(range(t, L))
     k = st_s.centroids
-    b_post = sum(spmm_layer_bytes(N, P, rows_l[l]) + 8.0 * N * k + 4.0 * N * k for l in post)
-    t_post = sum(ms_l[l] + cms_l[l] for l in post) / 1000.0
-    delta_roof = {"kernel": "panel_post_kernel (centroid and delta rows, layers >= t)",
-                  "achieved": round(b_post / t_post / 1e9, 1) if t_post > 0 else None,
-                  "frac": round(b_post / t_post / 1e9 / HBM_PEAK_GBS, 4) if t_post > 0 else None,
-                  "bytes_per_layer": b_post / max(1, len(post)),
-                  "avg_layer_ms": 1000.0 * t_post / max(1, len(post))}
-    log(f"roofline pre-t SpMM: {roof['achieved']} GB/s ({roof['frac']}) avg "
-        f"{roof['avg_launch_ms']:.3f} ms/launch; post-t: {delta_roof['achieved']} GB/s")
+    plain_name = ("panel_spmm_kernel (plain epilogue, one chain per distinct panel row; "
+                  "layers < t)")
+    post_name = "panel_post_kernel (centroid and delta rows, layers >= t)"
+    pre, post = list(range(t)), list(range(t, L))
+    kern = {
+        plain_name: {"bytes": sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre),
+                     "s": sum(ms_l[l] for l in pre) / 1000.0, "launches": len(pre),
+                     "per_unit": "8*N*rows + panel weights (4*N*32 + 4*P*32 + 4*(P+1) + 4*N)"},
+        post_name: {"bytes": sum(spmm_layer_bytes(N, P, rows_l[l]) + 8.0 * N * k for l in post),
+                    "s": sum(ms_l[l] + cms_l[l] for l in post) / 1000.0, "launches": len(post),
+                    "per_unit": "8*N*rows + 8*N*k (centroid inputs and outputs) + panel "
+                                "weights"},
+    }
+    dom = max(kern, key=lambda n: kern[n]["s"])
+
+    def roof(name):
+        d = kern[name]
+        gbs = d["bytes"] / d["s"] / 1e9 if d["s"] > 0 else None
+        return {"kernel": name, "bound": "hbm",
+                "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
+                "traffic": measured_traffic(wl, name.split(" ")[0]),
+                "bytes_per_launch": d["bytes"] / max(1, d["launches"]),
+                "avg_launch_ms": 1000.0 * d["s"] / max(1, d["launches"]),
+                "launches": d["launches"], "algorithmic_bytes": d["per_unit"]}
+
+    roofs = {n: roof(n) for n in kern}
+    for n, r in roofs.items():
+        log(f"roofline {n.split(' ')[0]}: {r['achieved']} GB/s ({r['frac']}), avg "
+            f"{r['avg_launch_ms']:.4f} ms over {r['launches']} launches, "
+            f"{kern[n]['s'] * 1000:.1f} ms in total")
 
     edges = float(B) * N * 32 * L
     cpu = None
@@ -593,8 +615,8 @@ def main():
         "snicit": {"live_at_threshold": st_s.live_at_threshold, "centroids": k,
                    "nerow_first": st_s.nerow_first, "nerow_last": st_s.nerow_last,
                    "active": st_s.active},
-        "roofline": roof,
-        "delta_roofline": delta_roof,
+        "roofline": roofs[dom],
+        "other_kernel_roofline": roofs[plain_name if dom == post_name else post_name],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
