@@ -206,6 +206,20 @@ int snicit_plan_profile(snicit_plan* plan, double* ms, int64_t* rows, double* ce
 int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
                  int64_t batch, int32_t* cat, snicit_stats* stats);
 
+/* snicit_infer without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops
+ * and harnesses that forbid hipMalloc).  Every device buffer of the run (activations,
+ * lists, counters, staging) is carved from one static device array of 64 MiB per GPU, held
+ * until the process ends; runs that need more return SNICIT_E_NOMEM before any kernel
+ * launches (64 MiB holds, e.g., N = 1024 with batch <= ~3500, or N = 4096 with batch <= ~900).
+ * It calls no hipMalloc or hipFree, captures no graphs and records no events (stage times
+ * in stats are 0); nets with streamed weights are refused.  The array is shared by every
+ * call on that GPU and keeps the last call's contents; the library clears what must start
+ * at zero.  Calls from threads of this process run one at a time per GPU; calls that
+ * overlap on one GPU in any other way race. */
+int snicit_infer_noalloc(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
+                         int64_t batch, int32_t* cat, snicit_stats* stats);
+
 /* ---- autotuning of the layer-kernel launch shape ----------------------------------
  * The reference derives its slab width from the device's shared memory per block
  * (utility.hpp:17-41) and its XY layout from an offline inspector (src/SDGC/inspector/).

@@ -153,6 +153,8 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_double), ctypes.c_int64]),
     "snicit_infer": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
                                     ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
+    "snicit_infer_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
+                                            ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
     "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32]),
@@ -625,7 +627,9 @@ class _Method:
         self.cfg = default_cfg(self.mode, threshold, **cfg)
         self.stats = None
 
-    def infer(self, inputs, batch=None):
+    def infer(self, inputs, batch=None, noalloc=False):
+        """Categories of every input, in rounds of `batch`.  noalloc=True runs
+        snicit_infer_noalloc: every device buffer from the library's static scratch."""
         inputs = np.ascontiguousarray(inputs, dtype=np.float32)
         B = inputs.shape[0]
         batch = B if batch is None else int(batch)
@@ -633,8 +637,9 @@ class _Method:
             raise ValueError("batch must divide the number of inputs")
         cat = np.empty(B, dtype=np.int32)
         st = Stats()
-        _check(lib().snicit_infer(self.net.h, ctypes.byref(self.cfg), _f32(inputs), B, batch,
-                                  _i32(cat), ctypes.byref(st)))
+        fn = lib().snicit_infer_noalloc if noalloc else lib().snicit_infer
+        _check(fn(self.net.h, ctypes.byref(self.cfg), _f32(inputs), B, batch, _i32(cat),
+                  ctypes.byref(st)))
         self.stats = st
         return cat
 

@@ -12,7 +12,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 
 #include "plan.hpp"
 
@@ -271,6 +273,38 @@ void destroy_net(Net* net) {
 namespace snicit {
 
 thread_local std::string g_last_error;
+thread_local DevArena* g_dev_arena = nullptr;
+
+// ---- static scratch of the allocation-free entry point -------------------------------------
+// snicit_infer_noalloc carves every device buffer of its run from this array instead of
+// calling hipMalloc.  Each GPU the library runs on holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device memory: the device
+// address is resolved once per device with hipGetSymbolAddress (at context creation, with
+// that device current) and only that address is used.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+
+struct ScratchSlot {
+  char* ptr = nullptr;
+  std::mutex run;  // one allocation-free run at a time per device within this process
+};
+std::mutex g_scratch_mu;
+std::map<int, std::unique_ptr<ScratchSlot>> g_scratch;
+
+// Resolves the device address of `scratch` on the current device (once per device).
+ScratchSlot* scratch_slot(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto& slot = g_scratch[device];
+  if (!slot) {
+    auto s = std::make_unique<ScratchSlot>();
+    void* ptr = nullptr;
+    SNICIT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SNICIT_REQUIRE(ptr != nullptr, "no device address for the static scratch");
+    s->ptr = static_cast<char*>(ptr);
+    slot = std::move(s);
+  }
+  return slot.get();
+}
 
 bool plain_mode(int32_t mode) { return mode != SNICIT_MODE_SNICIT; }
 bool compacting(const snicit_plan* p) {
@@ -908,29 +942,37 @@ void plan_run(snicit_plan* p, int32_t stop, snicit_stats* stats) {
   }
   // the recovery stage reads where the post stage leaves its state when it is enqueued
   plan_bookkeeping(p, stop);
-  SNICIT_HIP(hipEventRecord(p->ev[0], s));
+  const bool timed = p->ev[0] != nullptr;
+  auto mark = [&](int i) {
+    if (timed) SNICIT_HIP(hipEventRecord(p->ev[i], s));
+  };
+  mark(0);
   if (plain_mode(p->cfg.mode)) {
     const int l_end = stop < 0 ? L : stop;
     run_stage(p, 0, [&] {
       const int c = enqueue_plain_layers(p, l_end, true);
       enqueue_plain_categories(p, l_end, c);
     });
-    for (int i = 1; i < 5; ++i) SNICIT_HIP(hipEventRecord(p->ev[i], s));
+    for (int i = 1; i < 5; ++i) mark(i);
   } else {
     run_stage(p, 0, [&] { enqueue_plain_layers(p, p->cfg.threshold, false); });
-    SNICIT_HIP(hipEventRecord(p->ev[1], s));
+    mark(1);
     run_stage(p, 1, [&] { enqueue_cluster(p); });
-    SNICIT_HIP(hipEventRecord(p->ev[2], s));
+    mark(2);
     run_stage(p, 2, [&] { enqueue_post(p); });
-    SNICIT_HIP(hipEventRecord(p->ev[3], s));
+    mark(3);
     run_stage(p, 3, [&] { enqueue_recover(p); });
-    SNICIT_HIP(hipEventRecord(p->ev[4], s));
+    mark(4);
   }
-  SNICIT_HIP(hipEventSynchronize(p->ev[4]));
+  if (timed) SNICIT_HIP(hipEventSynchronize(p->ev[4]));
+  else SNICIT_HIP(hipStreamSynchronize(s));
   p->have_run = true;
   if (stats) {
     float ms[4];
-    for (int i = 0; i < 4; ++i) SNICIT_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
+    for (int i = 0; i < 4; ++i) {
+      ms[i] = 0.f;
+      if (timed) SNICIT_HIP(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
+    }
     plan_stats(p, stats, ms);
   }
 }
@@ -1048,6 +1090,7 @@ int snicit_ctx_create(int device, snicit_ctx** out) {
     hipDeviceProp_t prop;
     SNICIT_HIP(hipGetDeviceProperties(&prop, device));
     c->num_cus = prop.multiProcessorCount;
+    (void)scratch_slot(device);  // the static scratch's address on this device
     *out = c.release();
   });
 }
@@ -1472,7 +1515,8 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         SNICIT_HIP(hipMemset(p->d_nlist, 0, (L + 2) * 4));
         SNICIT_HIP(hipMemset(p->d_ring, 0, (size_t)kPostRing * p->ld * 4));
       }
-      for (auto& e : p->ev) SNICIT_HIP(hipEventCreate(&e));
+      if (!g_dev_arena)  // the allocation-free entry point runs untimed
+        for (auto& e : p->ev) SNICIT_HIP(hipEventCreate(&e));
       if (net->streamed()) {
         const int nb = net->stream_buffers;
         p->slot_layer.assign(nb, -1);
@@ -1881,6 +1925,57 @@ int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, in
       acc.ms_cluster += st.ms_cluster;
       acc.ms_post += st.ms_post;
       acc.ms_recover += st.ms_recover;
+      acc.edges += st.edges;
+      acc.batch += st.batch;
+      acc.active += st.active;
+      acc.rows_processed += st.rows_processed;
+      acc.layers = st.layers;
+      acc.centroids = std::max(acc.centroids, st.centroids);
+      acc.live_at_threshold += st.live_at_threshold;
+      acc.nerow_first += st.nerow_first;
+      acc.nerow_last += st.nerow_last;
+    }
+    if (stats) *stats = acc;
+  });
+}
+
+// The one-call form with every device buffer taken from the static scratch: no hipMalloc or
+// hipFree, no graph capture, no events (the stage times in stats are 0).
+int snicit_infer_noalloc(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
+                         int64_t batch, int32_t* cat, snicit_stats* stats) {
+  return guarded([&] {
+    SNICIT_REQUIRE(net && cfg && (y0 || B == 0) && (cat || B == 0), "null argument");
+    SNICIT_REQUIRE(batch > 0, "batch must be positive");
+    SNICIT_REQUIRE(B % batch == 0, "batch must divide the number of inputs");
+    SNICIT_REQUIRE(!net->streamed(),
+                   "nets with streamed weights need copy streams and events; use snicit_infer");
+    SNICIT_HIP(hipSetDevice(net->ctx->device));
+    ScratchSlot* slot = scratch_slot(net->ctx->device);
+    std::lock_guard<std::mutex> one_run(slot->run);
+    DevArena arena;
+    arena.base = slot->ptr;
+    arena.cap = kScratchBytes;
+    snicit_run_cfg c = *cfg;
+    c.use_graph = 0;
+    struct ArenaScope {
+      explicit ArenaScope(DevArena* a) { g_dev_arena = a; }
+      ~ArenaScope() { g_dev_arena = nullptr; }
+    } scope(&arena);
+    snicit_plan* p = nullptr;
+    int rc = snicit_plan_create(net, &c, batch, &p);
+    if (rc) throw Error(rc, g_last_error);
+    // destroyed while the arena is still set: its buffers are arena memory
+    std::unique_ptr<snicit_plan, int (*)(snicit_plan*)> hold(p, snicit_plan_destroy);
+    snicit_stats acc{};
+    const int64_t width = net->Nin > 0 ? net->Nin : net->N;
+    for (int64_t off = 0; off < B; off += batch) {
+      rc = snicit_plan_set_input(p, y0 + (size_t)off * width, batch);
+      if (rc) throw Error(rc, g_last_error);
+      snicit_stats st{};
+      rc = snicit_plan_run(p, &st);
+      if (rc) throw Error(rc, g_last_error);
+      rc = snicit_plan_categories(p, cat + off);
+      if (rc) throw Error(rc, g_last_error);
       acc.edges += st.edges;
       acc.batch += st.batch;
       acc.active += st.active;

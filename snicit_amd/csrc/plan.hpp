@@ -117,10 +117,28 @@ inline int guarded(F&& f) {
   }
 }
 
+// Device memory of the allocation-free entry point: while g_dev_arena is set on the calling
+// thread, dev_alloc carves buffers from [base, base + cap) (256-byte aligned, never freed
+// one by one) and dev_free leaves them alone.
+struct DevArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+extern thread_local DevArena* g_dev_arena;
+
 template <class T>
 inline T* dev_alloc(size_t n, int64_t* acct = nullptr) {
   void* p = nullptr;
   if (n == 0) n = 1;
+  if (DevArena* a = g_dev_arena) {
+    const size_t off = (a->used + 255) / 256 * 256, bytes = n * sizeof(T);
+    if (off > a->cap || bytes > a->cap - off)
+      throw Error(SNICIT_E_NOMEM, "the run needs more device memory than the " +
+                                      std::to_string(a->cap >> 20) + " MiB static scratch");
+    a->used = off + bytes;
+    if (acct) *acct += (int64_t)bytes;
+    return reinterpret_cast<T*>(a->base + off);
+  }
   hipError_t e = hipMalloc(&p, n * sizeof(T));
   if (e != hipSuccess)
     throw Error(e == hipErrorOutOfMemory ? SNICIT_E_NOMEM : SNICIT_E_HIP,
@@ -131,6 +149,10 @@ inline T* dev_alloc(size_t n, int64_t* acct = nullptr) {
 }
 
 inline void dev_free(void* p) {
+  if (DevArena* a = g_dev_arena) {
+    const char* q = static_cast<const char*>(p);
+    if (q >= a->base && q < a->base + a->cap) return;  // arena memory is never freed
+  }
   if (p) (void)hipFree(p);
 }
 
