@@ -28,8 +28,8 @@ WORKLOADS = {
                A=dict(N=4096, L=480, bias=-0.35, t=30),
                B=dict(Nin=784, N=1024, L=24, K=103, Nout=10, clip=1.0, t=8)),
     # Poisson request stream against two resident nets
-    "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=1000,
-               slots=2, window_ms=2.0, group_samples=0, pinned_mb=1024,
+    "c5": dict(smin=1, smax=4096, loads=(0.3, 0.7, 1.0), burst=160, requests=2000,
+               slots=2, window_ms=2.0, group_samples=0, pinned_mb=1024, round_ms=20.0,
                A=dict(N=16384, L=120, bias=-0.4, t=30),
                B=dict(Nin=3072, N=2048, L=16, K=307, Nout=10, clip=1.0, t=8)),
 }
@@ -325,12 +325,19 @@ def bench_serve(args, wl):
     del cols, vals
     cfgs = [sa.default_cfg("SNICIT", threshold=A["t"]),
             sa.default_cfg("SNICIT", threshold=Bn["t"], cluster=sa.CLUSTER_LOSSY)]
-    t0 = time.perf_counter()
-    srv = sa.Server(ctx, [net_a, net_b], cfgs, slot_samples=8192, slot_segments=64,
-                    max_slots_per_net=wl["slots"], group_window_ms=wl["window_ms"],
-                    group_samples=wl["group_samples"], hbm_budget_bytes=64 << 30,
-                    pinned_bytes=wl["pinned_mb"] << 20)
-    log(f"server: {srv.stats()} ({time.perf_counter() - t0:.1f} s to allocate and capture)")
+    def make_server(round_ms):
+        t0 = time.perf_counter()
+        s = sa.Server(ctx, [net_a, net_b], cfgs, slot_samples=8192, slot_segments=64,
+                      max_slots_per_net=wl["slots"], group_window_ms=wl["window_ms"],
+                      group_samples=wl["group_samples"], round_ms=round_ms,
+                      hbm_budget_bytes=64 << 30, pinned_bytes=wl["pinned_mb"] << 20)
+        log(f"server (round_ms {round_ms}): {s.stats()} ({time.perf_counter() - t0:.1f} s to "
+            "allocate and capture)")
+        return s
+
+    # saturation is measured with the greedy launch rule (a group launches as soon as a slot
+    # of its net is free), which the fixed-period rule cannot exceed
+    srv = make_server(0.0)
     pool = prototype_features(Bn["Nin"], 8192, 99)
     edges_per = {0: A["N"] * 32 * A["L"],
                  1: Bn["N"] * Bn["K"] * Bn["L"] + Bn["N"] * Bn["Nin"] + Bn["Nout"] * Bn["N"]}
@@ -380,11 +387,18 @@ def bench_serve(args, wl):
     run(draw(16), None)  # warm-up
     lat, edges, wall = run(draw(wl["burst"]), None)
     sat_rps = wl["burst"] / wall
+    if args.sat_rps:
+        log(f"measured burst rate {sat_rps:.1f} req/s; loads taken from --sat-rps {args.sat_rps}")
+        sat_rps = args.sat_rps
     log(f"saturated: {wl['burst']} requests in {wall:.2f} s = {sat_rps:.1f} req/s, "
         f"{edges / wall:.3e} edges/s; burst latency p50 {np.percentile(lat, 50):.1f} ms")
     table = {"saturated_burst": {"requests_per_s": sat_rps, "edges_per_s": edges / wall,
                                  "p50_ms": float(np.percentile(lat, 50)),
                                  "p99_ms": float(np.percentile(lat, 99))}}
+    if wl["round_ms"] > 0:
+        srv.close()
+        srv = make_server(wl["round_ms"])
+        run(draw(16), None)  # warm-up
     for load in wl["loads"]:
         lat, edges, wall = run(draw(wl["requests"]), load * sat_rps)
         row = {"offered_requests_per_s": load * sat_rps, "edges_per_s": edges / wall,
@@ -414,9 +428,13 @@ def bench_serve(args, wl):
                                f"bias={A['bias']} t={A['t']}, exact) and net B ({Bn['Nin']}->"
                                f"{Bn['N']}x{Bn['L']}->{Bn['Nout']}, {Bn['K']} nnz/row, t={Bn['t']}, "
                                f"lossy); slots of 8192 samples / 64 requests, {wl['slots']} "
-                               f"per net, grouping window {wl['window_ms']} ms, groups close "
-                               f"at {wl['group_samples'] or 8192} samples, request inputs "
-                               f"in a {wl['pinned_mb']} MiB pinned arena",
+                               f"per net, "
+                               + (f"fixed grouping periods of {wl['round_ms']} ms (saturation "
+                                  "measured with the greedy launch rule)"
+                                  if wl["round_ms"] > 0 else
+                                  f"grouping window {wl['window_ms']} ms, groups close at "
+                                  f"{wl['group_samples'] or 8192} samples")
+                               + f", request inputs in a {wl['pinned_mb']} MiB pinned arena",
                    "server": st},
         "serving": table,
         "roofline": None,
@@ -447,6 +465,14 @@ def main():
                          "them through SLOTS device buffers (c1-c3)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="read, write and copy rates of 2 GiB device buffers (torch)")
+    ap.add_argument("--slots", type=int, default=None, help="c5: slots per net")
+    ap.add_argument("--window-ms", type=float, default=None, help="c5: grouping window")
+    ap.add_argument("--group-samples", type=int, default=None, help="c5: group cap")
+    ap.add_argument("--round-ms", type=float, default=None,
+                    help="c5: fixed grouping period (0: launch rule with window and fill)")
+    ap.add_argument("--sat-rps", type=float, default=None,
+                    help="c5: saturated request rate to take the loads from (default: measured "
+                         "by the burst of this run)")
     ap.add_argument("--short-rows", type=int, default=0, metavar="K",
                     help="panel grouping with and without, on a random net of K-input rows")
     args = ap.parse_args()
@@ -477,6 +503,10 @@ def main():
     if args.config == "c4":
         bench_batch(args, wl)
         return
+    for key, val in (("slots", args.slots), ("window_ms", args.window_ms),
+                     ("group_samples", args.group_samples), ("round_ms", args.round_ms)):
+        if val is not None and key in wl:
+            wl[key] = val
     if args.config == "c5":
         if args.pmc_pass:
             pass_serve_net_b(args, wl)

@@ -310,6 +310,10 @@ typedef struct snicit_server_cfg {
   int64_t pinned_bytes;      /* pinned host memory that holds request inputs from submit  */
                              /* until their group is done (0: none); inputs that do not   */
                              /* fit are kept in pageable memory and staged at launch      */
+  double round_ms;           /* > 0: fixed periods instead of the launch rule above: at   */
+                             /* each period boundary the requests queued by then become   */
+                             /* due, and every due request of a net launches in its next  */
+                             /* free slot (window and group-fill triggers are not used)   */
 } snicit_server_cfg;
 
 typedef struct snicit_request_info {

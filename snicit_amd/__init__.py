@@ -71,7 +71,7 @@ class ServerCfg(ctypes.Structure):
                 ("slot_segments", ctypes.c_int32), ("max_slots_per_net", ctypes.c_int32),
                 ("cluster_min", ctypes.c_int32), ("warm", ctypes.c_int32),
                 ("group_window_ms", ctypes.c_double), ("group_samples", ctypes.c_int64),
-                ("pinned_bytes", ctypes.c_int64)]
+                ("pinned_bytes", ctypes.c_int64), ("round_ms", ctypes.c_double)]
 
 
 class RequestInfo(ctypes.Structure):

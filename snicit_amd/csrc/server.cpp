@@ -81,6 +81,11 @@ struct snicit_server {
   std::mutex pin_mu;
   std::thread worker;
   int64_t requests_done = 0, samples_done = 0, groups = 0;
+  // fixed-period grouping (cfg.round_ms > 0): the next period boundary, and per net the
+  // last boundary passed while it had requests queued (requests submitted by then are due)
+  Clock::time_point next_round;
+  std::vector<Clock::time_point> due;
+  std::vector<uint8_t> has_due;
 };
 
 namespace {
@@ -227,14 +232,23 @@ int64_t group_cap(const snicit_server* S) {
   return g > 0 && g < S->cfg.slot_samples ? g : S->cfg.slot_samples;
 }
 
+bool rounds(const snicit_server* S) { return S->cfg.round_ms > 0.0; }
+
+// With fixed periods, whether the queue head of `net` was submitted by its last boundary.
+bool head_due(const snicit_server* S, int32_t net) {
+  const auto& q = S->queues[net];
+  return !q.empty() && S->has_due[net] && q.front()->t_submit <= S->due[net];
+}
+
 // Moves queued requests of `net` into the free slot while they fit (the first one always
-// does: every request fits a slot).
+// does: every request fits a slot); with fixed periods, only requests that are due.
 void fill_group(snicit_server* S, Slot* sl) {
   auto& q = S->queues[sl->net];
   const int64_t cap = group_cap(S);
   int64_t total = 0;
   sl->reqs.clear();
   while (!q.empty() && (int32_t)sl->reqs.size() < S->cfg.slot_segments &&
+         (!rounds(S) || head_due(S, sl->net)) &&
          (sl->reqs.empty() || total + q.front()->samples <= cap)) {
     Request* r = q.front();
     q.pop_front();
@@ -246,6 +260,7 @@ void fill_group(snicit_server* S, Slot* sl) {
 bool group_ready(snicit_server* S, int32_t net, Clock::time_point now) {
   auto& q = S->queues[net];
   if (q.empty()) return false;
+  if (rounds(S)) return head_due(S, net);
   bool net_busy = false;
   for (const auto& sl : S->slots)
     if (sl->net == net && sl->busy) net_busy = true;
@@ -293,6 +308,17 @@ void worker_main(snicit_server* S) {
   while (!S->stop) {
     bool progressed = false;
     const Clock::time_point now = Clock::now();
+    if (rounds(S) && now >= S->next_round) {
+      // a period boundary: every request queued by now is due for the next free slot
+      for (size_t n = 0; n < S->queues.size(); ++n)
+        if (!S->queues[n].empty()) {
+          S->due[n] = now;
+          S->has_due[n] = 1;
+        }
+      const auto period = std::chrono::duration_cast<Clock::duration>(
+          std::chrono::duration<double, std::milli>(S->cfg.round_ms));
+      while (S->next_round <= now) S->next_round += period;
+    }
     for (auto& slp : S->slots) {
       Slot* sl = slp.get();
       if (!sl->busy) continue;
@@ -434,7 +460,7 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
                    "slot_samples out of range");
     SNICIT_REQUIRE(cfg->slot_segments >= 1 && cfg->max_slots_per_net >= 1,
                    "slot_segments and max_slots_per_net must be >= 1");
-    SNICIT_REQUIRE(cfg->group_window_ms >= 0.0 && cfg->cluster_min >= 0 &&
+    SNICIT_REQUIRE(cfg->group_window_ms >= 0.0 && cfg->round_ms >= 0.0 && cfg->cluster_min >= 0 &&
                        cfg->group_samples >= 0,
                    "bad grouping policy");
     SNICIT_REQUIRE(cfg->pinned_bytes >= 0, "pinned_bytes must be >= 0");
@@ -448,6 +474,9 @@ int snicit_server_create(snicit_ctx* ctx, snicit_net* const* nets, const snicit_
       S->cfgs.push_back(cfgs[n]);
     }
     S->queues.resize(n_nets);
+    S->due.resize(n_nets);
+    S->has_due.assign(n_nets, 0);
+    S->next_round = Clock::now();
     try {
       if (cfg->pinned_bytes > 0) {
         const int64_t bytes = cfg->pinned_bytes / kPinAlign * kPinAlign;

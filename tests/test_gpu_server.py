@@ -26,7 +26,9 @@ def _alone(net, cfg, set_input, B):
     return plan.categories()
 
 
-def test_server_results_equal_alone(ctx):
+@pytest.mark.parametrize("round_ms", [0.0, 4.0])
+def test_server_results_equal_alone(ctx, round_ms):
+    """Greedy launch rule (round_ms 0) and fixed periods alike."""
     N0, L0 = 1024, 40
     net0 = sa.Net.generate_sdgc(ctx, N0, L0, 7, bias=-0.3)
     cfg0 = sa.default_cfg("SNICIT", threshold=8)
@@ -43,7 +45,7 @@ def test_server_results_equal_alone(ctx):
     cfg2 = sa.default_cfg("BF")
     srv = sa.Server(ctx, [net0, net1, net2], [cfg0, cfg1, cfg2], slot_samples=2048,
                     slot_segments=16, max_slots_per_net=2, group_window_ms=5.0,
-                    hbm_budget_bytes=8 << 30)
+                    hbm_budget_bytes=8 << 30, round_ms=round_ms)
     st = srv.stats()
     assert st["slots"] == 6
     r = np.random.default_rng(11)
