@@ -175,8 +175,14 @@ int main(int argc, char** argv) {
       }
     }
     std::vector<uint8_t> lb;
-    if (read_idx(dataset + "/CIFAR-10/cifar-label.bin", 0, lb) && (int64_t)lb.size() >= num_input)
-      labels.assign(lb.begin(), lb.begin() + num_input);
+    // CIFAR-10 binary records: one label byte then 3072 pixel bytes; the label of sample i is
+    // byte i * 3073 (src/beyond/SNICIT/SNICIT.hpp:474-477)
+    constexpr int64_t kCifarRecord = 3073;
+    if (read_idx(dataset + "/CIFAR-10/cifar-label.bin", 0, lb) && num_input > 0 &&
+        (int64_t)lb.size() >= (num_input - 1) * kCifarRecord + 1) {
+      labels.resize(num_input);
+      for (int64_t i = 0; i < num_input; ++i) labels[i] = lb[(size_t)(i * kCifarRecord)];
+    }
   } else {
     std::vector<uint8_t> img, lb;
     if (read_idx(dataset + "/MNIST/t10k-images-idx3-ubyte", 4, img) &&

@@ -58,11 +58,29 @@ def csr_inputs_to_ell(row_ptr, cols, vals, N):
     return ec, ev
 
 
+# The reference's beyond nets (main/beyond.cu:57-100): hidden size, hidden layers, density,
+# default threshold, and whether the inputs are CIFAR-10 features fed to the hidden layers.
+BEYOND_NETS = {"A": dict(N=128, L=18, density=0.6, t=8, cifar=False),
+               "B": dict(N=256, L=18, density=0.6, t=8, cifar=False),
+               "C": dict(N=256, L=12, density=0.5, t=6, cifar=False),
+               "D": dict(N=256, L=12, density=0.5, t=6, cifar=True)}
+
+
+def beyond_net(key):
+    """The reference's beyond net `key` from the committed fixture (tests/golden/
+    beyond_net{key}.npz, written from its TSV files by make_beyond_fixture.py): row_ptr
+    (L, N+1) absolute offsets of a CSR over input neurons, cols (int32), vals, bias (L, N),
+    w_out (N, 10) input-major, b_out; net A also w_in (784, N) input-major and b_in."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", f"beyond_net{key}.npz"))
+    out = {k: d[k] for k in d.files}
+    out["cols"] = out["cols"].astype(np.int32)
+    return out
+
+
 def beyond_net_a():
     """The reference's beyond net A (784 -> 128 x 18 -> 10), from the committed fixture."""
-    import os
-    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "beyond_netA.npz"))
-    return {k: d[k] for k in d.files}
+    return beyond_net("A")
 
 
 def prototype_features(Nin, B, seed):

@@ -203,3 +203,38 @@ def test_post_tolerance_is_the_decimal_value():
     np.testing.assert_array_equal(r["yL"][0], expect)  # the centroid
     expect[kept] = np.float32(0.05)
     np.testing.assert_array_equal(r["yL"][1], expect)
+
+
+@pytest.mark.parametrize("key", ["A", "B", "C", "D"])
+def test_reference_beyond_fixtures(key):
+    """The fixtures of the reference's beyond nets have the shapes and densities of its net
+    table (main/beyond.cu:57-100), and the oracle's forward over them (hidden layers as CSR
+    over inputs, dense output layer, argmax) agrees with a float64 dense numpy forward."""
+    spec = nets.BEYOND_NETS[key]
+    a = nets.beyond_net(key)
+    N, L = spec["N"], spec["L"]
+    rp = a["row_ptr"]
+    assert rp.shape == (L, N + 1) and a["bias"].shape == (L, N) and a["w_out"].shape == (N, 10)
+    assert (np.diff(rp, axis=1) >= 0).all() and rp[0, 0] == 0
+    assert (rp[1:, 0] == rp[:-1, -1]).all() and rp[-1, -1] == len(a["vals"])
+    dens = (rp[:, -1] - rp[:, 0]) / (N * N)
+    assert abs(dens.mean() - spec["density"]) < 1e-3
+    assert ("w_in" in a) == (key == "A") and ("b_in" in a) == (not spec["cifar"])
+    rng = np.random.default_rng(7)
+    x = ((rng.random((40, N)) < 0.3) * rng.random((40, N))).astype(np.float32)
+    cols, vals = nets.csr_inputs_to_ell(rp, a["cols"], a["vals"], N)
+    yL, lg, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, w_out=a["w_out"].T.copy(),
+                                  b_out=a["b_out"])
+    y = x.astype(np.float64)
+    for h in range(L):
+        W = np.zeros((N, N))  # W[i, o]: input i feeds output o
+        for i in range(N):
+            for e in range(rp[h, i], rp[h, i + 1]):
+                W[i, a["cols"][e]] += a["vals"][e]
+        y = np.clip(y @ W + a["bias"][h], 0, 1)
+    logits = y @ a["w_out"].astype(np.float64) + a["b_out"]
+    assert np.abs(yL - y).max() < 1e-4
+    assert np.abs(lg - logits).max() < 1e-4
+    margin = np.sort(logits, axis=1)[:, -1] - np.sort(logits, axis=1)[:, -2]
+    sure = margin > 1e-3
+    np.testing.assert_array_equal(cat[sure], logits.argmax(axis=1)[sure])

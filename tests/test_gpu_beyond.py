@@ -240,3 +240,36 @@ def test_post_tolerance_is_the_decimal_value(ctx):
     np.testing.assert_array_equal(y, r["yL"])
     assert (y[1, kept] == np.float32(0.05)).all()
     np.testing.assert_array_equal(plan.categories(), r["cat"])
+
+
+@pytest.mark.parametrize("key", ["B", "C", "D"])
+@pytest.mark.parametrize("mode", ["BF", "SNICIT"])
+def test_real_nets_bcd_from_csr(ctx, key, mode):
+    """The reference's beyond nets B, C and D (fixtures of their TSV files) through
+    snicit_net_create_csr with their dense output layers; the hidden layers take inputs of
+    width N directly (net D's CIFAR-10 features are that by design; the reference tree lacks
+    the input-layer weights of B and C).  BF and lossy SNICIT at the reference's default
+    threshold equal the oracle bit for bit.  Inputs: clustered features in [0, 1]."""
+    spec = nets.BEYOND_NETS[key]
+    a = nets.beyond_net(key)
+    N, B, t = spec["N"], 900, spec["t"]
+    x = nets.prototype_features(N, B, 31)
+    net = sa.Net.from_csr(ctx, N, a["row_ptr"], a["cols"], a["vals"], a["bias"], clip=1.0,
+                          rows_are_inputs=True)
+    net.set_io(None, None, 1.0, a["w_out"].T.copy(), a["b_out"])
+    cols, vals = nets.csr_inputs_to_ell(a["row_ptr"], a["cols"], a["vals"], N)
+    kw = dict(w_out=a["w_out"].T.copy(), b_out=a["b_out"])
+    if mode == "BF":
+        plan, _ = _run(net, sa.default_cfg("BF"), x)
+        yL, lg, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, **kw)
+        np.testing.assert_array_equal(plan.activations(), yL)
+    else:
+        plan, st = _run(net, sa.default_cfg("SNICIT", threshold=t, cluster=sa.CLUSTER_LOSSY), x)
+        r = orc.model_snicit_lossy(cols, vals, a["bias"], 1.0, x, t, **kw)
+        assert st.centroids == r["k"] and r["k"] > 1
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
+        lg, cat = r["logits"], r["cat"]
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LOGITS).reshape(B, 10), lg)
+    np.testing.assert_array_equal(plan.categories(), cat)

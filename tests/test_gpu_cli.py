@@ -153,3 +153,67 @@ def test_beyond_cli_on_reference_files(tmp_path, mode):
                                         w_out=a["w_out"].T.copy(), b_out=a["b_out"])
             want += (rr["cat"] == cat[off:off + 1000]).sum()
         assert abs(acc - 100.0 * want / B) < 1e-3
+
+
+def _write_beyond_net_d(root):
+    """Net D's files in the reference layout (no input layer: hidden layers l1..l12, output
+    layer l13, src/beyond/SNICIT/SNICIT.hpp:175-365)."""
+    a = nets.beyond_net("D")
+    name = "n256-l12-acc75.86"
+    wdir = os.path.join(root, "networks", "tsv_weights", name)
+    bdir = os.path.join(root, "networks", "tsv_biases", name)
+    os.makedirs(wdir)
+    os.makedirs(bdir)
+
+    def values(path, arr):
+        with open(path, "w") as f:
+            f.write("\n".join(repr(float(v)) for v in arr.ravel()) + "\n")
+
+    for h in range(12):
+        rp = a["row_ptr"][h]
+        with open(os.path.join(wdir, f"l{h + 1}-sparse.tsv"), "w") as f:
+            for i in range(256):
+                for e in range(rp[i], rp[i + 1]):
+                    f.write(f"{i}\t{a['cols'][e]}\t{float(a['vals'][e])!r}\n")
+        values(os.path.join(bdir, f"l{h + 1}-sparse.tsv"), a["bias"][h])
+    values(os.path.join(wdir, "l13-dense.tsv"), a["w_out"])
+    values(os.path.join(bdir, "l13-dense.tsv"), a["b_out"])
+    return a
+
+
+@pytest.mark.parametrize("mode", ["BF", "SNICIT"])
+def test_beyond_cli_cifar_net_d(tmp_path, mode):
+    """Net D's files, CIFAR-10 inputs as text (one of the 256 features per line) and labels as
+    3073-byte binary records (label byte first), the formats the reference reads
+    (src/beyond/SNICIT/SNICIT.hpp:402-415,462-478).  Labels are the oracle's plain
+    categories, so BF scores 100% and lossy SNICIT scores the oracle's agreement."""
+    B, N = 1200, 256
+    x = nets.prototype_features(N, B, 41)
+    data = tmp_path / "dataset" / "beyond"
+    a = _write_beyond_net_d(str(data))
+    cols, vals = nets.csr_inputs_to_ell(a["row_ptr"], a["cols"], a["vals"], N)
+    kw = dict(w_out=a["w_out"].T.copy(), b_out=a["b_out"])
+    _, _, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, **kw)
+    cdir = os.path.join(str(data), "CIFAR-10")
+    os.makedirs(cdir)
+    with open(os.path.join(cdir, "cifar-input.txt"), "w") as f:
+        f.write("\n".join(repr(float(v)) for v in x.ravel()) + "\n")
+    rec = np.random.default_rng(2).integers(0, 256, (B, 3073)).astype(np.uint8)
+    rec[:, 0] = cat
+    with open(os.path.join(cdir, "cifar-label.bin"), "wb") as f:
+        f.write(rec.tobytes())
+    r = _run([os.path.join(PKG, "snicit_beyond"), "-m", mode, "-k", "D", "-n", str(B), "-b", "600",
+              "--dataset", str(data)], cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout
+    m = re.search(rf"{mode} info: accuracy ([0-9.]+)% runtime [0-9.e+-]+ms avgpost [0-9.e+-]+ms",
+                  r.stdout)
+    assert m, r.stdout
+    acc = float(m.group(1))
+    if mode == "BF":
+        assert acc == 100.0
+    else:
+        want = 0
+        for off in range(0, B, 600):
+            rr = orc.model_snicit_lossy(cols, vals, a["bias"], 1.0, x[off:off + 600], 6, **kw)
+            want += int((rr["cat"] == cat[off:off + 600]).sum())
+        assert abs(acc - 100.0 * want / B) < 1e-3
