@@ -42,8 +42,13 @@ std::vector<float> read_values(const std::string& path) {
   if (!f) die("cannot open " + path);
   std::vector<float> v;
   std::string line;
-  while (std::getline(f, line))
-    if (!line.empty()) v.push_back(std::stof(line));
+  for (int64_t no = 1; std::getline(f, line); ++no) {
+    if (line.empty()) continue;
+    char* end = nullptr;
+    const float x = std::strtof(line.c_str(), &end);  // std::stof's conversion, without throwing
+    if (end == line.c_str()) die("not a number at line " + std::to_string(no) + " of " + path);
+    v.push_back(x);
+  }
   return v;
 }
 
@@ -82,8 +87,13 @@ bool read_idx(const std::string& path, int header_words, std::vector<uint8_t>& d
   return true;
 }
 
-// Transposes an input-major [rows_in][cols] block into [cols][rows_in].
-std::vector<float> transpose(const std::vector<float>& a, int rows_in, int cols) {
+// Reads an input-major [rows_in][cols] dense layer and returns it as [cols][rows_in]; the
+// file must hold exactly rows_in * cols values.
+std::vector<float> read_dense_transposed(const std::string& path, int rows_in, int cols) {
+  const std::vector<float> a = read_values(path);
+  if ((int64_t)a.size() != (int64_t)rows_in * cols)
+    die("dense layer size mismatch in " + path + ": " + std::to_string(a.size()) +
+        " values, expected " + std::to_string((int64_t)rows_in * cols));
   std::vector<float> t((size_t)rows_in * cols);
   for (int i = 0; i < rows_in; ++i)
     for (int k = 0; k < cols; ++k) t[(size_t)k * rows_in + i] = a[(size_t)i * cols + k];
@@ -139,10 +149,9 @@ int main(int argc, char** argv) {
   const int Nin = cifar ? 0 : 784, Nout = 10;
   std::vector<float> w_in, b_in;
   if (!cifar) {
-    w_in = transpose(read_values(wdir + "l1-dense.tsv"), Nin, N);
+    w_in = read_dense_transposed(wdir + "l1-dense.tsv", Nin, N);
     b_in = read_values(bdir + "l1-dense.tsv");
-    if ((int64_t)w_in.size() != (int64_t)Nin * N || (int)b_in.size() != N)
-      die("input layer size mismatch");
+    if ((int)b_in.size() != N) die("input layer bias size mismatch");
   }
   std::vector<int64_t> row_ptr;
   std::vector<int32_t> cols;
@@ -155,9 +164,9 @@ int main(int argc, char** argv) {
     bias.insert(bias.end(), b.begin(), b.end());
   }
   const std::string out_name = "l" + std::to_string(L + off) + "-dense.tsv";
-  std::vector<float> w_out = transpose(read_values(wdir + out_name), N, Nout);
+  std::vector<float> w_out = read_dense_transposed(wdir + out_name, N, Nout);
   std::vector<float> b_out = read_values(bdir + out_name);
-  if ((int)b_out.size() != Nout) die("output layer size mismatch");
+  if ((int)b_out.size() != Nout) die("output layer bias size mismatch");
 
   // inputs and labels
   const int width = cifar ? N : Nin;

@@ -155,19 +155,18 @@ int main(int argc, char** argv) {
   if (batch <= 0 || inputs % batch) die("batch must be a factor of the number of inputs");
   std::cout << "Current mode: " << mode << std::endl;
 
-  snicit_ctx* ctx = nullptr;
-  check(snicit_ctx_create(0, &ctx), "context");
-  snicit_net* net = nullptr;
-  std::vector<float> y0;
-  std::vector<int32_t> golden;
+  // every file is read and checked before the device is touched
   const int K = 32;
-  if (synthetic && !hash_reorder) {
-    check(snicit_net_generate_sdgc(ctx, N, L, seed, 0.0625f, bias, 32.f, &net), "net");
+  const bool generate_on_device = synthetic && !hash_reorder;
+  std::vector<float> y0;
+  std::vector<int32_t> golden, cols;
+  std::vector<float> vals;
+  if (generate_on_device) {
     y0.resize((size_t)inputs * N);
     check(snicit_gen_images(N, seed + 1, 0, inputs, y0.data()), "images");
   } else {
-    std::vector<int32_t> cols((size_t)L * N * K);
-    std::vector<float> vals((size_t)L * N * K, 0.0625f);
+    cols.resize((size_t)L * N * K);
+    vals.assign((size_t)L * N * K, 0.0625f);
     if (synthetic) {
       check(snicit_gen_sdgc_ell(N, L, seed, cols.data()), "net");
       y0.resize((size_t)inputs * N);
@@ -186,9 +185,19 @@ int main(int argc, char** argv) {
       apply_permutation(hash_permutation(N), L, K, cols, vals, y0, inputs);
       std::cout << "Neurons renumbered by the bucket hash (" << N << " neurons)" << std::endl;
     }
+  }
+
+  snicit_ctx* ctx = nullptr;
+  check(snicit_ctx_create(0, &ctx), "context");
+  snicit_net* net = nullptr;
+  if (generate_on_device) {
+    check(snicit_net_generate_sdgc(ctx, N, L, seed, 0.0625f, bias, 32.f, &net), "net");
+  } else {
     check(snicit_net_create_ell(ctx, N, L, K, cols.data(), vals.data(), SNICIT_BIAS_SCALAR, &bias,
                                 32.f, &net),
           "net");
+    std::vector<int32_t>().swap(cols);
+    std::vector<float>().swap(vals);
   }
   snicit_run_cfg cfg;
   snicit_run_cfg_default(&cfg);

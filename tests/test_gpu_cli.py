@@ -11,7 +11,7 @@ import pytest
 
 import oracle as orc
 import snicit_amd as sa
-from tests import nets
+from tests import cli_files, nets
 
 pytestmark = pytest.mark.gpu
 
@@ -26,34 +26,11 @@ def _run(cmd, cwd):
     return r
 
 
-def _write_sdgc_dataset(root, N, L, seed, B):
-    cols = sa.gen_sdgc_cols(N, L, seed)
-    wdir = os.path.join(root, "tsv_weights", f"neuron{N}")
-    os.makedirs(wdir)
-    for l in range(L):
-        with open(os.path.join(wdir, f"n{N}-l{l + 1}.tsv"), "w") as f:
-            for o in range(N):
-                for i in cols[l, o]:
-                    f.write(f"{i + 1}\t{o + 1}\t0.0625\n")
-    y0 = sa.gen_images(N, 5, 0, B)
-    mdir = os.path.join(root, "MNIST")
-    os.makedirs(mdir)
-    with open(os.path.join(mdir, f"sparse-images-{N}.tsv"), "w") as f:
-        for b, n in zip(*np.nonzero(y0)):
-            f.write(f"{b + 1}\t{n + 1}\t1\n")
-    _, cat, _ = orc.plain_f32(cols, np.full(cols.shape, 0.0625, np.float32), sa.SDGC_BIAS[N],
-                              32.0, y0, want_y=False)
-    with open(os.path.join(mdir, f"neuron{N}-l{L}-categories.tsv"), "w") as f:
-        for b in np.nonzero(cat)[0]:
-            f.write(f"{b + 1}\n")
-    return cat
-
-
 @pytest.mark.parametrize("mode", ["SNICIT", "BF", "XY"])
 def test_sdgc_cli_on_graph_challenge_files(tmp_path, mode):
     N, L, B = 1024, 24, 2000
     data = tmp_path / "dataset" / "SDGC"
-    cat = _write_sdgc_dataset(str(data), N, L, 3, B)
+    cat = cli_files.write_sdgc_dataset(str(data), N, L, 3, B)
     assert 0 < cat.sum() < B
     r = _run([os.path.join(PKG, "snicit_sdgc"), "-m", mode, "-n", str(N), "-l", str(L), "-t", "6",
               "-b", "1000", "--inputs", str(B), "--dataset", str(data)], cwd=str(tmp_path))
@@ -69,7 +46,7 @@ def test_sdgc_cli_hash_reorder_keeps_categories(tmp_path, mode):
     written for the original numbering, so passing shows the renumbered run classifies alike."""
     N, L, B = 1024, 24, 2000
     data = tmp_path / "dataset" / "SDGC"
-    _write_sdgc_dataset(str(data), N, L, 4, B)
+    cli_files.write_sdgc_dataset(str(data), N, L, 4, B)
     base = [os.path.join(PKG, "snicit_sdgc"), "-m", mode, "-n", str(N), "-l", str(L), "-t", "6",
             "-b", "1000", "--inputs", str(B), "--dataset", str(data)]
     r = _run(base + ["--hash-reorder"], cwd=str(tmp_path))
@@ -89,36 +66,6 @@ def test_sdgc_cli_synthetic(tmp_path):
     assert re.search(r"SNICIT info: runtime [0-9.e+-]+ms avgpost [0-9.e+-]+ms", r.stdout)
 
 
-def _write_beyond_net_a(root, labels):
-    a = nets.beyond_net_a()
-    name = "n128-l18-acc94.94"
-    wdir = os.path.join(root, "networks", "tsv_weights", name)
-    bdir = os.path.join(root, "networks", "tsv_biases", name)
-    os.makedirs(wdir)
-    os.makedirs(bdir)
-
-    def values(path, arr):
-        with open(path, "w") as f:
-            f.write("\n".join(repr(float(v)) for v in arr.ravel()) + "\n")
-
-    values(os.path.join(wdir, "l1-dense.tsv"), a["w_in"])
-    values(os.path.join(bdir, "l1-dense.tsv"), a["b_in"])
-    for h in range(18):
-        rp = a["row_ptr"][h]
-        with open(os.path.join(wdir, f"l{h + 2}-sparse.tsv"), "w") as f:
-            for i in range(128):
-                for e in range(rp[i], rp[i + 1]):
-                    f.write(f"{i}\t{a['cols'][e]}\t{float(a['vals'][e])!r}\n")
-        values(os.path.join(bdir, f"l{h + 2}-sparse.tsv"), a["bias"][h])
-    values(os.path.join(wdir, "l20-dense.tsv"), a["w_out"])
-    values(os.path.join(bdir, "l20-dense.tsv"), a["b_out"])
-    mdir = os.path.join(root, "MNIST")
-    os.makedirs(mdir)
-    with open(os.path.join(mdir, "t10k-labels-idx1-ubyte"), "wb") as f:
-        f.write(struct.pack(">ii", 2049, len(labels)) + bytes(int(v) for v in labels))
-    return a
-
-
 @pytest.mark.parametrize("mode", ["BF", "SNICIT"])
 def test_beyond_cli_on_reference_files(tmp_path, mode):
     """Net A's files in the reference layout, idx images and labels written from a seeded
@@ -133,7 +80,7 @@ def test_beyond_cli_on_reference_files(tmp_path, mode):
                                 b_in=a["b_in"], in_clip=1.0, w_out=a["w_out"].T.copy(),
                                 b_out=a["b_out"])
     data = tmp_path / "dataset" / "beyond"
-    _write_beyond_net_a(str(data), cat)
+    cli_files.write_beyond_net_a(str(data), cat)
     with open(os.path.join(str(data), "MNIST", "t10k-images-idx3-ubyte"), "wb") as f:
         f.write(struct.pack(">iiii", 2051, B, 28, 28) + img.tobytes())
     r = _run([os.path.join(PKG, "snicit_beyond"), "-m", mode, "-k", "A", "-n", str(B), "-b", "1000",
@@ -155,32 +102,6 @@ def test_beyond_cli_on_reference_files(tmp_path, mode):
         assert abs(acc - 100.0 * want / B) < 1e-3
 
 
-def _write_beyond_net_d(root):
-    """Net D's files in the reference layout (no input layer: hidden layers l1..l12, output
-    layer l13, src/beyond/SNICIT/SNICIT.hpp:175-365)."""
-    a = nets.beyond_net("D")
-    name = "n256-l12-acc75.86"
-    wdir = os.path.join(root, "networks", "tsv_weights", name)
-    bdir = os.path.join(root, "networks", "tsv_biases", name)
-    os.makedirs(wdir)
-    os.makedirs(bdir)
-
-    def values(path, arr):
-        with open(path, "w") as f:
-            f.write("\n".join(repr(float(v)) for v in arr.ravel()) + "\n")
-
-    for h in range(12):
-        rp = a["row_ptr"][h]
-        with open(os.path.join(wdir, f"l{h + 1}-sparse.tsv"), "w") as f:
-            for i in range(256):
-                for e in range(rp[i], rp[i + 1]):
-                    f.write(f"{i}\t{a['cols'][e]}\t{float(a['vals'][e])!r}\n")
-        values(os.path.join(bdir, f"l{h + 1}-sparse.tsv"), a["bias"][h])
-    values(os.path.join(wdir, "l13-dense.tsv"), a["w_out"])
-    values(os.path.join(bdir, "l13-dense.tsv"), a["b_out"])
-    return a
-
-
 @pytest.mark.parametrize("mode", ["BF", "SNICIT"])
 def test_beyond_cli_cifar_net_d(tmp_path, mode):
     """Net D's files, CIFAR-10 inputs as text (one of the 256 features per line) and labels as
@@ -190,18 +111,11 @@ def test_beyond_cli_cifar_net_d(tmp_path, mode):
     B, N = 1200, 256
     x = nets.prototype_features(N, B, 41)
     data = tmp_path / "dataset" / "beyond"
-    a = _write_beyond_net_d(str(data))
+    a = cli_files.write_beyond_net_d(str(data))
     cols, vals = nets.csr_inputs_to_ell(a["row_ptr"], a["cols"], a["vals"], N)
     kw = dict(w_out=a["w_out"].T.copy(), b_out=a["b_out"])
     _, _, cat = orc.model_plain(cols, vals, a["bias"], 1.0, x, **kw)
-    cdir = os.path.join(str(data), "CIFAR-10")
-    os.makedirs(cdir)
-    with open(os.path.join(cdir, "cifar-input.txt"), "w") as f:
-        f.write("\n".join(repr(float(v)) for v in x.ravel()) + "\n")
-    rec = np.random.default_rng(2).integers(0, 256, (B, 3073)).astype(np.uint8)
-    rec[:, 0] = cat
-    with open(os.path.join(cdir, "cifar-label.bin"), "wb") as f:
-        f.write(rec.tobytes())
+    cli_files.write_cifar(str(data), x, cat)
     r = _run([os.path.join(PKG, "snicit_beyond"), "-m", mode, "-k", "D", "-n", str(B), "-b", "600",
               "--dataset", str(data)], cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout
