@@ -240,13 +240,26 @@ typedef struct snicit_tune_info {
   double ms_default;    /* one layer launch with the default shape (0 when cached) */
   double ms_tuned;      /* one layer launch with the chosen shape */
   char key[96];
+  /* post-threshold phase (SNICIT modes): workgroups per post layer launch, the delta-row
+   * count up to which it takes one thread per (row, column) pair, and the post stage of a
+   * whole run with the default and the chosen shape (0 when not tuned) */
+  int64_t post_grid_target;
+  int32_t post_narrow;
+  double ms_post_default;
+  double ms_post_tuned;
 } snicit_tune_info;
 
 /* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
- * may be null (measure, do not store). */
+ * may be null (measure, do not store).  Tunes the pre-threshold launch shape on layer 0 at the
+ * full batch, then (SNICIT modes) the post-threshold grid on whole runs at the plan's own
+ * delta-row counts.  Every shape gives the same results, bit for bit. */
 int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info);
 int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow,
                           int32_t ell_lds_kb, int32_t ell_chunked);
+/* Post-threshold layers: workgroups a launch aims for (default 2048) and the delta-row count
+ * up to which a launch takes one thread per (row, column) pair (default 64). */
+int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
+                               int32_t post_narrow);
 
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another

@@ -57,13 +57,17 @@ class TuneInfo(ctypes.Structure):
     _fields_ = [("grid_target", ctypes.c_int64), ("narrow", ctypes.c_int32),
                 ("ell_lds_kb", ctypes.c_int32), ("ell_chunked", ctypes.c_int32),
                 ("from_cache", ctypes.c_int32), ("ms_default", ctypes.c_double),
-                ("ms_tuned", ctypes.c_double), ("key", ctypes.c_char * 96)]
+                ("ms_tuned", ctypes.c_double), ("key", ctypes.c_char * 96),
+                ("post_grid_target", ctypes.c_int64), ("post_narrow", ctypes.c_int32),
+                ("ms_post_default", ctypes.c_double), ("ms_post_tuned", ctypes.c_double)]
 
     def as_dict(self):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
                 "ell_lds_kb": self.ell_lds_kb, "ell_chunked": self.ell_chunked,
                 "from_cache": bool(self.from_cache), "ms_default": self.ms_default,
-                "ms_tuned": self.ms_tuned, "key": self.key.decode()}
+                "ms_tuned": self.ms_tuned, "key": self.key.decode(),
+                "post_grid_target": self.post_grid_target, "post_narrow": self.post_narrow,
+                "ms_post_default": self.ms_post_default, "ms_post_tuned": self.ms_post_tuned}
 
 
 class ServerCfg(ctypes.Structure):
@@ -158,6 +162,7 @@ SIGNATURES = {
     "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32]),
+    "snicit_plan_set_post_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
     "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
@@ -438,6 +443,9 @@ class Plan:
 
     def set_shape(self, grid_target, narrow, ell_lds_kb=80, ell_chunked=2):
         _check(lib().snicit_plan_set_shape(self.h, grid_target, narrow, ell_lds_kb, ell_chunked))
+
+    def set_post_shape(self, post_grid_target, post_narrow=64):
+        _check(lib().snicit_plan_set_post_shape(self.h, post_grid_target, post_narrow))
 
     def activations(self):
         out = np.empty((self.B, self.net.N), dtype=np.float32)

@@ -2011,13 +2011,17 @@ std::string tune_key(const snicit_plan* p) {
 }
 
 // Cache file: a JSON object, one entry per line:
-//   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C, "ms": T},
+//   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C,
+//           "post_grid_target": Q, "post_narrow": V, "ms": T, "ms_post": U},
 struct TuneEntry {
   int64_t grid_target;
   int32_t narrow;
   int32_t ell_lds_kb;
   int32_t ell_chunked;
+  int64_t post_grid_target;
+  int32_t post_narrow;
   double ms;
+  double ms_post;
 };
 std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path) {
   std::vector<std::pair<std::string, TuneEntry>> out;
@@ -2027,14 +2031,15 @@ std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path)
   char line[512];
   while (std::fgets(line, sizeof(line), f)) {
     char key[128];
-    long long g = 0;
-    int w = 0, e = 0, ch = 0;
-    double ms = 0;
+    long long g = 0, q = 0;
+    int w = 0, e = 0, ch = 0, pn = 0;
+    double ms = 0, msp = 0;
     if (std::sscanf(line,
                     " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
-                    "\"ell_chunked\": %d, \"ms\": %lf",
-                    key, &g, &w, &e, &ch, &ms) == 6)
-      out.push_back({key, TuneEntry{g, w, e, ch, ms}});
+                    "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
+                    "\"ms\": %lf, \"ms_post\": %lf",
+                    key, &g, &w, &e, &ch, &q, &pn, &ms, &msp) == 9)
+      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, ms, msp}});
   }
   std::fclose(f);
   return out;
@@ -2049,10 +2054,12 @@ void tune_cache_write(const char* path,
   for (size_t i = 0; i < entries.size(); ++i)
     std::fprintf(f,
                  "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
-                 "\"ell_chunked\": %d, \"ms\": %.6f}%s\n",
+                 "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
+                 "\"ms\": %.6f, \"ms_post\": %.6f}%s\n",
                  entries[i].first.c_str(), (long long)entries[i].second.grid_target,
                  entries[i].second.narrow, entries[i].second.ell_lds_kb,
-                 entries[i].second.ell_chunked, entries[i].second.ms,
+                 entries[i].second.ell_chunked, (long long)entries[i].second.post_grid_target,
+                 entries[i].second.post_narrow, entries[i].second.ms, entries[i].second.ms_post,
                  i + 1 < entries.size() ? "," : "");
   std::fprintf(f, "}\n");
   std::fclose(f);
@@ -2104,7 +2111,10 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       p->shape.narrow = e.second.narrow;
       p->shape.ell_lds_kb = e.second.ell_lds_kb;
       p->shape.ell_chunked = e.second.ell_chunked;
+      p->shape.post_grid_target = e.second.post_grid_target;
+      p->shape.post_narrow = e.second.post_narrow;
       res.ms_tuned = e.second.ms;
+      res.ms_post_tuned = e.second.ms_post;
       res.from_cache = 1;
       found = true;
     }
@@ -2161,6 +2171,46 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       best.narrow = narrow;
       res.ms_tuned = best_ms;
       p->shape = best;
+      // post-threshold phase: whole runs at the plan's own delta-row counts, timed by the
+      // post stage's events (median of three after one run that builds the stage graphs);
+      // the grid first, then the narrow-path limit
+      if (!plain_mode(p->cfg.mode) && p->cfg.threshold < p->net->L && p->ev[0]) {
+        auto post_ms = [&](int64_t g, int32_t pn) {
+          p->shape.post_grid_target = g;
+          p->shape.post_narrow = pn;
+          drop_graphs(p);
+          snicit_stats st{};
+          plan_run(p, -1, &st);
+          double v[3];
+          for (double& m : v) {
+            plan_run(p, -1, &st);
+            m = st.ms_post;
+          }
+          std::sort(v, v + 3);
+          return v[1];
+        };
+        res.ms_post_default = post_ms(def.post_grid_target, def.post_narrow);
+        double best_post = res.ms_post_default;
+        int64_t best_g = def.post_grid_target;
+        int32_t best_n = def.post_narrow;
+        for (int64_t g : {512, 1024, 4096, 8192, 16384}) {
+          const double m = post_ms(g, best_n);
+          if (m < best_post) {
+            best_post = m;
+            best_g = g;
+          }
+        }
+        for (int32_t pn : {0, 32, 128, 256, 512}) {
+          const double m = post_ms(best_g, pn);
+          if (m < best_post) {
+            best_post = m;
+            best_n = pn;
+          }
+        }
+        p->shape.post_grid_target = best_g;
+        p->shape.post_narrow = best_n;
+        res.ms_post_tuned = best_post;
+      }
     } catch (...) {
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
@@ -2172,7 +2222,8 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
     dev_free(d_cnt);
     if (cache_path) {
       cache.push_back({key, TuneEntry{p->shape.grid_target, p->shape.narrow, p->shape.ell_lds_kb,
-                                      p->shape.ell_chunked, res.ms_tuned}});
+                                      p->shape.ell_chunked, p->shape.post_grid_target,
+                                      p->shape.post_narrow, res.ms_tuned, res.ms_post_tuned}});
       tune_cache_write(cache_path, cache);
     }
   }
@@ -2180,6 +2231,8 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
   res.narrow = p->shape.narrow;
   res.ell_lds_kb = p->shape.ell_lds_kb;
   res.ell_chunked = p->shape.ell_chunked;
+  res.post_grid_target = p->shape.post_grid_target;
+  res.post_narrow = p->shape.post_narrow;
   drop_graphs(p);
   p->have_run = false;
   if (info) *info = res;
@@ -2207,6 +2260,17 @@ int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow
     plan->shape.narrow = narrow;
     plan->shape.ell_lds_kb = ell_lds_kb;
     plan->shape.ell_chunked = ell_chunked;
+    drop_graphs(plan);
+  });
+}
+
+int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
+                               int32_t post_narrow) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_REQUIRE(post_grid_target >= 1 && post_narrow >= 0, "bad post launch shape");
+    plan->shape.post_grid_target = post_grid_target;
+    plan->shape.post_narrow = post_narrow;
     drop_graphs(plan);
   });
 }

@@ -2151,7 +2151,7 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
   const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
-  const int64_t gy = grid_rows(shape.grid_target, P, tiles);
+  const int64_t gy = grid_rows(shape.post_grid_target, P, tiles);
   dim3 grid((unsigned)P, (unsigned)gy);
   const int32_t* pcols = la.pcols;
   const int32_t* pbeg = la.pbeg;
@@ -2161,12 +2161,12 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.narrow, la.psame);
+                       mcent, flags, eps_post, shape.post_narrow, la.psame);
   else
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.narrow, la.psame);
+                       mcent, flags, eps_post, shape.post_narrow, la.psame);
   check_launch();
 }
 

@@ -51,6 +51,12 @@ constexpr int kPostRing = 64;
 struct LaunchShape {
   int64_t grid_target = 16384;
   int32_t narrow = 64;
+  // workgroups a post-threshold layer launch aims for (panels x 256-column ranges); each
+  // workgroup stages its panel's weights and centroid inputs once for all of its columns,
+  // so fewer, longer workgroups read less
+  int64_t post_grid_target = 2048;
+  // delta-row count up to which the post kernel takes one thread per (row, column) pair
+  int32_t post_narrow = 64;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
   // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in

@@ -21,10 +21,14 @@ def test_launch_shapes_do_not_change_results(ctx, mode):
     net = sa.Net.from_ell(ctx, cols, vals, -0.02)
     cfg = sa.default_cfg(mode, threshold=4)
     ref = None
-    for grid_target, narrow in [(16384, 64), (1, 0), (1, 100000), (1 << 20, 0), (4096, 256)]:
-        print(f"shape grid_target={grid_target} narrow={narrow}", flush=True)
+    for grid_target, narrow, post, post_narrow in [
+            (16384, 64, 2048, 64), (1, 0, 1, 0), (1, 100000, 1 << 20, 100000),
+            (1 << 20, 0, 7, 256), (4096, 256, 16384, 0), (16384, 0, 1, 100000)]:
+        print(f"shape grid_target={grid_target} narrow={narrow} post_grid_target={post} "
+              f"post_narrow={post_narrow}", flush=True)
         plan = sa.Plan(net, cfg, B)
         plan.set_shape(grid_target, narrow)
+        plan.set_post_shape(post, post_narrow)
         plan.set_input(y0)
         plan.run()
         got = (plan.categories(), plan.activations())
@@ -70,8 +74,13 @@ def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
     plan = sa.Plan(net, cfg, B)
     plan.set_input_images(3, 0, B)
     info = plan.autotune(cache)
+    print(info, flush=True)
     assert not info["from_cache"] and info["ms_tuned"] <= info["ms_default"]
-    assert info["key"] in json.loads(cache.read_text())
+    # the post phase is tuned on whole runs, timed by the post stage
+    assert 0 < info["ms_post_tuned"] <= info["ms_post_default"]
+    assert info["post_grid_target"] in (512, 1024, 2048, 4096, 8192, 16384)
+    entry = json.loads(cache.read_text())[info["key"]]
+    assert entry["post_grid_target"] == info["post_grid_target"]
     plan.run()
     cat = plan.categories()
     imgs = sa.gen_images(N, 3, 0, B)
@@ -82,6 +91,8 @@ def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
     plan2.set_input_images(3, 0, B)
     info2 = plan2.autotune(cache)
     assert info2["from_cache"]
-    assert (info2["grid_target"], info2["narrow"]) == (info["grid_target"], info["narrow"])
+    for k in ("grid_target", "narrow", "post_grid_target", "post_narrow"):
+        assert info2[k] == info[k], k
+    assert abs(info2["ms_post_tuned"] - info["ms_post_tuned"]) < 1e-5
     plan2.run()
     np.testing.assert_array_equal(plan2.categories(), ref)
