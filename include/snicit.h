@@ -260,6 +260,13 @@ int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow
  * up to which a launch takes one thread per (row, column) pair (default 64). */
 int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
                                int32_t post_narrow);
+/* Post-threshold layers whose panels hold at most two distinct weight rows (every SDGC net):
+ * 0 = the LDS-staged post kernel; 1 = the wavefront-local post kernel, each wavefront
+ * computing its panel's centroid rows; 2 (default) = the wavefront-local kernel with every
+ * layer's centroid rows computed on a second stream, up to 32 layers ahead of the delta rows
+ * (exact clustering with resident weights; otherwise as 1).  All three give the same bits.
+ * No counterpart in the reference (its post layer is one kernel, kernel.hpp:71-136). */
+int snicit_plan_set_post_form(snicit_plan* plan, int32_t post_form);
 
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another

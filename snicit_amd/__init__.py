@@ -163,6 +163,7 @@ SIGNATURES = {
     "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32]),
     "snicit_plan_set_post_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
+    "snicit_plan_set_post_form": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
     "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
@@ -446,6 +447,14 @@ class Plan:
 
     def set_post_shape(self, post_grid_target, post_narrow=64):
         _check(lib().snicit_plan_set_post_shape(self.h, post_grid_target, post_narrow))
+
+    # post layers with at most two distinct weight rows per panel: LDS-staged kernel,
+    # wavefront-local kernel, or wavefront-local kernel with the centroid rows on a second
+    # stream (default)
+    POST_LDS, POST_WAVE, POST_WAVE_CHAINS = 0, 1, 2
+
+    def set_post_form(self, form):
+        _check(lib().snicit_plan_set_post_form(self.h, form))
 
     def activations(self):
         out = np.empty((self.B, self.net.N), dtype=np.float32)

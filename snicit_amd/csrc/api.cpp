@@ -95,6 +95,7 @@ struct NetBuilder {
     pbeg.push_back(0);
     net->mfma.push_back(0);
     net->dup_rows.push_back(0);
+    net->few_rows.push_back(0);
     net->w_off.push_back(w_floats);
     net->ell_K.push_back(Kp);
     net->ell_off.push_back((int64_t)ecols.size());
@@ -151,7 +152,7 @@ struct NetBuilder {
     net->panel_off.push_back(net->panel_off.back() + P);
     HostLayerPanels grouped;
     const HostLayerPanels& hp = weights_given ? group_equal_rows(hp_in, grouped) : hp_in;
-    bool all_dup = P > 0;
+    bool all_dup = P > 0, all_few = P > 0;
     for (int64_t p = 0; p < P; ++p) {
       const int32_t b = hp.pbeg[p], G = hp.pbeg[p + 1] - b;
       uint32_t m = 0;
@@ -165,8 +166,10 @@ struct NetBuilder {
       psame.push_back(m);
       const uint32_t full = G >= 32 ? 0xFFFFFFFEu : ((1u << G) - 1u) & ~1u;
       all_dup = all_dup && m == full;
+      all_few = all_few && G <= 32 && __builtin_popcount(full & ~m) <= 1;  // rows 1.. that differ
     }
     net->dup_rows.push_back(all_dup ? 1 : 0);
+    net->few_rows.push_back(all_few ? 1 : 0);
     pcols.insert(pcols.end(), hp.pcols.begin(), hp.pcols.end());
     pbeg.insert(pbeg.end(), hp.pbeg.begin(), hp.pbeg.end());
     outs.insert(outs.end(), hp.outs.begin(), hp.outs.end());
@@ -692,12 +695,52 @@ const int32_t* post_count(const snicit_plan* p, int j) {
 // post_compacts_after names, the per-layer flags since the last compaction are counted
 // (non-empty rows after each layer) and the list is compacted.  Every layer's centroid
 // rows, delta rows and counts equal those of the per-layer compaction.
+// Whether the post stage runs its centroid rows on the chain stream: every post layer's
+// panels hold at most two distinct weight rows, and the weights are resident.
+bool post_chains(const snicit_plan* p) {
+  if (!p->compact_post || !p->chain_stream || p->net->streamed() || p->shape.post_form < 2)
+    return false;
+  for (int l = p->cfg.threshold; l < p->net->L; ++l)
+    if (!p->net->few_rows[l]) return false;
+  return true;
+}
+
 void enqueue_post_compact(snicit_plan* p) {
   hipStream_t s = p->stream;
   const int t = p->cfg.threshold, L = p->net->L, n_post = L - t;
+  const int N = p->net->N;
   SnicitState st = snicit_state(p);
   int orig = cluster_cur(p) ^ 1, mc = 1;
   int j0 = 0;  // first layer since the last compaction
+  // Centroid branch: the centroid columns at the compact stride, then each layer's chain
+  // records and centroid rows (they depend on the centroid columns only) in blocks of
+  // kChainSync layers, up to two blocks ahead of the delta rows: delta block b waits for
+  // chain block b, and chain block b + 2 for delta block b (ring slot reuse).  Events are
+  // recorded and waited in one order, so each wait binds to the record of its own block.
+  const bool chains = post_chains(p);
+  hipStream_t cs = chains ? p->chain_stream : s;
+  constexpr int B = kChainRing / 2, E = 3;
+  const int nblk = (n_post + B - 1) / B;
+  auto zslot = [&](int j) {
+    return p->d_zchain + (size_t)(j % kChainRing) * p->chain_panels * kPostChainRec;
+  };
+  auto chain_block = [&](int b) {
+    if (b >= 2) SNICIT_HIP(hipStreamWaitEvent(cs, p->ev_delta[(b - 2) % E], 0));
+    for (int j = b * B; j < std::min(n_post, (b + 1) * B); ++j) {
+      const int l = t + j;
+      launch_post_chain(*p->net, l, plan_layer_arrays(p, l), p->d_Cp[j & 1],
+                        p->d_Cp[(j + 1) & 1], p->kcap, p->d_k, true, zslot(j), cs);
+    }
+    SNICIT_HIP(hipEventRecord(p->ev_chain[b % E], cs));
+  };
+  if (chains) {
+    SNICIT_HIP(hipEventRecord(p->ev_chain[E], s));
+    SNICIT_HIP(hipStreamWaitEvent(cs, p->ev_chain[E], 0));
+  }
+  // centroid columns at the compact stride for the post layers, back at kcap in d_C[1]
+  launch_centroid_restride(p->d_C[0], false, p->d_Cp[0], true, p->kcap, p->d_k, N, cs);
+  if (chains)
+    for (int b = 0; b < std::min(nblk, 2); ++b) chain_block(b);
   weights_begin(p);
   for (int j = 0; j < n_post; ++j) {
     const int l = t + j;
@@ -706,14 +749,19 @@ void enqueue_post_compact(snicit_plan* p) {
     const bool after_compaction = j > 0 && post_compacts_after(j - 1, n_post);
     const bool compact_next = post_compacts_after(j, n_post);
     int32_t* ring = p->d_ring + (int64_t)(j % kPostRing) * p->ld;
+    if (chains && j % B == 0) SNICIT_HIP(hipStreamWaitEvent(s, p->ev_chain[(j / B) % E], 0));
     prof_mark(p, l, 0);
-    launch_panel_post(*p->net, l, plan_layer_arrays(p, l), false, p->d_C[q], p->d_C[q ^ 1],
+    launch_panel_post(*p->net, l, plan_layer_arrays(p, l), false, p->d_Cp[q], p->d_Cp[q ^ 1],
                       p->kcap, p->d_k, q == 0 ? st.Da : st.Db, p->ld,
                       post_count(p, j == 0 ? 0 : j - 1), q == 0 ? st.Db : st.Da, p->ld, true,
                       post_count(p, j), compact_next ? nullptr : p->d_nlist + l + 1,
                       after_compaction ? p->d_mapD : nullptr, p->d_mcar[mc], ring, 0.f, p->cap,
-                      p->shape, s);
+                      p->shape, s, true, chains ? zslot(j) : nullptr);
     prof_mark(p, l, 1);
+    if (chains && (j % B == B - 1 || j == n_post - 1) && j / B + 2 < nblk) {
+      SNICIT_HIP(hipEventRecord(p->ev_delta[(j / B) % E], s));
+      chain_block(j / B + 2);
+    }
     prof_mark(p, l, 2);
     prof_mark(p, l, 3);
     done_layer(p, l);
@@ -727,6 +775,11 @@ void enqueue_post_compact(snicit_plan* p) {
       mc ^= 1;
       j0 = j + 1;
     }
+  }
+  launch_centroid_restride(p->d_Cp[n_post & 1], true, p->d_C[1], false, p->kcap, p->d_k, N, cs);
+  if (chains) {
+    SNICIT_HIP(hipEventRecord(p->ev_delta[E], cs));
+    SNICIT_HIP(hipStreamWaitEvent(s, p->ev_delta[E], 0));
   }
 }
 
@@ -892,7 +945,7 @@ void plan_bookkeeping(snicit_plan* p, int32_t stop) {
     const int flips = p->compact_post ? post_compactions(n_post) : n_post;
     p->last_orig = (cluster_cur(p) ^ 1) ^ (flips & 1);
     p->last_mcar = 1 ^ (flips & 1);
-    p->last_C = p->d_C[n_post & 1];
+    p->last_C = p->compact_post ? p->d_C[1] : p->d_C[n_post & 1];
     p->last_D = (((n_post - 1) & 1) == 0) ? st.Db : st.Da;
     p->stop_layer = L;
   }
@@ -1038,6 +1091,12 @@ void plan_free(snicit_plan* p) {
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
   p->copy_stream = nullptr;
+  if (p->chain_stream) (void)hipStreamSynchronize(p->chain_stream);
+  for (auto* v : {&p->ev_chain, &p->ev_delta})
+    for (auto& e : *v)
+      if (e) (void)hipEventDestroy(e);
+  if (p->chain_stream) (void)hipStreamDestroy(p->chain_stream);
+  p->chain_stream = nullptr;
   for (auto& e : p->prof_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : p->ev)
@@ -1050,7 +1109,8 @@ void plan_free(snicit_plan* p) {
                   p->d_dist,     p->d_h0,      p->d_logits,   p->d_poolrows, p->d_poolcol,
                   p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
-                  p->d_stage,  p->d_nonfinite, p->d_nlist, p->d_ring};
+                  p->d_stage,  p->d_nonfinite, p->d_nlist, p->d_ring,   p->d_Cp[0],
+                  p->d_Cp[1],  p->d_zchain};
   for (void* q : ptrs) dev_free(q);
 }
 
@@ -1510,6 +1570,27 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_HIP(hipMemset(p->d_zc, 0, (size_t)N * p->kcap * 4));
       p->compact_post = compact_post_layout(p.get());
       if (p->compact_post) {
+        for (int i = 0; i < 2; ++i) {
+          p->d_Cp[i] = dev_alloc<float>((size_t)N * p->kcap, acct);
+          SNICIT_HIP(hipMemset(p->d_Cp[i], 0, (size_t)N * p->kcap * 4));
+        }
+        // centroid branch (not for the allocation-free entry point, which creates nothing)
+        bool few = !g_dev_arena;
+        int64_t maxP = 0;
+        for (int l = cfg->threshold; l < L; ++l) {
+          few = few && net->few_rows[l];
+          maxP = std::max<int64_t>(maxP, net->panels(l));
+        }
+        if (few && maxP > 0) {
+          p->chain_panels = maxP;
+          p->d_zchain = dev_alloc<float>((size_t)kChainRing * maxP * kPostChainRec, acct);
+          p->ev_chain.assign(kChainRing + 2, nullptr);
+          p->ev_delta.assign(kChainRing + 2, nullptr);
+          for (auto* v : {&p->ev_chain, &p->ev_delta})
+            for (auto& e : *v) SNICIT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+          // default priority, like every stream of the library
+          SNICIT_HIP(hipStreamCreateWithFlags(&p->chain_stream, hipStreamNonBlocking));
+        }
         p->d_nlist = dev_alloc<int32_t>(L + 2, acct);
         p->d_ring = dev_alloc<int32_t>((size_t)kPostRing * p->ld, acct);
         SNICIT_HIP(hipMemset(p->d_nlist, 0, (L + 2) * 4));
@@ -2271,6 +2352,15 @@ int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
     SNICIT_REQUIRE(post_grid_target >= 1 && post_narrow >= 0, "bad post launch shape");
     plan->shape.post_grid_target = post_grid_target;
     plan->shape.post_narrow = post_narrow;
+    drop_graphs(plan);
+  });
+}
+
+int snicit_plan_set_post_form(snicit_plan* plan, int32_t post_form) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_REQUIRE(post_form >= 0 && post_form <= 2, "post_form must be 0, 1 or 2");
+    plan->shape.post_form = post_form;
     drop_graphs(plan);
   });
 }

@@ -110,6 +110,7 @@ struct Net {
   // rows with equal weights are adjacent.
   uint32_t* d_psame = nullptr;
   std::vector<uint8_t> dup_rows;   // [L] every panel of the layer repeats one weight row
+  std::vector<uint8_t> few_rows;   // [L] every panel of the layer has at most two distinct rows
   std::vector<int64_t> w_off;      // [L] first float of each layer in d_w
   std::vector<uint8_t> mfma;       // [L] layer runs on the matrix cores (slot-major weights)
   // plain-epilogue launches of row-major panel layers run panel_plain_mfma_kernel

@@ -30,6 +30,11 @@ __device__ __forceinline__ float act_clip(float z, float clip) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Value of v in lane l (l wavefront-uniform).
+__device__ __forceinline__ float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 __device__ __forceinline__ int lane_prefix(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
@@ -319,21 +324,234 @@ __global__ __launch_bounds__(kBlock) void panel_centroid_kernel(
 }
 
 // ---- post-threshold layer -------------------------------------------------------------
-// One launch per post layer (src/SDGC/SNICIT/kernel.hpp:71-136, post_spMM + post_minus).
-// Each block stages its panel's weights (4 KB, one float4 per thread) and the panel's 32
-// inputs of every centroid in LDS, computes the centroid rows zc = W yhat_c + b of the panel
-// (block row 0 also writes yhat_c' = f(zc)), then produces the delta rows of its columns,
-// y = f(zc[m] + W yhat_i) - f(zc[m]), with weights and zc read from LDS.  Columns <=
-// `narrow` use one thread per (row, column) pair in block row 0; otherwise each
-// wavefront owns 64 columns and every row of the panel.  Same fused-multiply-add chain as
-// panel_spmm_kernel.
+// One launch per post layer (src/SDGC/SNICIT/kernel.hpp:71-136, post_spMM + post_minus):
+// the centroid rows zc = W yhat_c + b of every panel (workgroup row 0 also writes
+// yhat_c' = f(zc)), then the delta rows of the launch's columns,
+// y = f(zc[m] + W yhat_i) - f(zc[m]).  Same fused-multiply-add chain as panel_spmm_kernel.
+// psame (nullable) marks rows whose weights equal the previous row's: their chain is the
+// previous row's, bit for bit, so it is computed once (SDGC panels: one chain for 32 rows).
+// Two forms, chosen per panel:
+//  * at most two distinct weight rows and more than `narrow` columns: every wavefront works
+//    alone, with no LDS and no barrier.  Lanes hold the distinct rows' weights (lanes 0-31
+//    and 32-63) and the rows' output ids and biases; lane d*k + c runs the centroid chain of
+//    distinct row d and centroid c, and each column takes its centroid's value with a
+//    shuffle.  The wavefront's column gathers and its centroid loads are in flight together.
+//  * otherwise the workgroup stages the panel's weights, its 32 inputs of every centroid
+//    and the centroid rows in LDS; with at most `narrow` columns one thread per (row,
+//    column) pair, else each wavefront owns 64 columns and every row of the panel.
 // Compact post layout (ldin_cnt non-null, compact_out): the delta buffers' row stride is
 // compact_ld of the positions they hold, read from the device counters, so a layer's delta
 // rows fill one dense region; nlist_next (nullable) receives the list length for the next
-// layer when no compaction runs in between.  The counters, the centroid count and the
-// panel bounds are loaded together before the early exit.  psame (nullable) marks rows
-// whose weights equal the previous row's: their chain is the previous row's, bit for bit,
-// so the delta rows reuse it (SDGC panels: one chain for all 32 rows).
+// layer when no compaction runs in between.  ccompact: the centroid columns cin/cout have
+// the stride compact_kst(k, kcap) instead of kcap (fewer bytes per neuron when k is small).
+
+// Panel chain record of the wave-local form: centroid chain (before the bias) of distinct row
+// d and centroid c at d * 32 + c, one value per lane of a wavefront (d < 2, c < 32).
+constexpr int kChainRec = kPostChainRec;
+static_assert(kChainRec == 2 * kMaxCentroidsExact, "one chain per lane of a wavefront");
+
+// Second distinct row of a panel with at most two: the first row past row 0 whose weights
+// differ from the previous row's (G when every row has row 0's weights).
+__device__ __forceinline__ int second_row(const uint32_t* __restrict__ psame, int p, int G) {
+  const uint32_t same = psame ? psame[p] : 0u;
+  const uint32_t diff = ~same & (G >= 32 ? 0xfffffffeu : ((1u << G) - 1u) & ~1u);
+  return diff ? __ffs(diff) - 1 : G;
+}
+
+// Centroid chains of one panel on one wavefront: lane d * 32 + c runs the chain of distinct
+// row d (row 0, or row r1) and centroid c over the panel's 32 inputs of centroid column c
+// (cin, stride kst), in slot order; lanes past the panel's rows or centroids run a clamped
+// copy whose value is never taken.  Eight inputs are loaded at a time.
+__device__ __forceinline__ float panel_chain(int lane, int G, int r1, int beg,
+                                             const int32_t* __restrict__ pc,
+                                             const float* __restrict__ w,
+                                             const float* __restrict__ cin, int kst, int k) {
+  const int d = (lane >> 5) && r1 < G ? 1 : 0;
+  const int c = min(lane & 31, (k > 0 ? k : 1) - 1);
+  const float* __restrict__ wr = w + (size_t)(beg + (d ? r1 : 0)) * kSlots;
+  float z = 0.f;
+#pragma unroll
+  for (int r0 = 0; r0 < kSlots; r0 += 8) {
+    float xc[8], wc[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      xc[r] = cin[(int64_t)pc[r0 + r] * kst + c];
+      wc[r] = wr[r0 + r];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) z = __builtin_fmaf(wc[r], xc[r], z);
+  }
+  return z;
+}
+
+// Centroid rows of one panel from its chain record (lane d * 32 + c): cout[o][c] =
+// f(chain(d(o), c) + b(o)) for the panel's rows o, d(o) = 1 from row r1 on.
+__device__ __forceinline__ void panel_centroid_rows(int lane, int G, int r1,
+                                                    const int32_t* __restrict__ outs, int beg,
+                                                    const float* __restrict__ bias_vec,
+                                                    float bias_s, float clip, float zch,
+                                                    float* __restrict__ cout, int kst, int k) {
+  const int d = lane >> 5, c = lane & 31;
+  if (c >= k || (d == 1 && r1 >= G)) return;
+  for (int jj = d ? r1 : 0; jj < (d ? G : r1); ++jj) {
+    const int o = outs[beg + jj];
+    cout[(int64_t)o * kst + c] = act_clip(zch + (bias_vec ? bias_vec[o] : bias_s), clip);
+  }
+}
+
+// Wave-local form (see above).  G rows with one distinct weight row, or two (TWO) where the
+// second starts at row r1.  The wavefront's first column gathers are issued before anything
+// else, so every wavefront has loads in flight from its start.  Its panel's chain record
+// comes from zrec (precomputed by post_chain_kernel) or is computed here.  Each tile takes
+// its columns' centroid values with one shuffle per distinct row; with a scalar bias every
+// row of a distinct row's class has the same output.
+template <bool LOSSY, bool TWO>
+__device__ __forceinline__ void post_wave_form(
+    int G, int r1, int beg, const int32_t* __restrict__ pc, const int32_t* __restrict__ outs,
+    const float* __restrict__ w, const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ zrec, const float* __restrict__ cin, float* __restrict__ cout,
+    int kst, int k, const float* __restrict__ din, int64_t ldin, float* __restrict__ dout,
+    int64_t ldout, int64_t count, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post) {
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int64_t step = (int64_t)gridDim.y * kBlock;
+  int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave;
+  // The tile at `base`: columns past count load list position 0 (count > 0 whenever a tile
+  // runs) and store nothing; every load is unconditional, so the gathers and the centroid
+  // index are in flight together.
+  float x[kSlots];
+  int32_t mi = 0;
+  auto load_tile = [&](int64_t b) {
+    const int64_t pos = b + lane;
+    const uint32_t col = (uint32_t)(pos < count ? pos : 0);
+    const uint32_t src = map ? (uint32_t)map[col] : col;
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) x[r] = din[(int64_t)pc[r] * ldin + src];
+    mi = mcent[col];
+  };
+  if (base < count) load_tile(base);
+  // lanes 0-31: weights of row 0 (the first distinct row); lanes 32-63: row r1 (TWO; row 0
+  // again when the panel has one distinct row, r1 = G)
+  const float wl = w[(size_t)(beg + (TWO && lane >= 32 && r1 < G ? r1 : 0)) * kSlots + (lane & 31)];
+  const int ol = outs[beg + (lane < G ? lane : 0)];
+  const float bl = bias_vec ? bias_vec[ol] : bias_s;
+  float zch;
+  if (zrec) {
+    zch = zrec[lane];
+  } else {
+    zch = panel_chain(lane, G, r1, beg, pc, w, cin, kst, k);
+    if (blockIdx.y == 0 && wv == 0 && cout)
+      panel_centroid_rows(lane, G, r1, outs, beg, bias_vec, bias_s, clip, zch, cout, kst, k);
+  }
+  auto residue = [&](float acc, float z) {
+    float y = act_clip(z + acc, clip) - act_clip(z, clip);
+    if (LOSSY) y = fabsf(y) > eps_post ? y : 0.f;
+    return y;
+  };
+  while (base < count) {
+    const int64_t pos = base + lane;
+    const bool valid = pos < count;
+    bool nz = false;
+#pragma unroll
+    for (int r = 0; r < kSlots; ++r) nz |= (x[r] != 0.f);
+    // a wavefront whose inputs are all zero keeps acc = +0, what the chain gives
+    float acc0 = 0.f, acc1 = 0.f;
+    if (__ballot(nz) != 0ull) {
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) acc0 = __builtin_fmaf(readlane_f(wl, r), x[r], acc0);
+      if (TWO) {
+#pragma unroll
+        for (int r = 0; r < kSlots; ++r) acc1 = __builtin_fmaf(readlane_f(wl, 32 + r), x[r], acc1);
+      }
+    }
+    // centroid value of each column's centroid, per distinct row (before the bias)
+    const float zs0 = __shfl(zch, mi);
+    const float zs1 = TWO ? __shfl(zch, 32 + mi) : 0.f;
+    bool flag = false;
+    if (valid) {
+      float* __restrict__ ocol = dout + pos;
+      if (!bias_vec) {
+        const float y0 = residue(acc0, zs0 + bias_s);
+        const float y1 = TWO ? residue(acc1, zs1 + bias_s) : 0.f;
+        flag = y0 != 0.f || (TWO && y1 != 0.f);
+        if (G == kMaxPanelRows) {
+#pragma unroll
+          for (int jj = 0; jj < kMaxPanelRows; ++jj)
+            ocol[(int64_t)__builtin_amdgcn_readlane(ol, jj) * ldout] = (TWO && jj >= r1) ? y1 : y0;
+        } else {
+          for (int jj = 0; jj < G; ++jj)
+            ocol[(int64_t)__builtin_amdgcn_readlane(ol, jj) * ldout] = (TWO && jj >= r1) ? y1 : y0;
+        }
+      } else {
+        for (int jj = 0; jj < G; ++jj) {
+          const bool second = TWO && jj >= r1;
+          const float y = residue(second ? acc1 : acc0, (second ? zs1 : zs0) + readlane_f(bl, jj));
+          flag |= (y != 0.f);
+          ocol[(int64_t)__builtin_amdgcn_readlane(ol, jj) * ldout] = y;
+        }
+      }
+      if (flag) flags[pos] = 1;
+    }
+    base += step;
+    if (base < count) load_tile(base);
+  }
+}
+
+// zchain (nullable): the layer's chain records, kChainRec floats per panel
+// (post_chain_kernel); null: each wavefront computes its panel's chains from cin, and the
+// lead wavefront writes cout.
+template <bool LOSSY, bool TWO>
+__global__ __launch_bounds__(kBlock, 8) void panel_post_wave_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip,
+    const float* __restrict__ zchain, const float* __restrict__ cin, float* __restrict__ cout,
+    int32_t kcap, const int32_t* __restrict__ k_ptr, const float* __restrict__ din,
+    int64_t ldin_arg, const int32_t* __restrict__ ldin_cnt, float* __restrict__ dout,
+    int64_t ldout_arg, int32_t compact_out, const int32_t* __restrict__ count_ptr,
+    int32_t* __restrict__ nlist_next, const int32_t* __restrict__ map,
+    const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post,
+    const uint32_t* __restrict__ psame, int32_t ccompact) {
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int k = *k_ptr;
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int64_t ldin = ldin_cnt ? compact_ld(*ldin_cnt) : ldin_arg;
+  const int64_t ldout = compact_out ? compact_ld(count) : ldout_arg;
+  const int kst = ccompact ? compact_kst(k, kcap) : kcap;
+  if (nlist_next && p == 0 && blockIdx.y == 0 && threadIdx.x == 0) *nlist_next = (int32_t)count;
+  // without chain records workgroup row 0 always runs: it writes the centroid rows
+  if (blockIdx.y != 0 && (int64_t)blockIdx.y * kBlock >= count) return;
+  if (zchain && (int64_t)blockIdx.y * kBlock >= count) return;
+  const int r1 = TWO ? second_row(psame, p, G) : G;
+  post_wave_form<LOSSY, TWO>(G, r1, beg, pcols + (size_t)p * kSlots, outs, w, bias_vec, bias_s,
+                             clip, zchain ? zchain + (size_t)p * kChainRec : nullptr, cin, cout,
+                             kst, k, din, ldin, dout, ldout, count, map, mcent, flags, eps_post);
+}
+
+// Centroid rows of one post layer whose panels have at most two distinct weight rows, run
+// apart from the delta rows: one wavefront per panel computes its chain record (zchain,
+// kChainRec floats per panel) and the centroid rows cout = f(chain + b) from cin (stride
+// compact_kst(k, kcap) when ccompact, else kcap).
+__global__ __launch_bounds__(kBlock) void post_chain_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const uint32_t* __restrict__ psame, const float* __restrict__ bias_vec, float bias_s,
+    float clip, const float* __restrict__ cin, float* __restrict__ cout, int32_t kcap,
+    const int32_t* __restrict__ k_ptr, int32_t ccompact, float* __restrict__ zchain, int64_t P) {
+  const int64_t p = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  if (p >= P) return;
+  const int lane = lane_id();
+  const int k = *k_ptr;
+  const int kst = ccompact ? compact_kst(k, kcap) : kcap;
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int r1 = second_row(psame, (int)p, G);
+  const float z = panel_chain(lane, G, r1, beg, pcols + (size_t)p * kSlots, w, cin, kst, k);
+  zchain[p * kChainRec + lane] = z;
+  panel_centroid_rows(lane, G, r1, outs, beg, bias_vec, bias_s, clip, z, cout, kst, k);
+}
+
 template <bool LOSSY>
 __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
@@ -345,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int32_t* __restrict__ count_ptr, int32_t* __restrict__ nlist_next,
     const int32_t* __restrict__ map, const int32_t* __restrict__ mcent,
     int32_t* __restrict__ flags, float eps_post, int32_t narrow,
-    const uint32_t* __restrict__ psame) {
+    const uint32_t* __restrict__ psame, int32_t ccompact) {
   constexpr int kC = kMaxCentroidsExact + 1;  // padded centroid stride in LDS
   __shared__ float4 w_s[kMaxPanelRows * kSlots / 4];
   __shared__ float x_s[kSlots * kC];
@@ -358,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   const uint32_t same = psame ? psame[p] : 0u;
   const int64_t ldin = ldin_cnt ? compact_ld(*ldin_cnt) : ldin_arg;
   const int64_t ldout = compact_out ? compact_ld(count) : ldout_arg;
+  const int kst = ccompact ? compact_kst(k, kcap) : kcap;
   if (nlist_next && p == 0 && blockIdx.y == 0 && threadIdx.x == 0) *nlist_next = (int32_t)count;
   // narrow: every workgroup of the panel takes (row, column) pairs; wide: 256 columns each.
   // Workgroup row 0 always runs: it writes the centroid rows, delta rows or not.
@@ -371,7 +590,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     if (threadIdx.x < G) o_s[threadIdx.x] = outs[beg + threadIdx.x];
     for (int q = threadIdx.x; q < kSlots * k; q += kBlock) {
       const int r = q / k, c = q % k;
-      x_s[r * kC + c] = cin[(int64_t)pc[r] * kcap + c];
+      x_s[r * kC + c] = cin[(int64_t)pc[r] * kst + c];
     }
   }
   __syncthreads();
@@ -389,7 +608,7 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
     const int o = o_s[jj];
     const float z = acc + (bias_vec ? bias_vec[o] : bias_s);
     zc_s[jj * kC + c] = z;
-    if (blockIdx.y == 0) cout[(int64_t)o * kcap + c] = act_clip(z, clip);
+    if (blockIdx.y == 0 && cout) cout[(int64_t)o * kst + c] = act_clip(z, clip);
   }
   __syncthreads();
   auto residue = [&](float acc, float z) {
@@ -465,6 +684,21 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
       if (valid) dout[(int64_t)o_s[jj] * ldout + pos] = y;
     }
     if (valid && flag) flags[pos] = 1;
+  }
+}
+
+// Centroid columns between the kcap stride and the compact stride compact_kst(k, kcap).
+__global__ __launch_bounds__(256) void centroid_restride_kernel(
+    const float* __restrict__ src, int32_t src_compact, float* __restrict__ dst,
+    int32_t dst_compact, int32_t kcap, const int32_t* __restrict__ k_ptr, int64_t N) {
+  const int k = *k_ptr;
+  const int64_t ks = compact_kst(k, kcap);
+  const int64_t ss = src_compact ? ks : kcap, ds = dst_compact ? ks : kcap;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N * k;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = q / k;
+    const int c = (int)(q - n * k);
+    dst[n * ds + c] = src[n * ss + c];
   }
 }
 
@@ -2146,8 +2380,11 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
                        const float* din, int64_t ldin, const int32_t* ldin_cnt, float* dout,
                        int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
                        const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
-                       int64_t max_cols, const LaunchShape& shape, hipStream_t s) {
+                       int64_t max_cols, const LaunchShape& shape, hipStream_t s,
+                       bool ccompact, const float* zchain) {
   SNICIT_REQUIRE(net.mfma[l] == 0, "post kernel needs row-major weights");
+  SNICIT_REQUIRE(!zchain || (net.few_rows[l] && shape.post_form >= 1),
+                 "chain records need the wavefront-local post kernel");
   SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "post kernel expects 32 centroid slots");
   const int64_t P = net.panels(l);
   const int64_t tiles = (max_cols + kBlock - 1) / kBlock;  // 256-column tiles per panel
@@ -2157,16 +2394,55 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
   const int32_t* pbeg = la.pbeg;
   const int32_t* outs = la.outs;
   const float* w = la.w;
+  // panels of at most two distinct weight rows: the wavefront-local form
+  if (net.few_rows[l] && shape.post_form >= 1) {
+#define SNICIT_POST_WAVE(LO, TWO)                                                               \
+  hipLaunchKernelGGL((panel_post_wave_kernel<LO, TWO>), grid, dim3(kBlock), 0, s, pcols, pbeg,   \
+                     outs, w, net.bias_vec(l), net.bias_scalar(l), net.clip, zchain, cin, cout,  \
+                     kcap, k, din, ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count,      \
+                     nlist_next, map, mcent, flags, eps_post, la.psame, ccompact ? 1 : 0)
+    const bool two = !net.dup_rows[l];
+    if (lossy) {
+      if (two) SNICIT_POST_WAVE(true, true); else SNICIT_POST_WAVE(true, false);
+    } else {
+      if (two) SNICIT_POST_WAVE(false, true); else SNICIT_POST_WAVE(false, false);
+    }
+#undef SNICIT_POST_WAVE
+    check_launch();
+    return;
+  }
   if (lossy)
     hipLaunchKernelGGL(panel_post_kernel<true>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.post_narrow, la.psame);
+                       mcent, flags, eps_post, shape.post_narrow, la.psame, ccompact ? 1 : 0);
   else
     hipLaunchKernelGGL(panel_post_kernel<false>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs, w,
                        net.bias_vec(l), net.bias_scalar(l), net.clip, cin, cout, kcap, k, din,
                        ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count, nlist_next, map,
-                       mcent, flags, eps_post, shape.post_narrow, la.psame);
+                       mcent, flags, eps_post, shape.post_narrow, la.psame, ccompact ? 1 : 0);
+  check_launch();
+}
+
+void launch_post_chain(const Net& net, int l, const LayerArrays& la, const float* cin,
+                       float* cout, int32_t kcap, const int32_t* k, bool ccompact, float* zchain,
+                       hipStream_t s) {
+  SNICIT_REQUIRE(net.mfma[l] == 0 && net.few_rows[l], "chain records need panels of at most "
+                 "two distinct row-major weight rows");
+  SNICIT_REQUIRE(kcap == kMaxCentroidsExact, "chain records expect 32 centroid slots");
+  const int64_t P = net.panels(l);
+  const int64_t per = kBlock / kWave;
+  hipLaunchKernelGGL(post_chain_kernel, dim3((unsigned)((P + per - 1) / per)), dim3(kBlock), 0, s,
+                     la.pcols, la.pbeg, la.outs, la.w, la.psame, net.bias_vec(l),
+                     net.bias_scalar(l), net.clip, cin, cout, kcap, k, ccompact ? 1 : 0, zchain,
+                     P);
+  check_launch();
+}
+
+void launch_centroid_restride(const float* src, bool src_compact, float* dst, bool dst_compact,
+                              int32_t kcap, const int32_t* k, int64_t N, hipStream_t s) {
+  hipLaunchKernelGGL(centroid_restride_kernel, dim3(grid1(N * kcap, 256, 4096)), dim3(256), 0, s,
+                     src, src_compact ? 1 : 0, dst, dst_compact ? 1 : 0, kcap, k, N);
   check_launch();
 }
 

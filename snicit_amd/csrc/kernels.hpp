@@ -40,6 +40,12 @@ float tol_cut(float e);
 // Against the plan's batch-capacity stride this keeps a layer's delta rows in one dense
 // region (65536 rows of 2 KB instead of rows 240 KB apart at N = 65536, 485 rows).
 __host__ __device__ inline int64_t compact_ld(int64_t c) { return ((c > 0 ? c : 1) + 31) / 32 * 32; }
+// Compact centroid stride of the post stage: k rounded up to a power of two (at most kcap).
+__host__ __device__ inline int compact_kst(int k, int kcap) {
+  int s = 1;
+  while (s < k && s < kcap) s <<= 1;
+  return s < kcap ? s : kcap;
+}
 
 // Ring of per-layer non-empty flags of the compact post layout: post layer j writes slot
 // j % kPostRing, and a compaction at most every kPostRing layers counts and clears them.
@@ -57,6 +63,11 @@ struct LaunchShape {
   int64_t post_grid_target = 2048;
   // delta-row count up to which the post kernel takes one thread per (row, column) pair
   int32_t post_narrow = 64;
+  // post layers whose panels hold at most two distinct weight rows: 0 = the LDS-staged post
+  // kernel; 1 = the wavefront-local post kernel (no LDS, no barrier), computing its panel's
+  // centroid chains itself; 2 = the wavefront-local kernel with the centroid rows computed on
+  // a second stream ahead of the delta rows (compact post layout, resident weights; else 1)
+  int32_t post_form = 2;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
   // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in
@@ -99,7 +110,19 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
                        const float* din, int64_t ldin, const int32_t* ldin_cnt, float* dout,
                        int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
                        const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
-                       int64_t max_cols, const LaunchShape& shape, hipStream_t s);
+                       int64_t max_cols, const LaunchShape& shape, hipStream_t s,
+                       bool ccompact = false, const float* zchain = nullptr);
+// Chain records of a post layer (floats per panel, see post_chain_kernel) and its centroid
+// rows cin -> cout, apart from the delta rows; layers with net.few_rows[l] only.
+constexpr int kPostChainRec = 64;
+constexpr int kChainRing = 32;
+void launch_post_chain(const Net& net, int l, const LayerArrays& la, const float* cin,
+                       float* cout, int32_t kcap, const int32_t* k, bool ccompact, float* zchain,
+                       hipStream_t s);
+// Centroid columns (k of them, N rows) copied between the kcap stride and the compact stride
+// compact_kst(k, kcap) of the post stage: src_compact / dst_compact pick each side's stride.
+void launch_centroid_restride(const float* src, bool src_compact, float* dst, bool dst_compact,
+                              int32_t kcap, const int32_t* k, int64_t N, hipStream_t s);
 // Centroid rows of a post-threshold layer: zc = W cin + b, cout = f(zc) (k columns, ld kcap).
 void launch_panel_centroid(const Net& net, int l, const LayerArrays& la, const float* cin,
                            float* cout, float* zc, int32_t kcap, const int32_t* k,

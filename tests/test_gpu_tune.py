@@ -96,3 +96,47 @@ def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
     assert abs(info2["ms_post_tuned"] - info["ms_post_tuned"]) < 1e-5
     plan2.run()
     np.testing.assert_array_equal(plan2.categories(), ref)
+
+
+def _panel_net(N, L, distinct, seed):
+    """32 rows per input set (one panel each); within a panel the rows take `distinct`
+    weight vectors, in runs of random length."""
+    rng = np.random.default_rng(seed)
+    cols = np.empty((L, N, 32), np.int32)
+    vals = np.empty((L, N, 32), np.float32)
+    for l in range(L):
+        for g in range(0, N, 32):
+            cols[l, g:g + 32] = np.sort(rng.choice(N, size=32, replace=False))
+            cuts = np.sort(rng.choice(np.arange(1, 32), size=distinct - 1, replace=False))
+            for d, (a, b) in enumerate(zip(np.r_[0, cuts], np.r_[cuts, 32])):
+                vals[l, g + a:g + b] = (rng.standard_normal(32) * 0.25).astype(np.float32)
+    return cols, vals
+
+
+@pytest.mark.parametrize("distinct,per_neuron_bias", [(1, False), (2, False), (2, True),
+                                                      (8, False)])
+def test_post_forms_give_identical_bits(ctx, distinct, per_neuron_bias):
+    """The LDS-staged post kernel, the wavefront-local one and the wavefront-local one fed by
+    the centroid stream give the oracle's activations and clustering, bit for bit; 60 post
+    layers cover several chain blocks and delta-list compactions."""
+    N, L, B, t = 1024, 64, 600, 4
+    cols, vals = _panel_net(N, L, distinct, seed=10 + distinct)
+    bias = (np.random.default_rng(5).uniform(-0.05, 0.0, (L, N)).astype(np.float32)
+            if per_neuron_bias else np.float32(-0.02))
+    y0 = nets.random_inputs(N, B, seed=9, density=0.2)
+    y0[100:160] = y0[0]
+    y0[300:320] = y0[7]
+    net = sa.Net.from_ell(ctx, cols, vals, bias)
+    r = orc.snicit_exact(cols, vals, bias, 32.0, y0, t)
+    assert max(r["nerow"]) > 64  # wide delta launches, not only the narrow path
+    for form in (sa.Plan.POST_LDS, sa.Plan.POST_WAVE, sa.Plan.POST_WAVE_CHAINS):
+        print(f"distinct rows {distinct} per-neuron bias {per_neuron_bias} post form {form}",
+              flush=True)
+        plan = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
+        plan.set_post_form(form)
+        plan.set_input(y0)
+        for _ in range(2):  # captured graphs replayed
+            plan.run()
+            np.testing.assert_array_equal(plan.activations(), r["yL"])
+            np.testing.assert_array_equal(plan.categories(), r["cat"])
+            np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
