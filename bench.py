@@ -484,6 +484,9 @@ def main():
     ap.add_argument("--sat-rps", type=float, default=None,
                     help="c5: saturated request rate to take the loads from (default: measured "
                          "by the burst of this run)")
+    ap.add_argument("--trace-steps", type=int, default=0, metavar="K",
+                    help="run one untimed and K SNICIT forwards (captured graphs) and exit: the "
+                         "command kernel traces are taken from")
     ap.add_argument("--short-rows", type=int, default=0, metavar="K",
                     help="panel grouping with and without, on a random net of K-input rows")
     args = ap.parse_args()
@@ -559,6 +562,11 @@ def main():
 
     snic = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
     snic.set_input_images(IMG_SEED, 0, B)
+    if args.trace_steps:
+        ms_s, ev_s, st_s = time_steps(snic, args.trace_steps, 1)
+        log(f"trace steps: SNICIT {ms_s:.3f} ms/step; pre {st_s.ms_pre:.3f} cluster "
+            f"{st_s.ms_cluster:.3f} post {st_s.ms_post:.3f} recover {st_s.ms_recover:.3f}")
+        return
     ms_s, ev_s, st_s, (pre_ms, post_ms) = time_steps(snic, args.steps, args.warmup, stages=True)
     cat_s = snic.categories()
     log(f"SNICIT: {ms_s:.3f} ms/step (events {ev_s:.3f}); pre {st_s.ms_pre:.3f} cluster "

@@ -713,20 +713,26 @@ void enqueue_post_compact(snicit_plan* p) {
   int orig = cluster_cur(p) ^ 1, mc = 1;
   int j0 = 0;  // first layer since the last compaction
   // Centroid branch: the centroid columns at the compact stride, then each layer's chain
-  // records and centroid rows (they depend on the centroid columns only) in blocks of
-  // kChainSync layers, up to two blocks ahead of the delta rows: delta block b waits for
-  // chain block b, and chain block b + 2 for delta block b (ring slot reuse).  Events are
-  // recorded and waited in one order, so each wait binds to the record of its own block.
+  // records and centroid rows (they depend on the centroid columns only), in blocks that
+  // end where the delta list is compacted (1, 1, 1, 1, 4, 8, ... layers, at most kPostRing),
+  // up to two blocks ahead of the delta rows: delta block b waits for chain block b, and
+  // chain block b + 2 for delta block b (ring slot reuse; two blocks span at most
+  // kChainRing layers).  Events are recorded and waited in one order, so each wait binds
+  // to the record of its own block.
   const bool chains = post_chains(p);
   hipStream_t cs = chains ? p->chain_stream : s;
-  constexpr int B = kChainRing / 2, E = 3;
-  const int nblk = (n_post + B - 1) / B;
+  constexpr int E = 3;
+  std::vector<int> blk_end;  // last layer of each block
+  for (int j = 0; j < n_post; ++j)
+    if (post_compacts_after(j, n_post)) blk_end.push_back(j);
+  const int nblk = (int)blk_end.size();
+  auto blk_begin = [&](int b) { return b == 0 ? 0 : blk_end[b - 1] + 1; };
   auto zslot = [&](int j) {
     return p->d_zchain + (size_t)(j % kChainRing) * p->chain_panels * kPostChainRec;
   };
   auto chain_block = [&](int b) {
     if (b >= 2) SNICIT_HIP(hipStreamWaitEvent(cs, p->ev_delta[(b - 2) % E], 0));
-    for (int j = b * B; j < std::min(n_post, (b + 1) * B); ++j) {
+    for (int j = blk_begin(b); j <= blk_end[b]; ++j) {
       const int l = t + j;
       launch_post_chain(*p->net, l, plan_layer_arrays(p, l), p->d_Cp[j & 1],
                         p->d_Cp[(j + 1) & 1], p->kcap, p->d_k, true, zslot(j), cs);
@@ -741,6 +747,7 @@ void enqueue_post_compact(snicit_plan* p) {
   launch_centroid_restride(p->d_C[0], false, p->d_Cp[0], true, p->kcap, p->d_k, N, cs);
   if (chains)
     for (int b = 0; b < std::min(nblk, 2); ++b) chain_block(b);
+  int cur_blk = 0;
   weights_begin(p);
   for (int j = 0; j < n_post; ++j) {
     const int l = t + j;
@@ -749,7 +756,8 @@ void enqueue_post_compact(snicit_plan* p) {
     const bool after_compaction = j > 0 && post_compacts_after(j - 1, n_post);
     const bool compact_next = post_compacts_after(j, n_post);
     int32_t* ring = p->d_ring + (int64_t)(j % kPostRing) * p->ld;
-    if (chains && j % B == 0) SNICIT_HIP(hipStreamWaitEvent(s, p->ev_chain[(j / B) % E], 0));
+    if (chains && j == blk_begin(cur_blk))
+      SNICIT_HIP(hipStreamWaitEvent(s, p->ev_chain[cur_blk % E], 0));
     prof_mark(p, l, 0);
     launch_panel_post(*p->net, l, plan_layer_arrays(p, l), false, p->d_Cp[q], p->d_Cp[q ^ 1],
                       p->kcap, p->d_k, q == 0 ? st.Da : st.Db, p->ld,
@@ -758,9 +766,12 @@ void enqueue_post_compact(snicit_plan* p) {
                       after_compaction ? p->d_mapD : nullptr, p->d_mcar[mc], ring, 0.f, p->cap,
                       p->shape, s, true, chains ? zslot(j) : nullptr);
     prof_mark(p, l, 1);
-    if (chains && (j % B == B - 1 || j == n_post - 1) && j / B + 2 < nblk) {
-      SNICIT_HIP(hipEventRecord(p->ev_delta[(j / B) % E], s));
-      chain_block(j / B + 2);
+    if (chains && j == blk_end[cur_blk]) {
+      if (cur_blk + 2 < nblk) {
+        SNICIT_HIP(hipEventRecord(p->ev_delta[cur_blk % E], s));
+        chain_block(cur_blk + 2);
+      }
+      ++cur_blk;
     }
     prof_mark(p, l, 2);
     prof_mark(p, l, 3);
@@ -1584,8 +1595,8 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         if (few && maxP > 0) {
           p->chain_panels = maxP;
           p->d_zchain = dev_alloc<float>((size_t)kChainRing * maxP * kPostChainRec, acct);
-          p->ev_chain.assign(kChainRing + 2, nullptr);
-          p->ev_delta.assign(kChainRing + 2, nullptr);
+          p->ev_chain.assign(4, nullptr);  // three blocks in flight + the fork / join
+          p->ev_delta.assign(4, nullptr);
           for (auto* v : {&p->ev_chain, &p->ev_delta})
             for (auto& e : *v) SNICIT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
           // default priority, like every stream of the library

@@ -115,7 +115,8 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
 // Chain records of a post layer (floats per panel, see post_chain_kernel) and its centroid
 // rows cin -> cout, apart from the delta rows; layers with net.few_rows[l] only.
 constexpr int kPostChainRec = 64;
-constexpr int kChainRing = 32;
+constexpr int kChainRing = 128;  // two chain blocks of at most kPostRing layers
+static_assert(kChainRing >= 2 * kPostRing, "the chain ring holds two blocks");
 void launch_post_chain(const Net& net, int l, const LayerArrays& la, const float* cin,
                        float* cout, int32_t kcap, const int32_t* k, bool ccompact, float* zchain,
                        hipStream_t s);

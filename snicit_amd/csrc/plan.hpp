@@ -54,9 +54,10 @@ struct snicit_plan {
   // compact_kst(k, kcap), ping-pong (d_C[1] receives the last layer's at stride kcap)
   float* d_Cp[2] = {nullptr, nullptr};
   // centroid branch of the post stage (layers whose panels have at most two distinct weight
-  // rows): post_chain_kernel runs on chain_stream, up to kChainRing layers ahead of the
+  // rows): post_chain_kernel runs on chain_stream, up to two blocks of layers ahead of the
   // delta rows, writing layer j's chain records into ring slot j mod kChainRing of d_zchain;
-  // ev_chain[j mod (kChainRing + 1)] marks chains ready, ev_delta[...] the slot free again
+  // ev_chain[b mod 3] marks block b's chains ready, ev_delta[b mod 3] its slots free again,
+  // index 3 of each forks and joins the stage
   hipStream_t chain_stream = nullptr;
   float* d_zchain = nullptr;  // [kChainRing][chain_panels][kPostChainRec]
   int64_t chain_panels = 0;   // most panels of a post layer
