@@ -665,7 +665,7 @@ def main():
         "categories_identical_snicit_vs_plain": same,
         "stages_ms": {"pre": st_s.ms_pre, "cluster": st_s.ms_cluster, "post": st_s.ms_post,
                       "recover": st_s.ms_recover, "events_total": ev_s},
-        "snicit": {"live_at_threshold": st_s.live_at_threshold, "centroids": k,
+        "snicit": {"live_at_threshold": st_s.live_at_threshold, "centroids": st_s.centroids,
                    "nerow_first": st_s.nerow_first, "nerow_last": st_s.nerow_last,
                    "active": st_s.active},
         "roofline": roofs[dom],

@@ -361,7 +361,9 @@ __device__ __forceinline__ int second_row(const uint32_t* __restrict__ psame, in
 // Centroid chains of one panel on one wavefront: lane d * 32 + c runs the chain of distinct
 // row d (row 0, or row r1) and centroid c over the panel's 32 inputs of centroid column c
 // (cin, stride kst), in slot order; lanes past the panel's rows or centroids run a clamped
-// copy whose value is never taken.  Eight inputs are loaded at a time.
+// copy whose value is never taken.  CHUNK inputs are loaded at a time (8 where the caller
+// holds the tile's gathers meanwhile, 32 otherwise: one round trip).
+template <int CHUNK>
 __device__ __forceinline__ float panel_chain(int lane, int G, int r1, int beg,
                                              const int32_t* __restrict__ pc,
                                              const float* __restrict__ w,
@@ -371,31 +373,32 @@ __device__ __forceinline__ float panel_chain(int lane, int G, int r1, int beg,
   const float* __restrict__ wr = w + (size_t)(beg + (d ? r1 : 0)) * kSlots;
   float z = 0.f;
 #pragma unroll
-  for (int r0 = 0; r0 < kSlots; r0 += 8) {
-    float xc[8], wc[8];
+  for (int r0 = 0; r0 < kSlots; r0 += CHUNK) {
+    float xc[CHUNK], wc[CHUNK];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < CHUNK; ++r) {
       xc[r] = cin[(int64_t)pc[r0 + r] * kst + c];
       wc[r] = wr[r0 + r];
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) z = __builtin_fmaf(wc[r], xc[r], z);
+    for (int r = 0; r < CHUNK; ++r) z = __builtin_fmaf(wc[r], xc[r], z);
   }
   return z;
 }
 
 // Centroid rows of one panel from its chain record (lane d * 32 + c): cout[o][c] =
-// f(chain(d(o), c) + b(o)) for the panel's rows o, d(o) = 1 from row r1 on.
-__device__ __forceinline__ void panel_centroid_rows(int lane, int G, int r1,
-                                                    const int32_t* __restrict__ outs, int beg,
-                                                    const float* __restrict__ bias_vec,
-                                                    float bias_s, float clip, float zch,
+// f(chain(d(o), c) + b(o)) for the panel's rows o, d(o) = 1 from row r1 on.  Lane jj < G
+// holds row jj's output id (ol) and bias (bl), so the stores need no loads.
+__device__ __forceinline__ void panel_centroid_rows(int lane, int G, int r1, int ol, float bl,
+                                                    float clip, float zch,
                                                     float* __restrict__ cout, int kst, int k) {
   const int d = lane >> 5, c = lane & 31;
-  if (c >= k || (d == 1 && r1 >= G)) return;
-  for (int jj = d ? r1 : 0; jj < (d ? G : r1); ++jj) {
-    const int o = outs[beg + jj];
-    cout[(int64_t)o * kst + c] = act_clip(zch + (bias_vec ? bias_vec[o] : bias_s), clip);
+  const bool active = c < k && !(d == 1 && r1 >= G);
+  const int jb = d ? r1 : 0, je = d ? G : r1;
+  for (int jj = 0; jj < G; ++jj) {
+    const int o = __builtin_amdgcn_readlane(ol, jj);
+    const float b = readlane_f(bl, jj);
+    if (active && jj >= jb && jj < je) cout[(int64_t)o * kst + c] = act_clip(zch + b, clip);
   }
 }
 
@@ -440,9 +443,9 @@ __device__ __forceinline__ void post_wave_form(
   if (zrec) {
     zch = zrec[lane];
   } else {
-    zch = panel_chain(lane, G, r1, beg, pc, w, cin, kst, k);
+    zch = panel_chain<8>(lane, G, r1, beg, pc, w, cin, kst, k);
     if (blockIdx.y == 0 && wv == 0 && cout)
-      panel_centroid_rows(lane, G, r1, outs, beg, bias_vec, bias_s, clip, zch, cout, kst, k);
+      panel_centroid_rows(lane, G, r1, ol, bl, clip, zch, cout, kst, k);
   }
   auto residue = [&](float acc, float z) {
     float y = act_clip(z + acc, clip) - act_clip(z, clip);
@@ -547,9 +550,11 @@ __global__ __launch_bounds__(kBlock) void post_chain_kernel(
   const int kst = ccompact ? compact_kst(k, kcap) : kcap;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
   const int r1 = second_row(psame, (int)p, G);
-  const float z = panel_chain(lane, G, r1, beg, pcols + (size_t)p * kSlots, w, cin, kst, k);
+  const int ol = outs[beg + min(lane & 31, G - 1)];
+  const float bl = bias_vec ? bias_vec[ol] : bias_s;
+  const float z = panel_chain<kSlots>(lane, G, r1, beg, pcols + (size_t)p * kSlots, w, cin, kst, k);
   zchain[p * kChainRec + lane] = z;
-  panel_centroid_rows(lane, G, r1, outs, beg, bias_vec, bias_s, clip, z, cout, kst, k);
+  panel_centroid_rows(lane, G, r1, ol, bl, clip, z, cout, kst, k);
 }
 
 template <bool LOSSY>
