@@ -1531,10 +1531,10 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
     const int N = net->N, L = net->L;
     try {
       int64_t* acct = &p->device_bytes;
-      p->d_in = dev_alloc<float>((size_t)p->in_rows * p->ld, acct);
+      p->d_in = dev_alloc_big<float>((size_t)p->in_rows * p->ld, acct);
       if (net->Nin > 0)  // rows Nin..Kin-1 meet zero weights and stay zero
         SNICIT_HIP(hipMemset(p->d_in, 0, (size_t)p->in_rows * p->ld * 4));
-      if (net->has_io()) p->d_h0 = dev_alloc<float>((size_t)N * p->ld, acct);
+      if (net->has_io()) p->d_h0 = dev_alloc_big<float>((size_t)N * p->ld, acct);
       if (net->Nout > 0) p->d_logits = dev_alloc<float>((size_t)net->Nout * p->ld, acct);
       if (lossy) {
         p->d_poolrows = dev_alloc<float>((size_t)kMaxCentroidsLossy * nseg * N, acct);
@@ -1548,8 +1548,8 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_HIP(hipMemcpy(p->d_segstart, segstart.data(), (nseg + 1) * 4,
                            hipMemcpyHostToDevice));
       SNICIT_HIP(hipMemset(p->d_kseg, 0, nseg * 4));
-      p->d_y[0] = dev_alloc<float>((size_t)N * p->ld, acct);
-      p->d_y[1] = dev_alloc<float>((size_t)N * p->ld, acct);
+      p->d_y[0] = dev_alloc_big<float>((size_t)N * p->ld, acct);
+      p->d_y[1] = dev_alloc_big<float>((size_t)N * p->ld, acct);
       p->d_flags = dev_alloc<int32_t>(p->ld, acct);
       p->d_map = dev_alloc<int32_t>(p->ld, acct);
       p->d_mapD = dev_alloc<int32_t>(p->ld, acct);

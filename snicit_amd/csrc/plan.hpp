@@ -160,6 +160,24 @@ inline T* dev_alloc(size_t n, int64_t* acct = nullptr) {
   return static_cast<T*>(p);
 }
 
+// Activation buffers of 1 GiB and more: physically contiguous when the driver grants it,
+// else hipMalloc.  Each wavefront gathers its 32 input rows one buffer row (ld floats, up to
+// 240 KB) apart, so the span a layer touches depends on the address translation's reach:
+// with ordinary allocations the C3 layer stages varied by up to 11% from one process to the
+// next (where the buffers landed); contiguous buffers ran at one speed in every process.
+template <class T>
+inline T* dev_alloc_big(size_t n, int64_t* acct = nullptr) {
+  if (!g_dev_arena && n * sizeof(T) >= ((size_t)1 << 30)) {
+    void* p = nullptr;
+    if (hipExtMallocWithFlags(&p, n * sizeof(T), hipDeviceMallocContiguous) == hipSuccess && p) {
+      if (acct) *acct += (int64_t)(n * sizeof(T));
+      return static_cast<T*>(p);
+    }
+    (void)hipGetLastError();  // refused: fall back to an ordinary allocation
+  }
+  return dev_alloc<T>(n, acct);
+}
+
 inline void dev_free(void* p) {
   if (DevArena* a = g_dev_arena) {
     const char* q = static_cast<const char*>(p);
