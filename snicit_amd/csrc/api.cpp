@@ -695,59 +695,44 @@ const int32_t* post_count(const snicit_plan* p, int j) {
 // post_compacts_after names, the per-layer flags since the last compaction are counted
 // (non-empty rows after each layer) and the list is compacted.  Every layer's centroid
 // rows, delta rows and counts equal those of the per-layer compaction.
-// Whether the post stage runs its centroid rows on the chain stream: every post layer's
-// panels hold at most two distinct weight rows, and the weights are resident.
+// Whether the post stage's delta launches take chain records computed one layer ahead:
+// every post layer's panels hold at most two distinct weight rows, the records buffer
+// exists, and the weights are resident.
 bool post_chains(const snicit_plan* p) {
-  if (!p->compact_post || !p->chain_stream || p->net->streamed() || p->shape.post_form < 2)
+  if (!p->compact_post || !p->d_zchain || p->net->streamed() || p->shape.post_form < 2)
     return false;
   for (int l = p->cfg.threshold; l < p->net->L; ++l)
     if (!p->net->few_rows[l]) return false;
   return true;
 }
 
+// Stage "post" in the compact layout: one fused launch per layer, reading and writing the
+// delta rows at fixed list positions with stride compact_ld(positions); at the layers
+// post_compacts_after names, the per-layer flags since the last compaction are counted
+// (non-empty rows after each layer) and the list is compacted.  Every layer's centroid
+// rows, delta rows and counts equal those of the per-layer compaction.
+// With chain records (post_chains): the centroid rows depend on the centroid columns only,
+// so a post_chain_kernel launch computes layer 0's chain records and layer 1's centroid
+// columns, and the delta launch of layer j computes, beside its delta rows, layer j+1's
+// records and layer j+2's centroid columns (records in two slots, centroid columns in two
+// buffers, alternating).
 void enqueue_post_compact(snicit_plan* p) {
   hipStream_t s = p->stream;
-  const int t = p->cfg.threshold, L = p->net->L, n_post = L - t;
-  const int N = p->net->N;
+  const Net& net = *p->net;
+  const int t = p->cfg.threshold, L = net.L, n_post = L - t;
+  const int N = net.N;
   SnicitState st = snicit_state(p);
   int orig = cluster_cur(p) ^ 1, mc = 1;
   int j0 = 0;  // first layer since the last compaction
-  // Centroid branch: the centroid columns at the compact stride, then each layer's chain
-  // records and centroid rows (they depend on the centroid columns only), in blocks that
-  // end where the delta list is compacted (1, 1, 1, 1, 4, 8, ... layers, at most kPostRing),
-  // up to two blocks ahead of the delta rows: delta block b waits for chain block b, and
-  // chain block b + 2 for delta block b (ring slot reuse; two blocks span at most
-  // kChainRing layers).  Events are recorded and waited in one order, so each wait binds
-  // to the record of its own block.
   const bool chains = post_chains(p);
-  hipStream_t cs = chains ? p->chain_stream : s;
-  constexpr int E = 3;
-  std::vector<int> blk_end;  // last layer of each block
-  for (int j = 0; j < n_post; ++j)
-    if (post_compacts_after(j, n_post)) blk_end.push_back(j);
-  const int nblk = (int)blk_end.size();
-  auto blk_begin = [&](int b) { return b == 0 ? 0 : blk_end[b - 1] + 1; };
   auto zslot = [&](int j) {
-    return p->d_zchain + (size_t)(j % kChainRing) * p->chain_panels * kPostChainRec;
+    return p->d_zchain + (size_t)(j & 1) * p->chain_panels * kPostChainRec;
   };
-  auto chain_block = [&](int b) {
-    if (b >= 2) SNICIT_HIP(hipStreamWaitEvent(cs, p->ev_delta[(b - 2) % E], 0));
-    for (int j = blk_begin(b); j <= blk_end[b]; ++j) {
-      const int l = t + j;
-      launch_post_chain(*p->net, l, plan_layer_arrays(p, l), p->d_Cp[j & 1],
-                        p->d_Cp[(j + 1) & 1], p->kcap, p->d_k, true, zslot(j), cs);
-    }
-    SNICIT_HIP(hipEventRecord(p->ev_chain[b % E], cs));
-  };
-  if (chains) {
-    SNICIT_HIP(hipEventRecord(p->ev_chain[E], s));
-    SNICIT_HIP(hipStreamWaitEvent(cs, p->ev_chain[E], 0));
-  }
   // centroid columns at the compact stride for the post layers, back at kcap in d_C[1]
-  launch_centroid_restride(p->d_C[0], false, p->d_Cp[0], true, p->kcap, p->d_k, N, cs);
-  if (chains)
-    for (int b = 0; b < std::min(nblk, 2); ++b) chain_block(b);
-  int cur_blk = 0;
+  launch_centroid_restride(p->d_C[0], false, p->d_Cp[0], true, p->kcap, p->d_k, N, s);
+  if (chains && n_post > 0)
+    launch_post_chain(net, t, plan_layer_arrays(p, t), p->d_Cp[0], p->d_Cp[1], p->kcap, p->d_k,
+                      true, zslot(0), s);
   weights_begin(p);
   for (int j = 0; j < n_post; ++j) {
     const int l = t + j;
@@ -756,23 +741,29 @@ void enqueue_post_compact(snicit_plan* p) {
     const bool after_compaction = j > 0 && post_compacts_after(j - 1, n_post);
     const bool compact_next = post_compacts_after(j, n_post);
     int32_t* ring = p->d_ring + (int64_t)(j % kPostRing) * p->ld;
-    if (chains && j == blk_begin(cur_blk))
-      SNICIT_HIP(hipStreamWaitEvent(s, p->ev_chain[cur_blk % E], 0));
+    NextChain nx;
+    if (chains && j + 1 < n_post) {
+      const LayerArrays la = plan_layer_arrays(p, l + 1);
+      nx.pcols = la.pcols;
+      nx.pbeg = la.pbeg;
+      nx.outs = la.outs;
+      nx.w = la.w;
+      nx.psame = la.psame;
+      nx.bias_vec = net.bias_vec(l + 1);
+      nx.bias_s = net.bias_scalar(l + 1);
+      nx.cin = p->d_Cp[(j + 1) & 1];
+      nx.cout = p->d_Cp[j & 1];
+      nx.zchain = zslot(j + 1);
+      nx.P = net.panels(l + 1);
+    }
     prof_mark(p, l, 0);
-    launch_panel_post(*p->net, l, plan_layer_arrays(p, l), false, p->d_Cp[q], p->d_Cp[q ^ 1],
+    launch_panel_post(net, l, plan_layer_arrays(p, l), false, p->d_Cp[q], p->d_Cp[q ^ 1],
                       p->kcap, p->d_k, q == 0 ? st.Da : st.Db, p->ld,
                       post_count(p, j == 0 ? 0 : j - 1), q == 0 ? st.Db : st.Da, p->ld, true,
                       post_count(p, j), compact_next ? nullptr : p->d_nlist + l + 1,
                       after_compaction ? p->d_mapD : nullptr, p->d_mcar[mc], ring, 0.f, p->cap,
-                      p->shape, s, true, chains ? zslot(j) : nullptr);
+                      p->shape, s, true, chains ? zslot(j) : nullptr, nx);
     prof_mark(p, l, 1);
-    if (chains && j == blk_end[cur_blk]) {
-      if (cur_blk + 2 < nblk) {
-        SNICIT_HIP(hipEventRecord(p->ev_delta[cur_blk % E], s));
-        chain_block(cur_blk + 2);
-      }
-      ++cur_blk;
-    }
     prof_mark(p, l, 2);
     prof_mark(p, l, 3);
     done_layer(p, l);
@@ -787,11 +778,7 @@ void enqueue_post_compact(snicit_plan* p) {
       j0 = j + 1;
     }
   }
-  launch_centroid_restride(p->d_Cp[n_post & 1], true, p->d_C[1], false, p->kcap, p->d_k, N, cs);
-  if (chains) {
-    SNICIT_HIP(hipEventRecord(p->ev_delta[E], cs));
-    SNICIT_HIP(hipStreamWaitEvent(s, p->ev_delta[E], 0));
-  }
+  launch_centroid_restride(p->d_Cp[n_post & 1], true, p->d_C[1], false, p->kcap, p->d_k, N, s);
 }
 
 // Stage "post" (src/SDGC/SNICIT/SNICIT.hpp:526-538,603-614): per layer the centroid rows,
@@ -1102,12 +1089,7 @@ void plan_free(snicit_plan* p) {
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->copy_stream) (void)hipStreamDestroy(p->copy_stream);
   p->copy_stream = nullptr;
-  if (p->chain_stream) (void)hipStreamSynchronize(p->chain_stream);
-  for (auto* v : {&p->ev_chain, &p->ev_delta})
-    for (auto& e : *v)
-      if (e) (void)hipEventDestroy(e);
-  if (p->chain_stream) (void)hipStreamDestroy(p->chain_stream);
-  p->chain_stream = nullptr;
+
   for (auto& e : p->prof_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : p->ev)
@@ -1585,8 +1567,8 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
           p->d_Cp[i] = dev_alloc<float>((size_t)N * p->kcap, acct);
           SNICIT_HIP(hipMemset(p->d_Cp[i], 0, (size_t)N * p->kcap * 4));
         }
-        // centroid branch (not for the allocation-free entry point, which creates nothing)
-        bool few = !g_dev_arena;
+        // chain records of two post layers (panels with at most two distinct weight rows)
+        bool few = true;
         int64_t maxP = 0;
         for (int l = cfg->threshold; l < L; ++l) {
           few = few && net->few_rows[l];
@@ -1594,13 +1576,7 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         }
         if (few && maxP > 0) {
           p->chain_panels = maxP;
-          p->d_zchain = dev_alloc<float>((size_t)kChainRing * maxP * kPostChainRec, acct);
-          p->ev_chain.assign(4, nullptr);  // three blocks in flight + the fork / join
-          p->ev_delta.assign(4, nullptr);
-          for (auto* v : {&p->ev_chain, &p->ev_delta})
-            for (auto& e : *v) SNICIT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-          // default priority, like every stream of the library
-          SNICIT_HIP(hipStreamCreateWithFlags(&p->chain_stream, hipStreamNonBlocking));
+          p->d_zchain = dev_alloc<float>((size_t)2 * maxP * kPostChainRec, acct);
         }
         p->d_nlist = dev_alloc<int32_t>(L + 2, acct);
         p->d_ring = dev_alloc<int32_t>((size_t)kPostRing * p->ld, acct);

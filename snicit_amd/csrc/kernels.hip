@@ -30,6 +30,18 @@ __device__ __forceinline__ float act_clip(float z, float clip) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// 256-column tiles of a launch over `count` list positions, split into gridDim.y contiguous
+// runs: workgroup row blockIdx.y takes positions [p0, p1).  Neighbouring tiles share a
+// 128-byte line wherever the column list is shifted; in one run they are read back to back
+// by one workgroup, so the line is fetched once.
+__device__ __forceinline__ void tile_run(int64_t count, int64_t& p0, int64_t& p1) {
+  const int64_t tiles = (count + kBlock - 1) / kBlock;
+  const int64_t per = (tiles + gridDim.y - 1) / gridDim.y;
+  const int64_t t0 = min(tiles, (int64_t)blockIdx.y * per);
+  p0 = t0 * kBlock;
+  p1 = min(count, (t0 + per) * kBlock);
+}
+
 // Value of v in lane l (l wavefront-uniform).
 __device__ __forceinline__ float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -146,8 +158,9 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   }
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
-       base += (int64_t)gridDim.y * kBlock) {
+  int64_t run0, run1;
+  tile_run(count, run0, run1);
+  for (int64_t base = run0 + wv * kWave; base < run1; base += kBlock) {
     const int64_t pos = base + lane;
     const bool valid = pos < count;
     float x[kSlots];
@@ -233,8 +246,9 @@ __global__ __launch_bounds__(kBlock, 5) void panel_plain_mfma_kernel(
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int half = lane >> 5, jl = lane & 31;
-  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
-       base += (int64_t)gridDim.y * kBlock) {
+  int64_t run0, run1;
+  tile_run(count, run0, run1);
+  for (int64_t base = run0 + wv * kWave; base < run1; base += kBlock) {
     const int64_t pos = base + lane;
     const bool valid = pos < count;
     // columns past the count read column 0 (count > 0 here) and are zeroed
@@ -501,11 +515,37 @@ __device__ __forceinline__ void post_wave_form(
   }
 }
 
-// zchain (nullable): the layer's chain records, kChainRec floats per panel
-// (post_chain_kernel); null: each wavefront computes its panel's chains from cin, and the
-// lead wavefront writes cout.
+// Chain record and centroid rows of panel p of a post layer on one wavefront (see
+// panel_chain): zchain[p] = the panel's chains, cout = f(chain + b) from cin (stride kst).
+__device__ __forceinline__ void chain_panel(int64_t p, const int32_t* __restrict__ pcols,
+                                            const int32_t* __restrict__ pbeg,
+                                            const int32_t* __restrict__ outs,
+                                            const float* __restrict__ w,
+                                            const uint32_t* __restrict__ psame,
+                                            const float* __restrict__ bias_vec, float bias_s,
+                                            float clip, const float* __restrict__ cin,
+                                            float* __restrict__ cout, int kst, int k,
+                                            float* __restrict__ zchain) {
+  const int lane = lane_id();
+  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
+  const int r1 = second_row(psame, (int)p, G);
+  const int ol = outs[beg + min(lane & 31, G - 1)];
+  const float bl = bias_vec ? bias_vec[ol] : bias_s;
+  const float z = panel_chain<kSlots>(lane, G, r1, beg, pcols + (size_t)p * kSlots, w, cin, kst, k);
+  zchain[p * kChainRec + lane] = z;
+  panel_centroid_rows(lane, G, r1, ol, bl, clip, z, cout, kst, k);
+}
+
+// zchain (nullable): the layer's chain records, kChainRec floats per panel (from the launch
+// of the layer before, or post_chain_kernel); null: each wavefront computes its panel's
+// chains from cin, and the lead wavefront writes cout.
+// nx (nx.pbeg non-null): the next post layer's chain records and centroid rows, computed
+// here by a fifth wavefront (workgroups of kBlock + 64 threads) in workgroup row 0 of each
+// panel, beside the four that stream the delta rows (grid.x = the larger of the two layers'
+// panel counts; P = this layer's).  They depend on the centroid columns only and the next
+// launch needs them, so no layer after the first needs a launch of its own for them.
 template <bool LOSSY, bool TWO>
-__global__ __launch_bounds__(kBlock, 8) void panel_post_wave_kernel(
+__global__ __launch_bounds__(kBlock + kWave, 8) void panel_post_wave_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
     const int32_t* __restrict__ outs, const float* __restrict__ w,
     const float* __restrict__ bias_vec, float bias_s, float clip,
@@ -515,14 +555,21 @@ __global__ __launch_bounds__(kBlock, 8) void panel_post_wave_kernel(
     int64_t ldout_arg, int32_t compact_out, const int32_t* __restrict__ count_ptr,
     int32_t* __restrict__ nlist_next, const int32_t* __restrict__ map,
     const int32_t* __restrict__ mcent, int32_t* __restrict__ flags, float eps_post,
-    const uint32_t* __restrict__ psame, int32_t ccompact) {
+    const uint32_t* __restrict__ psame, int32_t ccompact, int64_t P, NextChain nx) {
   const int p = blockIdx.x;
-  const int64_t count = *count_ptr;
   const int k = *k_ptr;
+  const int kst = ccompact ? compact_kst(k, kcap) : kcap;
+  if (threadIdx.x >= kBlock) {  // the fifth wavefront: next layer's chains only
+    if (nx.pbeg && blockIdx.y == 0 && p < nx.P)
+      chain_panel(p, nx.pcols, nx.pbeg, nx.outs, nx.w, nx.psame, nx.bias_vec, nx.bias_s, clip,
+                  nx.cin, nx.cout, kst, k, nx.zchain);
+    return;
+  }
+  if (p >= P) return;
+  const int64_t count = *count_ptr;
   const int beg = pbeg[p], G = pbeg[p + 1] - beg;
   const int64_t ldin = ldin_cnt ? compact_ld(*ldin_cnt) : ldin_arg;
   const int64_t ldout = compact_out ? compact_ld(count) : ldout_arg;
-  const int kst = ccompact ? compact_kst(k, kcap) : kcap;
   if (nlist_next && p == 0 && blockIdx.y == 0 && threadIdx.x == 0) *nlist_next = (int32_t)count;
   // without chain records workgroup row 0 always runs: it writes the centroid rows
   if (blockIdx.y != 0 && (int64_t)blockIdx.y * kBlock >= count) return;
@@ -533,9 +580,9 @@ __global__ __launch_bounds__(kBlock, 8) void panel_post_wave_kernel(
                              kst, k, din, ldin, dout, ldout, count, map, mcent, flags, eps_post);
 }
 
-// Centroid rows of one post layer whose panels have at most two distinct weight rows, run
-// apart from the delta rows: one wavefront per panel computes its chain record (zchain,
-// kChainRec floats per panel) and the centroid rows cout = f(chain + b) from cin (stride
+// Centroid rows of one post layer whose panels have at most two distinct weight rows, apart
+// from the delta rows: one wavefront per panel computes its chain record (zchain, kChainRec
+// floats per panel) and the centroid rows cout = f(chain + b) from cin (stride
 // compact_kst(k, kcap) when ccompact, else kcap).
 __global__ __launch_bounds__(kBlock) void post_chain_kernel(
     const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
@@ -545,16 +592,9 @@ __global__ __launch_bounds__(kBlock) void post_chain_kernel(
     const int32_t* __restrict__ k_ptr, int32_t ccompact, float* __restrict__ zchain, int64_t P) {
   const int64_t p = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   if (p >= P) return;
-  const int lane = lane_id();
   const int k = *k_ptr;
   const int kst = ccompact ? compact_kst(k, kcap) : kcap;
-  const int beg = pbeg[p], G = pbeg[p + 1] - beg;
-  const int r1 = second_row(psame, (int)p, G);
-  const int ol = outs[beg + min(lane & 31, G - 1)];
-  const float bl = bias_vec ? bias_vec[ol] : bias_s;
-  const float z = panel_chain<kSlots>(lane, G, r1, beg, pcols + (size_t)p * kSlots, w, cin, kst, k);
-  zchain[p * kChainRec + lane] = z;
-  panel_centroid_rows(lane, G, r1, ol, bl, clip, z, cout, kst, k);
+  chain_panel(p, pcols, pbeg, outs, w, psame, bias_vec, bias_s, clip, cin, cout, kst, k, zchain);
 }
 
 template <bool LOSSY>
@@ -648,8 +688,9 @@ __global__ __launch_bounds__(kBlock) void panel_post_kernel(
   }
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
-       base += (int64_t)gridDim.y * kBlock) {
+  int64_t run0, run1;
+  tile_run(count, run0, run1);
+  for (int64_t base = run0 + wv * kWave; base < run1; base += kBlock) {
     const int64_t pos = base + lane;
     const bool valid = pos < count;
     float x[kSlots];
@@ -836,8 +877,9 @@ __global__ __launch_bounds__(kBlock) void ell_spmm_kernel(
   }
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
-  for (int64_t base = (int64_t)blockIdx.y * kBlock + wv * kWave; base < count;
-       base += (int64_t)gridDim.y * kBlock) {
+  int64_t run0, run1;
+  tile_run(count, run0, run1);
+  for (int64_t base = run0 + wv * kWave; base < run1; base += kBlock) {
     const int64_t pos = base + lane;
     const bool valid = pos < count;
     const int64_t src = valid ? (map ? (int64_t)map[pos] : pos) : 0;
@@ -2386,7 +2428,7 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
                        int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
                        const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
                        int64_t max_cols, const LaunchShape& shape, hipStream_t s,
-                       bool ccompact, const float* zchain) {
+                       bool ccompact, const float* zchain, const NextChain& nx) {
   SNICIT_REQUIRE(net.mfma[l] == 0, "post kernel needs row-major weights");
   SNICIT_REQUIRE(!zchain || (net.few_rows[l] && shape.post_form >= 1),
                  "chain records need the wavefront-local post kernel");
@@ -2400,12 +2442,17 @@ void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
   const int32_t* outs = la.outs;
   const float* w = la.w;
   // panels of at most two distinct weight rows: the wavefront-local form
+  SNICIT_REQUIRE(!nx.pbeg || (zchain && net.few_rows[l] && shape.post_form >= 1),
+                 "the next layer's chains need this layer's chain records");
   if (net.few_rows[l] && shape.post_form >= 1) {
+    if (nx.pbeg) grid.x = (unsigned)std::max<int64_t>(P, nx.P);
 #define SNICIT_POST_WAVE(LO, TWO)                                                               \
-  hipLaunchKernelGGL((panel_post_wave_kernel<LO, TWO>), grid, dim3(kBlock), 0, s, pcols, pbeg,   \
+  hipLaunchKernelGGL((panel_post_wave_kernel<LO, TWO>), grid, dim3(nx.pbeg ? kBlock + kWave : kBlock), \
+                     0, s, pcols, pbeg,                                                           \
                      outs, w, net.bias_vec(l), net.bias_scalar(l), net.clip, zchain, cin, cout,  \
                      kcap, k, din, ldin, ldin_cnt, dout, ldout, compact_out ? 1 : 0, count,      \
-                     nlist_next, map, mcent, flags, eps_post, la.psame, ccompact ? 1 : 0)
+                     nlist_next, map, mcent, flags, eps_post, la.psame, ccompact ? 1 : 0, P,  \
+                     nx)
     const bool two = !net.dup_rows[l];
     if (lossy) {
       if (two) SNICIT_POST_WAVE(true, true); else SNICIT_POST_WAVE(true, false);

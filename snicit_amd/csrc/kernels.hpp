@@ -105,18 +105,32 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s);
 // compact_out writes dout with stride compact_ld(*count), and nlist_next (nullable) receives
 // *count for the next layer.
 bool post_layer_fused(const Net& net, int l);
+// The next post layer's chain records and centroid rows, computed inside a delta launch
+// (pbeg null: none): that layer's panel arrays, bias, centroid columns in and out, records.
+struct NextChain {
+  const int32_t* pcols = nullptr;
+  const int32_t* pbeg = nullptr;
+  const int32_t* outs = nullptr;
+  const float* w = nullptr;
+  const uint32_t* psame = nullptr;
+  const float* bias_vec = nullptr;
+  float bias_s = 0.f;
+  const float* cin = nullptr;
+  float* cout = nullptr;
+  float* zchain = nullptr;
+  int64_t P = 0;
+};
 void launch_panel_post(const Net& net, int l, const LayerArrays& la, bool lossy,
                        const float* cin, float* cout, int32_t kcap, const int32_t* k,
                        const float* din, int64_t ldin, const int32_t* ldin_cnt, float* dout,
                        int64_t ldout, bool compact_out, const int32_t* count, int32_t* nlist_next,
                        const int32_t* map, const int32_t* mcent, int32_t* flags, float eps_post,
                        int64_t max_cols, const LaunchShape& shape, hipStream_t s,
-                       bool ccompact = false, const float* zchain = nullptr);
+                       bool ccompact = false, const float* zchain = nullptr,
+                       const NextChain& nx = NextChain());
 // Chain records of a post layer (floats per panel, see post_chain_kernel) and its centroid
 // rows cin -> cout, apart from the delta rows; layers with net.few_rows[l] only.
 constexpr int kPostChainRec = 64;
-constexpr int kChainRing = 128;  // two chain blocks of at most kPostRing layers
-static_assert(kChainRing >= 2 * kPostRing, "the chain ring holds two blocks");
 void launch_post_chain(const Net& net, int l, const LayerArrays& la, const float* cin,
                        float* cout, int32_t kcap, const int32_t* k, bool ccompact, float* zchain,
                        hipStream_t s);

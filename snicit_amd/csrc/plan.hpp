@@ -53,15 +53,10 @@ struct snicit_plan {
   // compact post layout: centroid columns of the post layers at the compact stride
   // compact_kst(k, kcap), ping-pong (d_C[1] receives the last layer's at stride kcap)
   float* d_Cp[2] = {nullptr, nullptr};
-  // centroid branch of the post stage (layers whose panels have at most two distinct weight
-  // rows): post_chain_kernel runs on chain_stream, up to two blocks of layers ahead of the
-  // delta rows, writing layer j's chain records into ring slot j mod kChainRing of d_zchain;
-  // ev_chain[b mod 3] marks block b's chains ready, ev_delta[b mod 3] its slots free again,
-  // index 3 of each forks and joins the stage
-  hipStream_t chain_stream = nullptr;
-  float* d_zchain = nullptr;  // [kChainRing][chain_panels][kPostChainRec]
+  // chain records of the post stage (layers whose panels have at most two distinct weight
+  // rows): layer j's in slot j mod 2, written by the launch of layer j - 1
+  float* d_zchain = nullptr;  // [2][chain_panels][kPostChainRec]
   int64_t chain_panels = 0;   // most panels of a post layer
-  std::vector<hipEvent_t> ev_chain, ev_delta;
   float* d_zc = nullptr;
   int32_t* d_rec = nullptr;
   int32_t* d_dist = nullptr;  // [cap][stride] distance accumulators (kept zero between runs)
