@@ -237,34 +237,43 @@ def bench_batch(args, wl):
 
 
 def short_rows(K):
-    """Panel grouping on a random net whose rows have K < 32 inputs (N=16384, 60 layers,
-    60000 samples at density 0.3): plain BF and SNICIT (t=20) ms/step with rows grouped into
-    shared panels (the default) and with one input set per panel (SNICIT_PANEL_MERGE=0);
-    categories must agree."""
+    """Panel grouping on nets whose rows have K < 32 inputs (N=16384, 60 layers, 60000
+    samples at density 0.3): plain BF and SNICIT (t=20) ms/step under each grouping (none,
+    consecutive, search), on random rows and on banded rows listed in a shuffled order (row o
+    reads o .. o+K-1); categories must agree across groupings."""
     import snicit_amd as sa
 
     N, L, B = 16384, 60, 60000
     rng = np.random.default_rng(5)
-    cols = rng.integers(0, N, (L, N, K)).astype(np.int32)
+    random_cols = rng.integers(0, N, (L, N, K)).astype(np.int32)
+    band = (np.arange(N)[:, None] + np.arange(K)[None, :]) % N
+    banded_cols = np.stack([band[rng.permutation(N)] for _ in range(L)]).astype(np.int32)
     vals = (rng.standard_normal((L, N, K)) * np.sqrt(2.0 / K)).astype(np.float32)
     y0 = (rng.random((B, N)) < 0.3).astype(np.float32)
     ctx = sa.Context(0)
-    cats = {}
-    for merge in ("1", "0"):
-        os.environ["SNICIT_PANEL_MERGE"] = merge
-        net = sa.Net.from_ell(ctx, cols, vals, -0.01)
-        for mode in ("BF", "SNICIT"):
-            plan = sa.Plan(net, sa.default_cfg(mode, threshold=20), B)
-            plan.set_input(y0)
-            ms, _, _ = time_steps(plan, 3, 1)
-            cats[(merge, mode)] = plan.categories().copy()
-            log(f"K={K} grouping={'on' if merge == '1' else 'off'} panels={net.panels} "
-                f"{mode}: {ms:.2f} ms/step")
-            plan.close()
-        net.close()
-    os.environ.pop("SNICIT_PANEL_MERGE", None)
-    same = all(np.array_equal(cats[("1", m)], cats[("0", m)]) for m in ("BF", "SNICIT"))
-    log(f"categories identical with and without grouping: {same}")
+    names = {ctx.GROUP_NONE: "none", ctx.GROUP_CONSECUTIVE: "consecutive",
+             ctx.GROUP_SEARCH: "search"}
+    same = True
+    for kind, cols in (("random", random_cols), ("banded, shuffled", banded_cols)):
+        cats = {}
+        for grouping in (ctx.GROUP_NONE, ctx.GROUP_CONSECUTIVE, ctx.GROUP_SEARCH):
+            ctx.set_grouping(grouping)
+            t0 = time.perf_counter()
+            net = sa.Net.from_ell(ctx, cols, vals, -0.01)
+            pack_s = time.perf_counter() - t0
+            for mode in ("BF", "SNICIT"):
+                plan = sa.Plan(net, sa.default_cfg(mode, threshold=20), B)
+                plan.set_input(y0)
+                ms, _, _ = time_steps(plan, 3, 1)
+                cats[(grouping, mode)] = plan.categories().copy()
+                log(f"K={K} {kind} rows, grouping {names[grouping]}: panels={net.panels} "
+                    f"(net built in {pack_s:.1f} s) {mode}: {ms:.2f} ms/step")
+                plan.close()
+            net.close()
+        same = same and all(np.array_equal(cats[(g, m)], cats[(ctx.GROUP_NONE, m)])
+                            for g in names for m in ("BF", "SNICIT"))
+    ctx.set_grouping(ctx.GROUP_SEARCH)
+    log(f"categories identical under every grouping: {same}")
 
 
 def stream_ceiling():

@@ -106,6 +106,18 @@ typedef struct snicit_stats {
  * HIP stream (default priority). */
 int snicit_ctx_create(int device, snicit_ctx** out);
 int snicit_ctx_destroy(snicit_ctx* ctx);
+
+/* Panel grouping of rows with fewer than 32 inputs in the nets this context creates next
+ * (the role of the reference's offline inspector, src/SDGC/inspector/gpu_block_scheduler.h:
+ * 58-171, which merges row blocks by input reuse): NONE keeps one input set per panel;
+ * CONSECUTIVE lets each group of rows join the panel before it when their input sets
+ * together hold at most 32 inputs; SEARCH (default) seeds a panel and adds the groups that
+ * share the most inputs with it, at any distance, then fills what room is left in order.
+ * Every grouping gives the same results, bit for bit. */
+#define SNICIT_GROUP_NONE 0
+#define SNICIT_GROUP_CONSECUTIVE 1
+#define SNICIT_GROUP_SEARCH 2
+int snicit_ctx_set_grouping(snicit_ctx* ctx, int32_t grouping);
 const char* snicit_last_error(void);
 void snicit_run_cfg_default(snicit_run_cfg* cfg);
 

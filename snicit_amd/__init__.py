@@ -123,6 +123,7 @@ _I64P = ctypes.POINTER(ctypes.c_int64)
 SIGNATURES = {
     "snicit_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
     "snicit_ctx_destroy": (ctypes.c_int, [_P]),
+    "snicit_ctx_set_grouping": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_last_error": (ctypes.c_char_p, []),
     "snicit_run_cfg_default": (None, [ctypes.POINTER(RunCfg)]),
     "snicit_net_create_ell": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -282,6 +283,12 @@ class Context:
         h = _P()
         _check(lib().snicit_ctx_create(device, ctypes.byref(h)))
         self.h = h
+
+    GROUP_NONE, GROUP_CONSECUTIVE, GROUP_SEARCH = 0, 1, 2
+
+    def set_grouping(self, grouping):
+        """Panel grouping of short rows in the nets created next (see snicit_ctx_set_grouping)."""
+        _check(lib().snicit_ctx_set_grouping(self.h, grouping))
 
     def close(self):
         if self.h:

@@ -1148,6 +1148,16 @@ int snicit_ctx_create(int device, snicit_ctx** out) {
   });
 }
 
+int snicit_ctx_set_grouping(snicit_ctx* ctx, int32_t grouping) {
+  return guarded([&] {
+    SNICIT_REQUIRE(ctx != nullptr, "null context");
+    SNICIT_REQUIRE(grouping == SNICIT_GROUP_NONE || grouping == SNICIT_GROUP_CONSECUTIVE ||
+                       grouping == SNICIT_GROUP_SEARCH,
+                   "grouping must be SNICIT_GROUP_NONE, _CONSECUTIVE or _SEARCH");
+    ctx->grouping = grouping;
+  });
+}
+
 int snicit_ctx_destroy(snicit_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
@@ -1181,7 +1191,7 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, cons
         nb.add_ell(cols + (size_t)l * N * K, vals + (size_t)l * N * K, K);
         continue;
       }
-      pack_layer_ell(N, K, cols + (size_t)l * N * K, vals + (size_t)l * N * K, hp);
+      pack_layer_ell(N, K, cols + (size_t)l * N * K, vals + (size_t)l * N * K, hp, ctx->grouping);
       nb.add(hp);
     }
     try {

@@ -89,6 +89,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int32_t grouping = SNICIT_GROUP_SEARCH;  // short-row panel grouping of nets created next
 };
 
 struct Net {
@@ -209,7 +210,8 @@ void dense_layer(int32_t rows, int32_t fan_in, uint64_t seed, uint64_t tag, floa
 void uniform_features(int32_t Nin, uint64_t seed, int64_t first, int64_t B, float* out);
 
 // ---- packing (net.cpp) -----------------------------------------------------------
+// grouping: SNICIT_GROUP_NONE / _CONSECUTIVE / _SEARCH for rows with fewer than 32 inputs.
 void pack_layer_ell(int32_t N, int32_t K, const int32_t* cols, const float* vals,
-                    HostLayerPanels& out);
+                    HostLayerPanels& out, int32_t grouping);
 
 }  // namespace snicit

@@ -54,9 +54,11 @@ int main() {
       std::vector<float> vals(cols.size(), 0.0625f);
       for (int32_t o = 0; o < N; ++o)
         for (int r = 0; r < 32; ++r) CHECK(cols[(size_t)o * 32 + r] >= 0 && cols[(size_t)o * 32 + r] < N);
-      HostLayerPanels packed;
-      pack_layer_ell(N, 32, cols.data(), vals.data(), packed);
-      check_packed(N, 32, cols, vals, packed);
+      for (int32_t grouping : {SNICIT_GROUP_NONE, SNICIT_GROUP_CONSECUTIVE, SNICIT_GROUP_SEARCH}) {
+        HostLayerPanels packed;
+        pack_layer_ell(N, 32, cols.data(), vals.data(), packed, grouping);
+        check_packed(N, 32, cols, vals, packed);
+      }
     }
     // images
     ImageGeom g = image_geometry(N, 60000);
@@ -98,9 +100,11 @@ int main() {
         vals[(size_t)o * K + q] = (o % 3 == 0) ? 0.5f : (float)((r.next() % 1000) / 250.0 - 2.0);
       }
     }
-    HostLayerPanels packed;
-    pack_layer_ell(N, K, cols.data(), vals.data(), packed);
-    check_packed(N, K, cols, vals, packed);
+    for (int32_t grouping : {SNICIT_GROUP_NONE, SNICIT_GROUP_CONSECUTIVE, SNICIT_GROUP_SEARCH}) {
+      HostLayerPanels packed;
+      pack_layer_ell(N, K, cols.data(), vals.data(), packed, grouping);
+      check_packed(N, K, cols, vals, packed);
+    }
   }
   if (failures) {
     std::fprintf(stderr, "%d checks failed\n", failures);

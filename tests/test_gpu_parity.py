@@ -299,11 +299,11 @@ def test_infer_rounds(ctx):
 
 
 @pytest.mark.parametrize("K,share", [(8, 1), (12, 3), (5, 2)])
-def test_merged_short_row_panels(ctx, K, share, monkeypatch):
+def test_merged_short_row_panels(ctx, K, share):
     """Rows with fewer than 32 inputs share panels when their input sets fit 32 slots
     (inspector-style grouping): fewer panels, and BF and exact SNICIT equal the oracle bit for
-    bit, as does the layout without merging.  One row lists an input twice; its group keeps
-    its own panel."""
+    bit under every grouping, as does the layout without merging.  One row lists an input
+    twice; its group keeps its own panel."""
     N, L, B = 1024, 8, 700
     cols, vals = nets.random_ell(N, L, K=K, seed=13, share=share)
     cols[2, 5, 1] = cols[2, 5, 0]
@@ -311,15 +311,47 @@ def test_merged_short_row_panels(ctx, K, share, monkeypatch):
     yL, _, _ = orc.plain_f32(cols, vals, -0.01, 32.0, y0)
     r = orc.snicit_exact(cols, vals, -0.01, 32.0, y0, 3)
     panels = {}
-    for merge in ("1", "0"):
-        print(f"K={K} share={share} merge={merge}", flush=True)
-        monkeypatch.setenv("SNICIT_PANEL_MERGE", merge)
-        net = sa.Net.from_ell(ctx, cols, vals, -0.01)
-        panels[merge] = net.panels
-        for mode, t in (("BF", 0), ("SNICIT", 3)):
-            plan, _ = _run(ctx, net, sa.default_cfg(mode, threshold=t), y0)
-            np.testing.assert_array_equal(plan.activations(), yL if mode == "BF" else r["yL"])
-    assert panels["1"] < panels["0"]
+    try:
+        for grouping in (ctx.GROUP_SEARCH, ctx.GROUP_CONSECUTIVE, ctx.GROUP_NONE):
+            print(f"K={K} share={share} grouping={grouping}", flush=True)
+            ctx.set_grouping(grouping)
+            net = sa.Net.from_ell(ctx, cols, vals, -0.01)
+            panels[grouping] = net.panels
+            for mode, t in (("BF", 0), ("SNICIT", 3)):
+                plan, _ = _run(ctx, net, sa.default_cfg(mode, threshold=t), y0)
+                np.testing.assert_array_equal(plan.activations(), yL if mode == "BF" else r["yL"])
+    finally:
+        ctx.set_grouping(ctx.GROUP_SEARCH)
+    assert panels[ctx.GROUP_SEARCH] <= panels[ctx.GROUP_CONSECUTIVE] < panels[ctx.GROUP_NONE]
+
+
+@pytest.mark.parametrize("K", [6, 12])
+def test_search_grouping_finds_overlaps_at_any_distance(ctx, K):
+    """Banded short rows (row o reads inputs o .. o+K-1, so neighbouring rows share most
+    inputs) listed in a shuffled order: consecutive grouping finds few merges, the search
+    packs up to 33-K rows per panel; both equal the oracle bit for bit."""
+    N, L, B = 1024, 6, 500
+    rng = np.random.default_rng(21)
+    cols = np.empty((L, N, K), np.int32)
+    for l in range(L):
+        band = (np.arange(N)[:, None] + np.arange(K)[None, :]) % N
+        cols[l] = band[rng.permutation(N)]
+    vals = (rng.standard_normal((L, N, K)) * 0.3).astype(np.float32)
+    y0 = nets.random_inputs(N, B, seed=8, density=0.3)
+    yL, _, _ = orc.plain_f32(cols, vals, -0.01, 32.0, y0)
+    panels = {}
+    try:
+        for grouping in (ctx.GROUP_SEARCH, ctx.GROUP_CONSECUTIVE):
+            ctx.set_grouping(grouping)
+            net = sa.Net.from_ell(ctx, cols, vals, -0.01)
+            panels[grouping] = net.panels
+            plan, _ = _run(ctx, net, sa.default_cfg("BF"), y0)
+            np.testing.assert_array_equal(plan.activations(), yL)
+    finally:
+        ctx.set_grouping(ctx.GROUP_SEARCH)
+    print(f"K={K} panels: search {panels[ctx.GROUP_SEARCH]} consecutive "
+          f"{panels[ctx.GROUP_CONSECUTIVE]}", flush=True)
+    assert panels[ctx.GROUP_SEARCH] * 3 < panels[ctx.GROUP_CONSECUTIVE]
 
 
 @pytest.mark.parametrize("share", [1, 13, 16, 32])
