@@ -50,11 +50,11 @@ def spmm_layer_bytes(N, P, rows):
 
 
 def post_layer_bytes(N, P, rows):
-    """Algorithmic HBM bytes of one post-threshold delta launch (panel_post_wave_kernel fed
-    by the centroid stream): read and write every listed delta row once and read its
-    centroid index, plus each panel's 32 input ids, distinct weight row and 64-value chain
-    record, panel offsets and output ids.  The centroid rows (post_chain_kernel, 8*N*k
-    bytes) run on the second stream and are not counted here."""
+    """Algorithmic HBM bytes of one post-threshold delta launch (panel_post_wave_kernel):
+    read and write every listed delta row once and read its centroid index, plus each
+    panel's 32 input ids, distinct weight row and 64-value chain record, panel offsets and
+    output ids.  The next layer's centroid rows and chain records, which the same launch
+    computes in a fifth wavefront (8*N*k bytes and 256 bytes per panel), are not counted."""
     return 8.0 * N * rows + 4.0 * rows + P * (32 * 4 * 2 + 64 * 4) + (P + 1) * 4 + N * 4
 
 
@@ -75,6 +75,19 @@ def measured_traffic(wl, kernel):
             continue
         return rec.get("traffic_bytes_per_launch")
     return None
+
+
+def apply_shapes(plan, path):
+    """Launch shapes and post form of the single entry of an autotuner cache file, set on the
+    plan directly (no measurement: a profiled process runs only the forwards it traces)."""
+    entries = json.load(open(path))
+    if len(entries) != 1:
+        raise SystemExit(f"{path}: expected one tuned entry, found {len(entries)}")
+    e = next(iter(entries.values()))
+    plan.set_shape(e["grid_target"], e["narrow"], e["ell_lds_kb"], e["ell_chunked"])
+    plan.set_post_shape(e["post_grid_target"], e["post_narrow"])
+    plan.set_post_form(e["post_form"])
+    log(f"shapes from {path}: {json.dumps(e)}")
 
 
 def time_steps(plan, steps, warmup, stages=False):
@@ -496,6 +509,14 @@ def main():
     ap.add_argument("--trace-steps", type=int, default=0, metavar="K",
                     help="run one untimed and K SNICIT forwards (captured graphs) and exit: the "
                          "command kernel traces are taken from")
+    ap.add_argument("--tune", default="placement", choices=["none", "shape", "placement"],
+                    help="autotune the SNICIT plan before timing: launch shapes and post form "
+                         "(shape), plus fresh placements of the activation buffers (placement)")
+    ap.add_argument("--tune-cache", default=None, metavar="PATH",
+                    help="the autotuner's JSON cache (read and written)")
+    ap.add_argument("--shape-from", default=None, metavar="PATH",
+                    help="set the SNICIT plan's launch shapes and post form from the single "
+                         "entry of an autotuner cache instead of tuning (profiled runs)")
     ap.add_argument("--short-rows", type=int, default=0, metavar="K",
                     help="panel grouping with and without, on a random net of K-input rows")
     args = ap.parse_args()
@@ -557,6 +578,8 @@ def main():
         # processes: live columns before t, delta-list positions after it
         plan = sa.Plan(net, sa.default_cfg(args.pmc_pass, threshold=t), B)
         plan.set_input_images(IMG_SEED, 0, B)
+        if args.shape_from:
+            apply_shapes(plan, args.shape_from)
         ms_l, rows_l, _ = plan.profile()
         tt = t if args.pmc_pass == "SNICIT" else L
         rec = {"N": N, "P": P, "rows": [int(r) for r in rows_l[:tt]],
@@ -571,7 +594,31 @@ def main():
 
     snic = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
     snic.set_input_images(IMG_SEED, 0, B)
+    tune = None
+    if args.shape_from:
+        apply_shapes(snic, args.shape_from)
+    elif args.tune != "none":
+        # untuned steps for the record (not in a traced run: the plan captures its stage
+        # graphs once, after tuning), then the autotuner, outside the timed region
+        ms_u, st_u = None, None
+        if not args.trace_steps:
+            ms_u, _, st_u = time_steps(snic, max(2, args.steps // 2), 1)
+        t0 = time.perf_counter()
+        info = snic.autotune(args.tune_cache, placement=args.tune == "placement")
+        tune = {"mode": args.tune, "untuned_ms_per_step": ms_u,
+                "untuned_stages_ms": st_u and {"pre": st_u.ms_pre, "post": st_u.ms_post},
+                "tuning_s": time.perf_counter() - t0,
+                **{k: info[k] for k in ("grid_target", "narrow", "post_grid_target",
+                                        "post_narrow", "post_form", "ms_run_default",
+                                        "ms_run_tuned", "ms_pre_default", "ms_pre_tuned",
+                                        "ms_post_default", "ms_post_tuned", "placements",
+                                        "placement")}}
+        log(f"autotune ({args.tune}, {tune['tuning_s']:.1f} s): untuned {ms_u} ms/step; "
+            f"{json.dumps(info)}")
     if args.trace_steps:
+        # the trace's marker: every dispatch after this images_kernel belongs to the one
+        # untimed and K timed forwards (the tuner's runs come before it)
+        snic.set_input_images(IMG_SEED, 0, B)
         ms_s, ev_s, st_s = time_steps(snic, args.trace_steps, 1)
         log(f"trace steps: SNICIT {ms_s:.3f} ms/step; pre {st_s.ms_pre:.3f} cluster "
             f"{st_s.ms_cluster:.3f} post {st_s.ms_post:.3f} recover {st_s.ms_recover:.3f}")
@@ -609,8 +656,8 @@ def main():
     ms_l, rows_l, cms_l = snic.profile()
     plain_name = ("panel_spmm_kernel (plain epilogue, one chain per distinct panel row; "
                   "layers < t)")
-    post_name = ("panel_post_wave_kernel (delta rows, layers >= t; centroid rows on a second "
-                 "stream)")
+    post_name = ("panel_post_wave_kernel (delta rows, layers >= t; the next layer's centroid "
+                 "rows in a fifth wavefront)")
     pre, post = list(range(t)), list(range(t, L))
     kern = {
         plain_name: {"bytes": sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre),
@@ -680,6 +727,7 @@ def main():
         "roofline": roofs[dom],
         "other_kernel_roofline": roofs[plain_name if dom == post_name else post_name],
         "cpu_baseline": cpu,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
 

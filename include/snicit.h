@@ -236,13 +236,15 @@ int snicit_infer_noalloc(snicit_net* net, const snicit_run_cfg* cfg, const float
  * The reference derives its slab width from the device's shared memory per block
  * (utility.hpp:17-41) and its XY layout from an offline inspector (src/SDGC/inspector/).
  * Here the launch shape of the layer kernels is measured on the plan's own net and input:
- * grid_target = workgroups a layer launch aims for, narrow = column count up to which the
- * thread-per-(row, column) path runs, ell_lds_kb = LDS budget (KB, <= 160) of the input tile
+ * grid_target = workgroups a layer launch aims for (with the panel count it sets the tile of
+ * one workgroup: one panel's rows by a run of 64-column tiles), narrow = column count up to
+ * which the thread-per-(row, column) path runs, ell_lds_kb = LDS budget (KB, <= 160) of the input tile
  * that wide-row layers stage (0: gather inputs directly), ell_chunked = 1 to stage 64-column
  * tiles in chunks of input rows with a thread per row (layers of >= 128 rows), 2 (default)
  * for a wavefront per row over double-buffered chunks of 128 input rows (sparse hidden
  * layers).  Results are cached in a JSON file keyed by kernel kind, N, row length, rows per
- * panel and batch size (next power of two). */
+ * panel, batch size (next power of two), input density (nonzero fraction of the plan's input
+ * in halving steps), the net's panel grouping and whether the sample size was tuned. */
 typedef struct snicit_tune_info {
   int64_t grid_target;
   int32_t narrow;
@@ -259,13 +261,40 @@ typedef struct snicit_tune_info {
   int32_t post_narrow;
   double ms_post_default;
   double ms_post_tuned;
+  /* whole runs (plans with their own timing events; 0 otherwise): the pre-threshold stage
+   * (plain modes: the first min(L, 64) layers) and the whole run with the default settings
+   * and with the chosen ones */
+  double ms_pre_default, ms_pre_tuned;
+  double ms_run_default, ms_run_tuned;
+  int32_t post_form;    /* chosen post-layer form (snicit_plan_set_post_form) */
+  int32_t sample_size;  /* run configuration's sample_size after tuning */
+  int32_t placements;   /* placements of the activation buffers timed (SNICIT_TUNE_PLACEMENT) */
+  int32_t placement;    /* the one kept: 0 = the plan's own buffers */
 } snicit_tune_info;
 
 /* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
  * may be null (measure, do not store).  Tunes the pre-threshold launch shape on layer 0 at the
- * full batch, then (SNICIT modes) the post-threshold grid on whole runs at the plan's own
- * delta-row counts.  Every shape gives the same results, bit for bit. */
+ * full batch, then on whole runs at the plan's own live counts: the pre-threshold grid, and
+ * (SNICIT modes) the post-threshold grid, narrow-path limit and post-layer form at the plan's
+ * own delta-row counts.  Every shape gives the same results, bit for bit.  Same as
+ * snicit_plan_autotune_ex with flags 0. */
 int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info);
+/* Options of snicit_plan_autotune_ex.
+ * SNICIT_TUNE_SAMPLE (exact SNICIT clustering only): also choose the clustering sample size
+ *   (the pool of Alg. 1, SNICIT.hpp:228) among 8, 16, 24, 32 and the configured one, by
+ *   whole-run time.  A size is kept only when the run's categories equal those of the
+ *   configured size on the plan's input; the centroids, and with them the last bits of the
+ *   recovered activations, may differ.
+ * SNICIT_TUNE_PLACEMENT: also time the whole run (issued directly, without graphs) on up to
+ *   two fresh allocations of the activation ping-pong buffers (when device memory allows)
+ *   and keep the fastest.  Layer
+ *   kernels stream these buffers at rates that depend on where the allocation lands in
+ *   physical memory (C3: whole runs from 178 to 204 ms on different allocations of the same
+ *   plan); results are unchanged.  Not cached: it is measured on every call. */
+#define SNICIT_TUNE_SAMPLE 1u
+#define SNICIT_TUNE_PLACEMENT 2u
+int snicit_plan_autotune_ex(snicit_plan* plan, const char* cache_path, uint32_t flags,
+                            snicit_tune_info* info);
 int snicit_plan_set_shape(snicit_plan* plan, int64_t grid_target, int32_t narrow,
                           int32_t ell_lds_kb, int32_t ell_chunked);
 /* Post-threshold layers: workgroups a launch aims for (default 2048) and the delta-row count
@@ -274,9 +303,9 @@ int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
                                int32_t post_narrow);
 /* Post-threshold layers whose panels hold at most two distinct weight rows (every SDGC net):
  * 0 = the LDS-staged post kernel; 1 = the wavefront-local post kernel, each wavefront
- * computing its panel's centroid rows; 2 (default) = the wavefront-local kernel with every
- * layer's centroid rows computed on a second stream, up to 32 layers ahead of the delta rows
- * (exact clustering with resident weights; otherwise as 1).  All three give the same bits.
+ * computing its panel's centroid rows; 2 (default) = the wavefront-local kernel reading each
+ * panel's centroid chain records, which the delta launch of the layer before computes in a
+ * fifth wavefront per workgroup (compact post layout, resident weights; otherwise as 1).  All three give the same bits.
  * No counterpart in the reference (its post layer is one kernel, kernel.hpp:71-136). */
 int snicit_plan_set_post_form(snicit_plan* plan, int32_t post_form);
 

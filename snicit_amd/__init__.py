@@ -59,7 +59,11 @@ class TuneInfo(ctypes.Structure):
                 ("from_cache", ctypes.c_int32), ("ms_default", ctypes.c_double),
                 ("ms_tuned", ctypes.c_double), ("key", ctypes.c_char * 96),
                 ("post_grid_target", ctypes.c_int64), ("post_narrow", ctypes.c_int32),
-                ("ms_post_default", ctypes.c_double), ("ms_post_tuned", ctypes.c_double)]
+                ("ms_post_default", ctypes.c_double), ("ms_post_tuned", ctypes.c_double),
+                ("ms_pre_default", ctypes.c_double), ("ms_pre_tuned", ctypes.c_double),
+                ("ms_run_default", ctypes.c_double), ("ms_run_tuned", ctypes.c_double),
+                ("post_form", ctypes.c_int32), ("sample_size", ctypes.c_int32),
+                ("placements", ctypes.c_int32), ("placement", ctypes.c_int32)]
 
     def as_dict(self):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
@@ -67,7 +71,11 @@ class TuneInfo(ctypes.Structure):
                 "from_cache": bool(self.from_cache), "ms_default": self.ms_default,
                 "ms_tuned": self.ms_tuned, "key": self.key.decode(),
                 "post_grid_target": self.post_grid_target, "post_narrow": self.post_narrow,
-                "ms_post_default": self.ms_post_default, "ms_post_tuned": self.ms_post_tuned}
+                "ms_post_default": self.ms_post_default, "ms_post_tuned": self.ms_post_tuned,
+                "ms_pre_default": self.ms_pre_default, "ms_pre_tuned": self.ms_pre_tuned,
+                "ms_run_default": self.ms_run_default, "ms_run_tuned": self.ms_run_tuned,
+                "post_form": self.post_form, "sample_size": self.sample_size,
+                "placements": self.placements, "placement": self.placement}
 
 
 class ServerCfg(ctypes.Structure):
@@ -161,6 +169,8 @@ SIGNATURES = {
     "snicit_infer_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(RunCfg), _F32P, ctypes.c_int64,
                                             ctypes.c_int64, _I32P, ctypes.POINTER(Stats)]),
     "snicit_plan_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
+    "snicit_plan_autotune_ex": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_uint32,
+                                               ctypes.POINTER(TuneInfo)]),
     "snicit_plan_set_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32]),
     "snicit_plan_set_post_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
@@ -441,12 +451,17 @@ class Plan:
         _check(lib().snicit_plan_categories(self.h, _i32(out)))
         return out
 
-    def autotune(self, cache_path=None):
+    TUNE_SAMPLE, TUNE_PLACEMENT = 1, 2
+
+    def autotune(self, cache_path=None, sample=False, placement=False):
         """Measures the layer-kernel launch shape on this plan's input (or takes it from the
-        JSON cache); returns the TuneInfo as a dict."""
+        JSON cache); sample: also the exact clustering sample size (categories checked equal
+        to the configured size's); placement: also up to two fresh allocations of the
+        activation buffers.  Returns the TuneInfo as a dict."""
         info = TuneInfo()
         path = None if cache_path is None else os.fsencode(cache_path)
-        _check(lib().snicit_plan_autotune(self.h, path, ctypes.byref(info)))
+        flags = (self.TUNE_SAMPLE if sample else 0) | (self.TUNE_PLACEMENT if placement else 0)
+        _check(lib().snicit_plan_autotune_ex(self.h, path, flags, ctypes.byref(info)))
         return info.as_dict()
 
     def set_shape(self, grid_target, narrow, ell_lds_kb=80, ell_chunked=2):
@@ -456,8 +471,8 @@ class Plan:
         _check(lib().snicit_plan_set_post_shape(self.h, post_grid_target, post_narrow))
 
     # post layers with at most two distinct weight rows per panel: LDS-staged kernel,
-    # wavefront-local kernel, or wavefront-local kernel with the centroid rows on a second
-    # stream (default)
+    # wavefront-local kernel, or wavefront-local kernel reading centroid chain records that
+    # the launch of the layer before computed (default)
     POST_LDS, POST_WAVE, POST_WAVE_CHAINS = 0, 1, 2
 
     def set_post_form(self, form):

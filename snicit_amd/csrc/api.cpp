@@ -1180,6 +1180,7 @@ int snicit_net_create_ell(snicit_ctx* ctx, int32_t N, int32_t L, int32_t K, cons
     net->L = L;
     net->K = K;
     net->clip = clip;
+    net->grouping = ctx->grouping;
     net_set_bias(net.get(), bias_kind, bias);
     net->panel_off.push_back(0);
     net->nnz_hidden = (int64_t)N * K;
@@ -2072,7 +2073,7 @@ int snicit_infer_noalloc(snicit_net* net, const snicit_run_cfg* cfg, const float
 }  // extern "C"
 
 namespace {
-std::string tune_key(const snicit_plan* p) {
+std::string tune_key(const snicit_plan* p, int32_t density, uint32_t flags) {
   const Net& n = *p->net;
   const char* kind = n.is_ell(0)       ? "ell"
                      : n.mfma[0]       ? "panel-mfma"
@@ -2083,14 +2084,38 @@ std::string tune_key(const snicit_plan* p) {
   int64_t bucket = 1;
   while (bucket < p->B) bucket <<= 1;
   char buf[96];
-  std::snprintf(buf, sizeof(buf), "%s/N%d/K%d/R%lld/B%lld", kind, n.N, K, (long long)rows,
-                (long long)bucket);
+  std::snprintf(buf, sizeof(buf), "%s/N%d/K%d/R%lld/B%lld/D%d/G%d%s", kind, n.N, K,
+                (long long)rows, (long long)bucket, density, n.grouping,
+                (flags & SNICIT_TUNE_SAMPLE) ? "/S" : "");
   return buf;
+}
+
+// Input density in halving steps: 0 when at least half of the plan's input entries are
+// nonzero, d when the nonzero fraction lies in [2^-(d+1), 2^-d), 31 for an all-zero input.
+int32_t input_density(snicit_plan* p) {
+  unsigned long long* d_n = dev_alloc<unsigned long long>(1);
+  unsigned long long nz = 0;
+  try {
+    SNICIT_HIP(hipMemsetAsync(d_n, 0, sizeof(nz), p->stream));
+    launch_count_nonzero(p->d_in, p->in_rows, p->ld, p->B, d_n, p->stream);
+    SNICIT_HIP(hipMemcpyAsync(&nz, d_n, sizeof(nz), hipMemcpyDeviceToHost, p->stream));
+    SNICIT_HIP(hipStreamSynchronize(p->stream));
+  } catch (...) {
+    dev_free(d_n);
+    throw;
+  }
+  dev_free(d_n);
+  if (nz == 0) return 31;
+  const double total = (double)p->in_rows * (double)p->B;
+  int32_t d = 0;
+  while (d < 30 && (double)nz * std::ldexp(1.0, d + 1) < total) ++d;
+  return d;
 }
 
 // Cache file: a JSON object, one entry per line:
 //   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C,
-//           "post_grid_target": Q, "post_narrow": V, "ms": T, "ms_post": U},
+//           "post_grid_target": Q, "post_narrow": V, "post_form": F, "sample_size": S,
+//           "ms": T, "ms_post": U},
 struct TuneEntry {
   int64_t grid_target;
   int32_t narrow;
@@ -2098,6 +2123,8 @@ struct TuneEntry {
   int32_t ell_chunked;
   int64_t post_grid_target;
   int32_t post_narrow;
+  int32_t post_form;
+  int32_t sample_size;
   double ms;
   double ms_post;
 };
@@ -2110,14 +2137,14 @@ std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path)
   while (std::fgets(line, sizeof(line), f)) {
     char key[128];
     long long g = 0, q = 0;
-    int w = 0, e = 0, ch = 0, pn = 0;
+    int w = 0, e = 0, ch = 0, pn = 0, pf = 0, ss = 0;
     double ms = 0, msp = 0;
     if (std::sscanf(line,
                     " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                     "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                    "\"ms\": %lf, \"ms_post\": %lf",
-                    key, &g, &w, &e, &ch, &q, &pn, &ms, &msp) == 9)
-      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, ms, msp}});
+                    "\"post_form\": %d, \"sample_size\": %d, \"ms\": %lf, \"ms_post\": %lf",
+                    key, &g, &w, &e, &ch, &q, &pn, &pf, &ss, &ms, &msp) == 11)
+      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, pf, ss, ms, msp}});
   }
   std::fclose(f);
   return out;
@@ -2129,16 +2156,16 @@ void tune_cache_write(const char* path,
   FILE* f = std::fopen(tmp.c_str(), "w");
   SNICIT_REQUIRE(f != nullptr, std::string("cannot write tuning cache ") + tmp);
   std::fprintf(f, "{\n");
-  for (size_t i = 0; i < entries.size(); ++i)
+  for (size_t i = 0; i < entries.size(); ++i) {
+    const TuneEntry& t = entries[i].second;
     std::fprintf(f,
                  "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                  "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                 "\"ms\": %.6f, \"ms_post\": %.6f}%s\n",
-                 entries[i].first.c_str(), (long long)entries[i].second.grid_target,
-                 entries[i].second.narrow, entries[i].second.ell_lds_kb,
-                 entries[i].second.ell_chunked, (long long)entries[i].second.post_grid_target,
-                 entries[i].second.post_narrow, entries[i].second.ms, entries[i].second.ms_post,
-                 i + 1 < entries.size() ? "," : "");
+                 "\"post_form\": %d, \"sample_size\": %d, \"ms\": %.6f, \"ms_post\": %.6f}%s\n",
+                 entries[i].first.c_str(), (long long)t.grid_target, t.narrow, t.ell_lds_kb,
+                 t.ell_chunked, (long long)t.post_grid_target, t.post_narrow, t.post_form,
+                 t.sample_size, t.ms, t.ms_post, i + 1 < entries.size() ? "," : "");
+  }
   std::fprintf(f, "}\n");
   std::fclose(f);
   SNICIT_REQUIRE(std::rename(tmp.c_str(), path) == 0,
@@ -2175,10 +2202,108 @@ double time_layer0(snicit_plan* p, const LaunchShape& shape, const int32_t* coun
   return ms[ms.size() / 2];
 }
 
-void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* info) {
+// Medians over three whole runs (after one that builds the stage graphs) of the pre-threshold
+// stage, the post stage and the whole run; stop < 0 runs every layer.
+struct RunTimes {
+  double pre, post, total;
+};
+RunTimes time_runs(snicit_plan* p, int32_t stop) {
+  drop_graphs(p);
+  snicit_stats st{};
+  plan_run(p, stop, &st);
+  double pre[3], post[3], tot[3];
+  for (int i = 0; i < 3; ++i) {
+    plan_run(p, stop, &st);
+    pre[i] = st.ms_pre;
+    post[i] = st.ms_post;
+    tot[i] = st.ms_total;
+  }
+  std::sort(pre, pre + 3);
+  std::sort(post, post + 3);
+  std::sort(tot, tot + 3);
+  return {pre[1], post[1], tot[1]};
+}
+
+std::vector<int32_t> run_categories(snicit_plan* p) {
+  std::vector<int32_t> c((size_t)p->B);
+  if (p->B) SNICIT_HIP(hipMemcpy(c.data(), p->d_cat, (size_t)p->B * 4, hipMemcpyDeviceToHost));
+  return c;
+}
+
+// Whole runs on up to tries allocations of the activation ping-pong buffers (the plan's own
+// first); keeps the fastest and frees the rest.  A fresh pair is tried only while the device
+// has room for it and 2 GiB more.  The runs are issued directly, without stage graphs, so
+// the candidates cost no graph captures; the kept buffers' graphs are captured by the next
+// run.  Returns the whole-run time on the plan's own buffers.
+double tune_placement(snicit_plan* p, int tries, snicit_tune_info& res) {
+  struct Direct {
+    snicit_plan* p;
+    int32_t use_graph;
+    ~Direct() { p->cfg.use_graph = use_graph; }
+  } direct{p, p->cfg.use_graph};
+  p->cfg.use_graph = 0;
+  const size_t n = (size_t)p->net->N * p->ld;
+  std::vector<std::pair<float*, float*>> cand{{p->d_y[0], p->d_y[1]}};
+  const double own = time_runs(p, -1).total;
+  double best = own;
+  int keep = 0;
+  try {
+    for (int r = 1; r < tries; ++r) {
+      size_t free_b = 0, total_b = 0;
+      SNICIT_HIP(hipMemGetInfo(&free_b, &total_b));
+      if (free_b < 2 * n * sizeof(float) + ((size_t)2 << 30)) break;
+      float* a = dev_alloc_big<float>(n);
+      float* b = nullptr;
+      try {
+        b = dev_alloc_big<float>(n);
+      } catch (...) {
+        dev_free(a);
+        throw;
+      }
+      cand.push_back({a, b});
+      p->d_y[0] = a;
+      p->d_y[1] = b;
+      const double t = time_runs(p, -1).total;
+      if (t < best) {
+        best = t;
+        keep = r;
+      }
+    }
+  } catch (...) {
+    p->d_y[0] = cand[0].first;
+    p->d_y[1] = cand[0].second;
+    drop_graphs(p);
+    for (size_t i = 1; i < cand.size(); ++i) {
+      dev_free(cand[i].first);
+      dev_free(cand[i].second);
+    }
+    throw;
+  }
+  p->d_y[0] = cand[keep].first;
+  p->d_y[1] = cand[keep].second;
+  drop_graphs(p);
+  for (size_t i = 0; i < cand.size(); ++i)
+    if ((int)i != keep) {
+      dev_free(cand[i].first);
+      dev_free(cand[i].second);
+    }
+  res.placements = (int32_t)cand.size();
+  res.placement = keep;
+  return own;
+}
+
+void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
+                   snicit_tune_info* info) {
   if (!p->have_input) throw Error(SNICIT_E_STATE, "autotuning needs the plan's input");
   SNICIT_REQUIRE(p->B > 0, "autotuning needs a non-empty batch");
-  const std::string key = tune_key(p);
+  SNICIT_REQUIRE((flags & ~(SNICIT_TUNE_SAMPLE | SNICIT_TUNE_PLACEMENT)) == 0,
+                 "unknown autotune flags");
+  const bool snic = !plain_mode(p->cfg.mode) && p->cfg.threshold < p->net->L;
+  const bool exact = p->cfg.mode == SNICIT_MODE_SNICIT && p->cfg.cluster == SNICIT_CLUSTER_EXACT;
+  // whole runs need the plan's own stage events (a batch's plans have none)
+  const bool whole = p->ev[0] != nullptr;
+  const int32_t pre_stop = plain_mode(p->cfg.mode) ? std::min(p->net->L, 64) : -1;
+  const std::string key = tune_key(p, input_density(p), flags);
   auto cache = tune_cache_read(cache_path);
   snicit_tune_info res{};
   std::snprintf(res.key, sizeof(res.key), "%s", key.c_str());
@@ -2191,6 +2316,9 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       p->shape.ell_chunked = e.second.ell_chunked;
       p->shape.post_grid_target = e.second.post_grid_target;
       p->shape.post_narrow = e.second.post_narrow;
+      p->shape.post_form = e.second.post_form;
+      if ((flags & SNICIT_TUNE_SAMPLE) && exact && e.second.sample_size > 0)
+        p->cfg.sample_size = e.second.sample_size;
       res.ms_tuned = e.second.ms;
       res.ms_post_tuned = e.second.ms_post;
       res.from_cache = 1;
@@ -2199,6 +2327,14 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
   if (!found) {
     hipStream_t s = p->stream;
     SNICIT_HIP(hipStreamSynchronize(s));
+    const LaunchShape def{};
+    p->shape = def;
+    if (whole) {
+      const RunTimes r0 = time_runs(p, -1);
+      res.ms_run_default = r0.total;
+      res.ms_pre_default = pre_stop < 0 ? r0.pre : time_runs(p, pre_stop).pre;
+      res.ms_post_default = r0.post;
+    }
     if (p->net->Nin > 0) enqueue_input_layer(p);  // layer 0 reads the input layer's output
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int32_t* d_cnt = nullptr;
@@ -2206,12 +2342,10 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       SNICIT_HIP(hipEventCreate(&e0));
       SNICIT_HIP(hipEventCreate(&e1));
       d_cnt = dev_alloc<int32_t>(1);
-      const LaunchShape def{};
-      // launch size for the full batch
+      // layer 0 at the full batch: wide-row forms, then the grid size
       LaunchShape best = def;
       res.ms_default = time_layer0(p, def, p->d_counts, e0, e1);
       double best_ms = res.ms_default;
-      // wide-row layers: staged-tile LDS budget (0 = direct gathers), then the grid size
       if (p->net->is_ell(0)) {
         const int32_t cand[][2] = {{0, 0}, {64, 0}, {128, 0}, {40, 1}, {144, 1}, {80, 2}};
         for (const auto& kc : cand) {
@@ -2249,45 +2383,79 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
       best.narrow = narrow;
       res.ms_tuned = best_ms;
       p->shape = best;
+      if (whole) {
+        // pre-threshold stage at the plan's own live counts: the grid around layer 0's
+        // choice (plain modes: the first min(L, 64) layers)
+        double best_pre = time_runs(p, pre_stop).pre;
+        int64_t best_g = best.grid_target;
+        for (int64_t g : {best.grid_target / 4, best.grid_target / 2, best.grid_target * 2,
+                          best.grid_target * 4, def.grid_target}) {
+          if (g < 1024 || g > 262144 || g == best.grid_target) continue;
+          p->shape.grid_target = g;
+          const double m = time_runs(p, pre_stop).pre;
+          if (m < best_pre) {
+            best_pre = m;
+            best_g = g;
+          }
+        }
+        p->shape.grid_target = best_g;
+        res.ms_pre_tuned = best_pre;
+      }
       // post-threshold phase: whole runs at the plan's own delta-row counts, timed by the
-      // post stage's events (median of three after one run that builds the stage graphs);
-      // the grid first, then the narrow-path limit
-      if (!plain_mode(p->cfg.mode) && p->cfg.threshold < p->net->L && p->ev[0]) {
-        auto post_ms = [&](int64_t g, int32_t pn) {
+      // post stage's events; the grid, then the narrow-path limit, then the post-layer form
+      if (snic && whole) {
+        auto post_ms = [&](int64_t g, int32_t pn, int32_t form) {
           p->shape.post_grid_target = g;
           p->shape.post_narrow = pn;
-          drop_graphs(p);
-          snicit_stats st{};
-          plan_run(p, -1, &st);
-          double v[3];
-          for (double& m : v) {
-            plan_run(p, -1, &st);
-            m = st.ms_post;
-          }
-          std::sort(v, v + 3);
-          return v[1];
+          p->shape.post_form = form;
+          return time_runs(p, -1).post;
         };
-        res.ms_post_default = post_ms(def.post_grid_target, def.post_narrow);
-        double best_post = res.ms_post_default;
+        double best_post = post_ms(def.post_grid_target, def.post_narrow, def.post_form);
         int64_t best_g = def.post_grid_target;
-        int32_t best_n = def.post_narrow;
+        int32_t best_n = def.post_narrow, best_f = def.post_form;
         for (int64_t g : {512, 1024, 4096, 8192, 16384}) {
-          const double m = post_ms(g, best_n);
+          const double m = post_ms(g, best_n, best_f);
           if (m < best_post) {
             best_post = m;
             best_g = g;
           }
         }
         for (int32_t pn : {0, 32, 128, 256, 512}) {
-          const double m = post_ms(best_g, pn);
+          const double m = post_ms(best_g, pn, best_f);
           if (m < best_post) {
             best_post = m;
             best_n = pn;
           }
         }
+        for (int32_t f : {0, 1}) {
+          const double m = post_ms(best_g, best_n, f);
+          if (m < best_post) {
+            best_post = m;
+            best_f = f;
+          }
+        }
         p->shape.post_grid_target = best_g;
         p->shape.post_narrow = best_n;
+        p->shape.post_form = best_f;
         res.ms_post_tuned = best_post;
+      }
+      // clustering sample size (exact SNICIT, on request): whole runs, kept only when the
+      // categories equal those of the configured size
+      if ((flags & SNICIT_TUNE_SAMPLE) && exact && snic && whole) {
+        const int32_t own = p->cfg.sample_size;
+        double best_run = time_runs(p, -1).total;
+        const std::vector<int32_t> want = run_categories(p);
+        int32_t best_ss = own;
+        for (int32_t ss : {8, 16, 24, 32}) {
+          if (ss == own || ss > kMaxCentroidsExact) continue;
+          p->cfg.sample_size = ss;
+          const double m = time_runs(p, -1).total;
+          if (m < best_run && run_categories(p) == want) {
+            best_run = m;
+            best_ss = ss;
+          }
+        }
+        p->cfg.sample_size = best_ss;
       }
     } catch (...) {
       if (e0) (void)hipEventDestroy(e0);
@@ -2301,17 +2469,30 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
     if (cache_path) {
       cache.push_back({key, TuneEntry{p->shape.grid_target, p->shape.narrow, p->shape.ell_lds_kb,
                                       p->shape.ell_chunked, p->shape.post_grid_target,
-                                      p->shape.post_narrow, res.ms_tuned, res.ms_post_tuned}});
+                                      p->shape.post_narrow, p->shape.post_form,
+                                      (flags & SNICIT_TUNE_SAMPLE) ? p->cfg.sample_size : 0,
+                                      res.ms_tuned, res.ms_post_tuned}});
       tune_cache_write(cache_path, cache);
     }
   }
+  // settings from the cache: the run before placement stands for the default
+  res.placements = 1;
+  if ((flags & SNICIT_TUNE_PLACEMENT) && whole) {
+    const double own = tune_placement(p, 3, res);
+    if (found) res.ms_run_default = own;
+  }
+  // the last whole run captured the stage graphs with the final settings: they are kept
+  const bool final_run = whole && (!found || (flags & SNICIT_TUNE_PLACEMENT));
+  if (final_run) res.ms_run_tuned = time_runs(p, -1).total;
   res.grid_target = p->shape.grid_target;
   res.narrow = p->shape.narrow;
   res.ell_lds_kb = p->shape.ell_lds_kb;
   res.ell_chunked = p->shape.ell_chunked;
   res.post_grid_target = p->shape.post_grid_target;
   res.post_narrow = p->shape.post_narrow;
-  drop_graphs(p);
+  res.post_form = p->shape.post_form;
+  res.sample_size = p->cfg.sample_size;
+  if (!final_run) drop_graphs(p);
   p->have_run = false;
   if (info) *info = res;
 }
@@ -2320,10 +2501,15 @@ void plan_autotune(snicit_plan* p, const char* cache_path, snicit_tune_info* inf
 extern "C" {
 
 int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_info* info) {
+  return snicit_plan_autotune_ex(plan, cache_path, 0, info);
+}
+
+int snicit_plan_autotune_ex(snicit_plan* plan, const char* cache_path, uint32_t flags,
+                            snicit_tune_info* info) {
   return guarded([&] {
     SNICIT_REQUIRE(plan != nullptr, "null plan");
     SNICIT_HIP(hipSetDevice(plan->net->ctx->device));
-    plan_autotune(plan, cache_path, info);
+    plan_autotune(plan, cache_path, flags, info);
   });
 }
 
@@ -2578,7 +2764,7 @@ int snicit_batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tu
     for (size_t i = 0; i < batch->plans.size(); ++i) {
       snicit_plan* p = batch->plans[i];
       p->have_input = true;
-      plan_autotune(p, cache_path, info ? &info[i] : nullptr);
+      plan_autotune(p, cache_path, 0, info ? &info[i] : nullptr);
     }
     if (batch->graph) {
       (void)hipGraphExecDestroy(batch->graph);

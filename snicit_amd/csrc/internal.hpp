@@ -99,6 +99,7 @@ struct Net {
   std::vector<float> bias_host;  // scalar: [1]; per layer: [L]; per neuron: [L*N]
   float clip = 32.f;
   bool skip_dead = true;  // every bias <= 0 -> an all-zero row stays all-zero
+  int32_t grouping = -1;  // panel grouping the rows were packed with (-1: generated net)
   // Panel data, all layers concatenated.
   std::vector<int64_t> panel_off;  // [L+1] first panel of each layer
   int32_t* d_pcols = nullptr;      // [Ptot][kSlots]

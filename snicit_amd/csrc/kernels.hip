@@ -1811,6 +1811,25 @@ __global__ void fill_f32_kernel(float* __restrict__ p, float v, int64_t n) {
     p[i] = v;
 }
 
+// Nonzero entries of the first B columns of rows x ld feature-major activations: one
+// column range of 64 per lane group, rows strided over the grid; wavefront sums through
+// cross-lane adds, one atomic per wavefront.
+__global__ __launch_bounds__(256) void count_nonzero_kernel(const float* __restrict__ y,
+                                                            int64_t rows, int64_t ld, int64_t B,
+                                                            unsigned long long* __restrict__ out) {
+  unsigned int c = 0;
+  const int64_t tiles = (B + 63) / 64;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  for (int64_t i = w; i < rows * tiles; i += waves) {
+    const int64_t n = i / tiles, pos = (i % tiles) * 64 + lane;
+    if (pos < B && y[n * ld + pos] != 0.f) ++c;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+  if (lane == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
 // 64 x 64 tile transpose through LDS (65-float rows avoid bank conflicts).
 // in: [rows][cols] row-major, out[c * ldo + r].
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
@@ -2802,6 +2821,14 @@ void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s) {
 }
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_f32_kernel, dim3(grid1(n, 256, 65535)), dim3(256), 0, s, p, v, n);
+  check_launch();
+}
+
+void launch_count_nonzero(const float* y, int64_t rows, int64_t ld, int64_t B,
+                          unsigned long long* out, hipStream_t s) {
+  const int64_t waves = rows * ((B + 63) / 64);
+  hipLaunchKernelGGL(count_nonzero_kernel, dim3(grid1(waves * 64, 256, 8192)), dim3(256), 0, s,
+                     y, rows, ld, B, out);
   check_launch();
 }
 

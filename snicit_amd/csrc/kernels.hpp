@@ -65,8 +65,9 @@ struct LaunchShape {
   int32_t post_narrow = 64;
   // post layers whose panels hold at most two distinct weight rows: 0 = the LDS-staged post
   // kernel; 1 = the wavefront-local post kernel (no LDS, no barrier), computing its panel's
-  // centroid chains itself; 2 = the wavefront-local kernel with the centroid rows computed on
-  // a second stream ahead of the delta rows (compact post layout, resident weights; else 1)
+  // centroid chains itself; 2 = the wavefront-local kernel reading chain records that the
+  // launch of the layer before computed in a fifth wavefront (compact post layout, resident
+  // weights; else 1)
   int32_t post_form = 2;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
@@ -213,6 +214,9 @@ void launch_ring_count(int32_t* ring, int64_t ring_ld, int32_t j0, int32_t nslot
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 void launch_iota_i32(int32_t* p, int64_t n, hipStream_t s);
 void launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+// adds the count of nonzero entries of y[0..rows) x [0..B) (row pitch ld) to *out
+void launch_count_nonzero(const float* y, int64_t rows, int64_t ld, int64_t B,
+                          unsigned long long* out, hipStream_t s);
 
 // [B][N] row-major (device) -> Y[n*ld + b]
 // nonfinite (nullable): set to 1 when any value read is inf or NaN.

@@ -79,8 +79,17 @@ def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
     # the post phase is tuned on whole runs, timed by the post stage
     assert 0 < info["ms_post_tuned"] <= info["ms_post_default"]
     assert info["post_grid_target"] in (512, 1024, 2048, 4096, 8192, 16384)
-    entry = json.loads(cache.read_text())[info["key"]]
+    assert info["post_form"] in (0, 1, 2) and info["sample_size"] == 32
+    # whole runs before and after; the pre-threshold stage is tuned around layer 0's choice
+    # and the default grid (separate measurements: a small allowance for noise)
+    assert info["ms_run_default"] > 0 and info["ms_run_tuned"] > 0
+    assert 0 < info["ms_pre_tuned"] <= 1.1 * info["ms_pre_default"] + 0.05
+    assert info["placements"] == 1 and info["placement"] == 0
+    key = info["key"]
+    assert "/D" in key and "/G-1" in key and not key.endswith("/S"), key
+    entry = json.loads(cache.read_text())[key]
     assert entry["post_grid_target"] == info["post_grid_target"]
+    assert entry["post_form"] == info["post_form"]
     plan.run()
     cat = plan.categories()
     imgs = sa.gen_images(N, 3, 0, B)
@@ -91,11 +100,50 @@ def test_autotune_caches_and_keeps_parity(ctx, tmp_path):
     plan2.set_input_images(3, 0, B)
     info2 = plan2.autotune(cache)
     assert info2["from_cache"]
-    for k in ("grid_target", "narrow", "post_grid_target", "post_narrow"):
+    for k in ("grid_target", "narrow", "post_grid_target", "post_narrow", "post_form"):
         assert info2[k] == info[k], k
     assert abs(info2["ms_post_tuned"] - info["ms_post_tuned"]) < 1e-5
     plan2.run()
     np.testing.assert_array_equal(plan2.categories(), ref)
+
+
+def test_autotune_sample_size_and_placement(ctx, tmp_path):
+    """The tuner's optional phases: the exact clustering sample size (kept only when the
+    categories equal those of the configured size) and fresh placements of the activation
+    buffers.  The tuned plan, on whichever buffers it kept, still matches the oracle, run
+    after run, and a second plan takes the sample size from the cache."""
+    N, L, B = 4096, 12, 3000
+    net = sa.Net.generate_sdgc(ctx, N, L, 5, bias=-0.35)
+    cfg = sa.default_cfg("SNICIT", threshold=4)
+    cache = tmp_path / "tune.json"
+    imgs = sa.gen_images(N, 3, 0, B)
+    _, ref, _ = orc.plain_f32(sa.gen_sdgc_cols(N, L, 5), np.full((L, N, 32), 0.0625, np.float32),
+                              -0.35, 32.0, imgs, want_y=False)
+    plan = sa.Plan(net, cfg, B)
+    plan.set_input_images(3, 0, B)
+    info = plan.autotune(cache, sample=True, placement=True)
+    print(info, flush=True)
+    assert info["key"].endswith("/S")
+    assert info["sample_size"] in (8, 16, 24, 32)
+    assert 1 <= info["placements"] <= 3 and 0 <= info["placement"] < info["placements"]
+    assert info["ms_run_tuned"] > 0
+    for _ in range(2):
+        st = plan.run()
+        np.testing.assert_array_equal(plan.categories(), ref)
+    plan2 = sa.Plan(net, cfg, B)
+    plan2.set_input_images(3, 0, B)
+    info2 = plan2.autotune(cache, sample=True)
+    assert info2["from_cache"] and info2["sample_size"] == info["sample_size"]
+    assert info2["placements"] == 1
+    plan2.run()
+    np.testing.assert_array_equal(plan2.categories(), ref)
+    # without the option the configured size stays, under its own cache key
+    plan3 = sa.Plan(net, cfg, B)
+    plan3.set_input_images(3, 0, B)
+    info3 = plan3.autotune(cache)
+    assert not info3["from_cache"] and info3["sample_size"] == 32
+    with pytest.raises(sa.SnicitError, match="unknown autotune flags"):
+        sa._check(sa.lib().snicit_plan_autotune_ex(plan3.h, None, 4, None))
 
 
 def _panel_net(N, L, distinct, seed):
