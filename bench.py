@@ -158,7 +158,8 @@ def prototype_features(Nin, B, seed):
 def bench_batch(args, wl):
     """Form (a): 64 batches of seeded uneven sizes over net A (SDGC-like, exact SNICIT) and
     net B (unstructured MLP with dense ends, lossy SNICIT), one graph launch per step,
-    launch shapes autotuned per net.  The plain (BF) batch of the same segments is timed
+    launch shapes autotuned per net on whole-batch launches (the SNICIT batch is also timed
+    with the default shapes first).  The plain (BF) batch of the same segments is timed
     beside it."""
     import snicit_amd as sa
 
@@ -189,6 +190,14 @@ def bench_batch(args, wl):
                 batch.set_input_images(i, IMG_SEED, 100000 * i)
             else:
                 batch.set_input(i, feats[i])
+        untuned = None
+        if mode == "SNICIT":  # the default launch shapes, for the record
+            for _ in range(args.warmup):
+                batch.run()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                batch.run()
+            untuned = (time.perf_counter() - t0) * 1000.0 / args.steps
         tune = batch.autotune(cache)
         for _ in range(args.warmup):
             batch.run()
@@ -199,7 +208,8 @@ def bench_batch(args, wl):
             ev.append(ms)
         wall = (time.perf_counter() - t0) * 1000.0 / args.steps
         cats = [batch.categories(i) for i in range(len(segments))]
-        res[mode] = dict(ms=wall, ev=float(np.mean(ev)), stats=stats, cats=cats, tune=tune)
+        res[mode] = dict(ms=wall, ev=float(np.mean(ev)), stats=stats, cats=cats, tune=tune,
+                         untuned=untuned)
         log(f"{mode} batch: {wall:.3f} ms/step (graph events {np.mean(ev):.3f}); tuned "
             f"{[(t['key'], t['grid_target'], t['narrow']) for t in tune]}; net A k="
             f"{stats[0].centroids} nerow {stats[0].nerow_first}->{stats[0].nerow_last}; net B "
@@ -243,6 +253,7 @@ def bench_batch(args, wl):
         "categories_identical_net_a": same_a,
         "categories_agreement_net_b_lossy": agree_b,
         "autotuned_shapes": res["SNICIT"]["tune"],
+        "untuned_ms_per_step": res["SNICIT"]["untuned"],
         "roofline": None,
         "cpu_baseline": None,
     }

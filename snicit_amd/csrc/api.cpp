@@ -2703,30 +2703,149 @@ int snicit_batch_set_input_images(snicit_batch* batch, int32_t seg, uint64_t see
   });
 }
 
+}  // extern "C"
+
+namespace {
+// One launch of the batch graph (captured first when there is none); returns its HIP-event
+// time.  Every segment must hold its input.
+double batch_run(snicit_batch* batch, snicit_stats* per_net) {
+  hipStream_t s0 = batch->streams[0];
+  for (snicit_plan* p : batch->plans) SNICIT_HIP(hipStreamSynchronize(p->stream));
+  // before the capture: the recovery stage reads where the post stage leaves its state
+  for (snicit_plan* p : batch->plans) plan_bookkeeping(p, -1);
+  if (!batch->graph) batch_capture(batch);
+  SNICIT_HIP(hipEventRecord(batch->ev[0], s0));
+  SNICIT_HIP(hipGraphLaunch(batch->graph, s0));
+  SNICIT_HIP(hipEventRecord(batch->ev[1], s0));
+  SNICIT_HIP(hipEventSynchronize(batch->ev[1]));
+  float t = 0.f;
+  SNICIT_HIP(hipEventElapsedTime(&t, batch->ev[0], batch->ev[1]));
+  for (size_t i = 0; i < batch->plans.size(); ++i) {
+    snicit_plan* p = batch->plans[i];
+    p->have_input = p->have_run = true;
+    if (per_net) plan_stats(p, &per_net[i], nullptr);
+  }
+  batch->have_run = true;
+  return t;
+}
+
+void batch_drop_graph(snicit_batch* batch) {
+  if (batch->graph) {
+    (void)hipGraphExecDestroy(batch->graph);
+    batch->graph = nullptr;
+  }
+}
+
+// Median of three batch launches after one that captures the batch graph.
+double batch_time(snicit_batch* batch) {
+  batch_drop_graph(batch);
+  batch_run(batch, nullptr);
+  double v[3];
+  for (double& m : v) m = batch_run(batch, nullptr);
+  std::sort(v, v + 3);
+  return v[1];
+}
+
+// Each plan's layer-0 shape (plan_autotune), then, on whole-batch launches, each plan's
+// pre-threshold grid and (SNICIT modes) post grid, narrow-path limit and post-layer form by
+// coordinate search; the refined shapes replace the plans' cache entries.
+void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_info* info) {
+  const size_t n = batch->plans.size();
+  std::vector<snicit_tune_info> res(n);
+  const LaunchShape def{};
+  for (snicit_plan* p : batch->plans) {
+    p->have_input = true;
+    p->shape = def;
+  }
+  bool cached = true;
+  for (size_t i = 0; i < n; ++i) {
+    plan_autotune(batch->plans[i], cache_path, 0, &res[i]);
+    cached = cached && res[i].from_cache;
+  }
+  if (!cached) {
+    // the batch with default shapes, for the record
+    std::vector<LaunchShape> tuned(n);
+    for (size_t i = 0; i < n; ++i) {
+      tuned[i] = batch->plans[i]->shape;
+      batch->plans[i]->shape = def;
+    }
+    const double ms_default = batch_time(batch);
+    for (size_t i = 0; i < n; ++i) batch->plans[i]->shape = tuned[i];
+    double best = batch_time(batch);
+    auto probe = [&](snicit_plan* p, LaunchShape sh) {
+      const LaunchShape keep = p->shape;
+      p->shape = sh;
+      const double m = batch_time(batch);
+      if (m < best) best = m;
+      else p->shape = keep;
+    };
+    for (snicit_plan* p : batch->plans) {
+      const int64_t g0 = p->shape.grid_target;
+      for (int64_t g : {g0 / 2, g0 * 2, def.grid_target}) {
+        if (g < 1024 || g > 262144 || g == p->shape.grid_target) continue;
+        LaunchShape sh = p->shape;
+        sh.grid_target = g;
+        probe(p, sh);
+      }
+      if (plain_mode(p->cfg.mode) || p->cfg.threshold >= p->net->L) continue;
+      for (int64_t g : {512, 1024, 4096, 8192}) {
+        LaunchShape sh = p->shape;
+        sh.post_grid_target = g;
+        probe(p, sh);
+      }
+      for (int32_t pn : {0, 256, 512}) {
+        LaunchShape sh = p->shape;
+        sh.post_narrow = pn;
+        probe(p, sh);
+      }
+      for (int32_t f : {0, 1}) {
+        LaunchShape sh = p->shape;
+        sh.post_form = f;
+        probe(p, sh);
+      }
+    }
+    for (size_t i = 0; i < n; ++i) {
+      const LaunchShape& sh = batch->plans[i]->shape;
+      res[i].grid_target = sh.grid_target;
+      res[i].post_grid_target = sh.post_grid_target;
+      res[i].post_narrow = sh.post_narrow;
+      res[i].post_form = sh.post_form;
+      res[i].ms_run_default = ms_default;
+      res[i].ms_run_tuned = best;
+    }
+    if (cache_path) {
+      auto cache = tune_cache_read(cache_path);
+      for (size_t i = 0; i < n; ++i) {
+        const LaunchShape& sh = batch->plans[i]->shape;
+        for (auto& e : cache)
+          if (e.first == res[i].key) {
+            e.second.grid_target = sh.grid_target;
+            e.second.post_grid_target = sh.post_grid_target;
+            e.second.post_narrow = sh.post_narrow;
+            e.second.post_form = sh.post_form;
+          }
+      }
+      tune_cache_write(cache_path, cache);
+    }
+  }
+  batch_drop_graph(batch);
+  batch->have_run = false;
+  for (snicit_plan* p : batch->plans) p->have_run = false;
+  if (info)
+    for (size_t i = 0; i < n; ++i) info[i] = res[i];
+}
+}  // namespace
+
+extern "C" {
+
 int snicit_batch_run(snicit_batch* batch, double* ms, snicit_stats* per_net) {
   return guarded([&] {
     SNICIT_REQUIRE(batch != nullptr, "null batch");
     for (const auto& sg : batch->segs)
       if (!sg.loaded) throw Error(SNICIT_E_STATE, "every segment needs its input before a run");
     SNICIT_HIP(hipSetDevice(batch->ctx->device));
-    hipStream_t s0 = batch->streams[0];
-    for (snicit_plan* p : batch->plans) SNICIT_HIP(hipStreamSynchronize(p->stream));
-    // before the capture: the recovery stage reads where the post stage leaves its state
-    for (snicit_plan* p : batch->plans) plan_bookkeeping(p, -1);
-    if (!batch->graph) batch_capture(batch);
-    SNICIT_HIP(hipEventRecord(batch->ev[0], s0));
-    SNICIT_HIP(hipGraphLaunch(batch->graph, s0));
-    SNICIT_HIP(hipEventRecord(batch->ev[1], s0));
-    SNICIT_HIP(hipEventSynchronize(batch->ev[1]));
-    float t = 0.f;
-    SNICIT_HIP(hipEventElapsedTime(&t, batch->ev[0], batch->ev[1]));
+    const double t = batch_run(batch, per_net);
     if (ms) *ms = t;
-    for (size_t i = 0; i < batch->plans.size(); ++i) {
-      snicit_plan* p = batch->plans[i];
-      p->have_input = p->have_run = true;
-      if (per_net) plan_stats(p, &per_net[i], nullptr);
-    }
-    batch->have_run = true;
   });
 }
 
@@ -2761,16 +2880,7 @@ int snicit_batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tu
     for (const auto& sg : batch->segs)
       if (!sg.loaded) throw Error(SNICIT_E_STATE, "every segment needs its input first");
     SNICIT_HIP(hipSetDevice(batch->ctx->device));
-    for (size_t i = 0; i < batch->plans.size(); ++i) {
-      snicit_plan* p = batch->plans[i];
-      p->have_input = true;
-      plan_autotune(p, cache_path, 0, info ? &info[i] : nullptr);
-    }
-    if (batch->graph) {
-      (void)hipGraphExecDestroy(batch->graph);
-      batch->graph = nullptr;
-    }
-    batch->have_run = false;
+    batch_autotune(batch, cache_path, info);
   });
 }
 

@@ -111,3 +111,58 @@ def test_batch_rejects_bad_layouts(ctx):
     b.set_input_images(0, 1, 0)
     with pytest.raises(sa.SnicitError):
         b.run()  # segment 1 has no input
+
+
+def test_batch_autotune_refines_on_whole_batches(ctx, tmp_path):
+    """The batch tuner: each plan's layer-0 shape, then pre and post shapes refined on
+    whole-batch launches; results unchanged (bit for bit, against the untuned batch), the
+    refined shapes in the cache, and a second batch that takes them from there."""
+    import json
+
+    N, L = 1024, 30
+    net = sa.Net.generate_sdgc(ctx, N, L, 5, bias=-0.3)
+    cfg = sa.default_cfg("SNICIT", threshold=6)
+    cols2, vals2 = nets.random_ell(512, 10, K=32, seed=3, share=8)
+    net2 = sa.Net.from_ell(ctx, cols2, vals2, -0.02)
+    segments = [(0, 700), (1, 300), (0, 2000), (1, 64)]
+    inputs = {i: nets.random_inputs(512, B, seed=i, density=0.2)
+              for i, (n, B) in enumerate(segments) if n == 1}
+
+    def make():
+        b = sa.Batch(ctx, [net, net2], [cfg, sa.default_cfg("BF")], segments)
+        for i, (n, B) in enumerate(segments):
+            if n == 0:
+                b.set_input_images(i, 4, 3000 * i)
+            else:
+                b.set_input(i, inputs[i])
+        return b
+
+    batch = make()
+    batch.run()
+    want = [(batch.categories(i), batch.activations(i)) for i in range(len(segments))]
+    cache = tmp_path / "tune.json"
+    info = batch.autotune(cache)
+    print(info, flush=True)
+    assert not any(t["from_cache"] for t in info)
+    assert all(t["ms_run_default"] > 0 and t["ms_run_tuned"] > 0 for t in info)
+    # whole-batch times: every plan reports the batch's
+    assert info[0]["ms_run_tuned"] == info[1]["ms_run_tuned"]
+    entries = json.loads(cache.read_text())
+    for t in info:
+        e = entries[t["key"]]
+        for k in ("grid_target", "post_grid_target", "post_narrow", "post_form"):
+            assert e[k] == t[k], k
+    for _ in range(2):
+        batch.run()
+        for i in range(len(segments)):
+            np.testing.assert_array_equal(batch.categories(i), want[i][0])
+            np.testing.assert_array_equal(batch.activations(i), want[i][1])
+    batch2 = make()
+    info2 = batch2.autotune(cache)
+    assert all(t["from_cache"] for t in info2)
+    for a, b in zip(info, info2):
+        for k in ("grid_target", "narrow", "post_grid_target", "post_narrow", "post_form"):
+            assert a[k] == b[k], k
+    batch2.run()
+    for i in range(len(segments)):
+        np.testing.assert_array_equal(batch2.categories(i), want[i][0])
