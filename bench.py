@@ -44,8 +44,9 @@ def log(*a):
 
 def spmm_layer_bytes(N, P, rows):
     """Algorithmic HBM bytes of one plain panel launch on an SDGC-like net: read and write
-    every processed activation once, plus the layer in panel form (the 32 input ids and the
-    one distinct weight row of each panel, panel offsets, output ids)."""
+    every live activation once (rows = live columns entering the layer; dead columns a list
+    carries between compactions are not counted), plus the layer in panel form (the 32 input
+    ids and the one distinct weight row of each panel, panel offsets, output ids)."""
     return 8.0 * N * rows + P * 32 * 4 * 2 + (P + 1) * 4 + N * 4
 
 
@@ -77,6 +78,13 @@ def measured_traffic(wl, kernel):
     return None
 
 
+def live_rows(plan, B, n):
+    """Live columns entering each of the first n plain layers (the lists may also carry dead
+    columns between compactions; the algorithmic bytes count only the live ones)."""
+    after = plan.trace(0)
+    return [int(B)] + [int(after[l]) for l in range(n - 1)]
+
+
 def apply_shapes(plan, path):
     """Launch shapes and post form of the single entry of an autotuner cache file, set on the
     plan directly (no measurement: a profiled process runs only the forwards it traces)."""
@@ -87,6 +95,7 @@ def apply_shapes(plan, path):
     plan.set_shape(e["grid_target"], e["narrow"], e["ell_lds_kb"], e["ell_chunked"])
     plan.set_post_shape(e["post_grid_target"], e["post_narrow"])
     plan.set_post_form(e["post_form"])
+    plan.set_compact_slack(e["compact_slack"])
     log(f"shapes from {path}: {json.dumps(e)}")
 
 
@@ -594,7 +603,7 @@ def main():
         ms_l, rows_l, _ = plan.profile()
         tt = t if args.pmc_pass == "SNICIT" else L
         rec = {"N": N, "P": P, "rows": [int(r) for r in rows_l[:tt]],
-               "ms_total": float(sum(ms_l))}
+               "live_rows": live_rows(plan, B, tt), "ms_total": float(sum(ms_l))}
         if args.pmc_pass == "SNICIT":
             rec["post_rows"] = [int(r) for r in rows_l[tt:L]]
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
@@ -623,7 +632,7 @@ def main():
                                         "post_narrow", "post_form", "ms_run_default",
                                         "ms_run_tuned", "ms_pre_default", "ms_pre_tuned",
                                         "ms_post_default", "ms_post_tuned", "placements",
-                                        "placement")}}
+                                        "placement", "compact_slack")}}
         log(f"autotune ({args.tune}, {tune['tuning_s']:.1f} s): untuned {ms_u} ms/step; "
             f"{json.dumps(info)}")
     if args.trace_steps:
@@ -665,16 +674,17 @@ def main():
     # is the captured-graph rate the kernel trace sees (the stage's few list compactions
     # are counted in it).  The one with the larger total is the roofline kernel.
     ms_l, rows_l, cms_l = snic.profile()
+    live_l = live_rows(snic, B, t)
     plain_name = ("panel_spmm_kernel (plain epilogue, one chain per distinct panel row; "
                   "layers < t)")
     post_name = ("panel_post_wave_kernel (delta rows, layers >= t; the next layer's centroid "
                  "rows in a fifth wavefront)")
     pre, post = list(range(t)), list(range(t, L))
     kern = {
-        plain_name: {"bytes": sum(spmm_layer_bytes(N, P, rows_l[l]) for l in pre),
+        plain_name: {"bytes": sum(spmm_layer_bytes(N, P, live_l[l]) for l in pre),
                      "s": pre_ms / 1000.0, "eager_s": sum(ms_l[l] for l in pre) / 1000.0,
                      "launches": len(pre),
-                     "per_unit": "8*N*rows + 8*P*32 (input ids, one distinct weight row) + "
+                     "per_unit": "8*N*rows (live columns) + 8*P*32 (input ids, one distinct weight row) + "
                                  "4*(P+1) + 4*N"},
         post_name: {"bytes": sum(post_layer_bytes(N, P, rows_l[l]) for l in post),
                     "s": post_ms / 1000.0, "eager_s": sum(ms_l[l] for l in post) / 1000.0,

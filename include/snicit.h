@@ -270,6 +270,7 @@ typedef struct snicit_tune_info {
   int32_t sample_size;  /* run configuration's sample_size after tuning */
   int32_t placements;   /* placements of the activation buffers timed (SNICIT_TUNE_PLACEMENT) */
   int32_t placement;    /* the one kept: 0 = the plan's own buffers */
+  int32_t compact_slack; /* chosen compaction slack (snicit_plan_set_compact_slack) */
 } snicit_tune_info;
 
 /* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
@@ -308,6 +309,13 @@ int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
  * fifth wavefront per workgroup (compact post layout, resident weights; otherwise as 1).  All three give the same bits.
  * No counterpart in the reference (its post layer is one kernel, kernel.hpp:71-136). */
 int snicit_plan_set_post_form(snicit_plan* plan, int32_t post_form);
+/* Compaction of the live list after each plain layer (and SNICIT's layers before t): the
+ * list is kept as it is when at most slack/1024 of its columns died in the layer (0..1023;
+ * default 16), so the next layer reads its columns in place, aligned, and carries the dead
+ * ones as zeros until a later compaction drops them.  The lists entering clustering and
+ * after the last layer are always exact; results and reported live counts do not change.
+ * No counterpart in the reference (BF compacts its row list every layer, BF.hpp:183-200). */
+int snicit_plan_set_compact_slack(snicit_plan* plan, int32_t slack);
 
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another

@@ -63,7 +63,8 @@ class TuneInfo(ctypes.Structure):
                 ("ms_pre_default", ctypes.c_double), ("ms_pre_tuned", ctypes.c_double),
                 ("ms_run_default", ctypes.c_double), ("ms_run_tuned", ctypes.c_double),
                 ("post_form", ctypes.c_int32), ("sample_size", ctypes.c_int32),
-                ("placements", ctypes.c_int32), ("placement", ctypes.c_int32)]
+                ("placements", ctypes.c_int32), ("placement", ctypes.c_int32),
+                ("compact_slack", ctypes.c_int32)]
 
     def as_dict(self):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
@@ -75,7 +76,8 @@ class TuneInfo(ctypes.Structure):
                 "ms_pre_default": self.ms_pre_default, "ms_pre_tuned": self.ms_pre_tuned,
                 "ms_run_default": self.ms_run_default, "ms_run_tuned": self.ms_run_tuned,
                 "post_form": self.post_form, "sample_size": self.sample_size,
-                "placements": self.placements, "placement": self.placement}
+                "placements": self.placements, "placement": self.placement,
+                "compact_slack": self.compact_slack}
 
 
 class ServerCfg(ctypes.Structure):
@@ -175,6 +177,7 @@ SIGNATURES = {
                                              ctypes.c_int32, ctypes.c_int32]),
     "snicit_plan_set_post_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
     "snicit_plan_set_post_form": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "snicit_plan_set_compact_slack": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
     "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
@@ -477,6 +480,10 @@ class Plan:
 
     def set_post_form(self, form):
         _check(lib().snicit_plan_set_post_form(self.h, form))
+
+    def set_compact_slack(self, slack):
+        """Dead columns per 1024 a plain layer's compaction may leave in the list (0: exact)."""
+        _check(lib().snicit_plan_set_compact_slack(self.h, slack))
 
     def activations(self):
         out = np.empty((self.B, self.net.N), dtype=np.float32)

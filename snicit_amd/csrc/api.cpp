@@ -563,8 +563,10 @@ int enqueue_plain_layers(snicit_plan* p, int l_end, bool final_flags) {
     prof_mark(p, l, 1);
     done_layer(p, l);
     if (cmp) {
+      // the list entering l_end (clustering, or the categories) is always exact
       launch_compact(p->d_flags, p->d_counts + l, p->d_counts + l + 1, p->d_map, nullptr,
-                     p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, p->cap, p->d_tilecnt, s);
+                     p->d_orig[cur], p->d_orig[cur ^ 1], nullptr, nullptr, p->cap, p->d_tilecnt, s,
+                     l + 1 < l_end ? p->shape.compact_slack : 0, p->d_live + l + 1);
       cur ^= 1;
     }
   }
@@ -656,7 +658,7 @@ void enqueue_cluster(snicit_plan* p) {
       launch_iota_i32(p->d_orig[0], p->cap, s);
       launch_row_any(p->d_in, p->ld, p->d_counts, p->cap, N, p->d_flags, s);
       launch_compact(p->d_flags, p->d_counts, p->d_counts + 1, p->d_map, nullptr, p->d_orig[0],
-                     p->d_orig[1], nullptr, nullptr, p->cap, p->d_tilecnt, s);
+                     p->d_orig[1], nullptr, nullptr, p->cap, p->d_tilecnt, s, 0, p->d_live + 1);
       count = p->d_counts + 1;
     } else {
       count = p->d_counts + t;
@@ -1103,7 +1105,7 @@ void plan_free(snicit_plan* p) {
                   p->d_centmark, p->d_close, p->d_segstart, p->d_seglo, p->d_kseg,
                   p->d_act,    p->d_cat,     p->d_img_params, p->d_img_blobs, p->d_img_proto,
                   p->d_stage,  p->d_nonfinite, p->d_nlist, p->d_ring,   p->d_Cp[0],
-                  p->d_Cp[1],  p->d_zchain};
+                  p->d_Cp[1],  p->d_zchain, p->d_live};
   for (void* q : ptrs) dev_free(q);
 }
 
@@ -1552,6 +1554,7 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
         p->d_C[i] = dev_alloc<float>((size_t)N * p->kcap, acct);
       }
       p->d_counts = dev_alloc<int32_t>(L + 4, acct);
+      p->d_live = dev_alloc<int32_t>(L + 4, acct);
       p->d_tilecnt = dev_alloc<int32_t>((size_t)compact_tiles(p->cap), acct);
       p->d_nerow = dev_alloc<int32_t>(L + 2, acct);
       p->d_k = dev_alloc<int32_t>(1, acct);
@@ -1567,6 +1570,7 @@ snicit_plan* plan_make(snicit_net* net, const snicit_run_cfg* cfg, int64_t max_b
       SNICIT_HIP(hipMemset(p->d_flags, 0, p->ld * 4));
       SNICIT_HIP(hipMemset(p->d_act, 0, p->ld * 4));
       SNICIT_HIP(hipMemset(p->d_counts, 0, (L + 4) * 4));
+      SNICIT_HIP(hipMemset(p->d_live, 0, (L + 4) * 4));
       SNICIT_HIP(hipMemset(p->d_nerow, 0, (L + 2) * 4));
       SNICIT_HIP(hipMemset(p->d_k, 0, 4));
       SNICIT_HIP(hipMemset(p->d_C[0], 0, (size_t)N * p->kcap * 4));
@@ -1657,6 +1661,7 @@ void set_count(snicit_plan* p, int64_t B) {
   int32_t b32 = (int32_t)B;
   hipStream_t s = p->stream;
   SNICIT_HIP(hipMemcpyAsync(p->d_counts, &b32, 4, hipMemcpyHostToDevice, s));
+  SNICIT_HIP(hipMemcpyAsync(p->d_live, &b32, 4, hipMemcpyHostToDevice, s));
   SNICIT_HIP(hipStreamSynchronize(s));
   p->B = B;
   p->have_input = true;
@@ -1833,8 +1838,9 @@ int64_t snicit_plan_trace(snicit_plan* p, int32_t what, void* out, int64_t cap) 
     const int L = p->net->L;
     const int t = p->cfg.threshold;
     const bool snic = !plain_mode(p->cfg.mode);
+    // live columns (the lists may carry dead ones between compactions)
     std::vector<int32_t> counts(L + 4), nerow(L + 2);
-    SNICIT_HIP(hipMemcpy(counts.data(), p->d_counts, counts.size() * 4, hipMemcpyDeviceToHost));
+    SNICIT_HIP(hipMemcpy(counts.data(), p->d_live, counts.size() * 4, hipMemcpyDeviceToHost));
     SNICIT_HIP(hipMemcpy(nerow.data(), p->d_nerow, nerow.size() * 4, hipMemcpyDeviceToHost));
     int32_t k = 0;
     SNICIT_HIP(hipMemcpy(&k, p->d_k, 4, hipMemcpyDeviceToHost));
@@ -2115,7 +2121,7 @@ int32_t input_density(snicit_plan* p) {
 // Cache file: a JSON object, one entry per line:
 //   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C,
 //           "post_grid_target": Q, "post_narrow": V, "post_form": F, "sample_size": S,
-//           "ms": T, "ms_post": U},
+//           "compact_slack": C, "ms": T, "ms_post": U},
 struct TuneEntry {
   int64_t grid_target;
   int32_t narrow;
@@ -2125,6 +2131,7 @@ struct TuneEntry {
   int32_t post_narrow;
   int32_t post_form;
   int32_t sample_size;
+  int32_t compact_slack;
   double ms;
   double ms_post;
 };
@@ -2137,14 +2144,15 @@ std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path)
   while (std::fgets(line, sizeof(line), f)) {
     char key[128];
     long long g = 0, q = 0;
-    int w = 0, e = 0, ch = 0, pn = 0, pf = 0, ss = 0;
+    int w = 0, e = 0, ch = 0, pn = 0, pf = 0, ss = 0, cs = 0;
     double ms = 0, msp = 0;
     if (std::sscanf(line,
                     " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                     "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                    "\"post_form\": %d, \"sample_size\": %d, \"ms\": %lf, \"ms_post\": %lf",
-                    key, &g, &w, &e, &ch, &q, &pn, &pf, &ss, &ms, &msp) == 11)
-      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, pf, ss, ms, msp}});
+                    "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, \"ms\": %lf, "
+                    "\"ms_post\": %lf",
+                    key, &g, &w, &e, &ch, &q, &pn, &pf, &ss, &cs, &ms, &msp) == 12)
+      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, pf, ss, cs, ms, msp}});
   }
   std::fclose(f);
   return out;
@@ -2161,10 +2169,11 @@ void tune_cache_write(const char* path,
     std::fprintf(f,
                  "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                  "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                 "\"post_form\": %d, \"sample_size\": %d, \"ms\": %.6f, \"ms_post\": %.6f}%s\n",
+                 "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, \"ms\": %.6f, "
+                 "\"ms_post\": %.6f}%s\n",
                  entries[i].first.c_str(), (long long)t.grid_target, t.narrow, t.ell_lds_kb,
                  t.ell_chunked, (long long)t.post_grid_target, t.post_narrow, t.post_form,
-                 t.sample_size, t.ms, t.ms_post, i + 1 < entries.size() ? "," : "");
+                 t.sample_size, t.compact_slack, t.ms, t.ms_post, i + 1 < entries.size() ? "," : "");
   }
   std::fprintf(f, "}\n");
   std::fclose(f);
@@ -2317,6 +2326,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
       p->shape.post_grid_target = e.second.post_grid_target;
       p->shape.post_narrow = e.second.post_narrow;
       p->shape.post_form = e.second.post_form;
+      p->shape.compact_slack = e.second.compact_slack;
       if ((flags & SNICIT_TUNE_SAMPLE) && exact && e.second.sample_size > 0)
         p->cfg.sample_size = e.second.sample_size;
       res.ms_tuned = e.second.ms;
@@ -2399,6 +2409,20 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
           }
         }
         p->shape.grid_target = best_g;
+        // compaction slack (lists with dead columns carried, read aligned)
+        if (compacting(p)) {
+          int32_t best_slack = p->shape.compact_slack;
+          for (int32_t sl : {0, 16, 32, 64, 128}) {
+            if (sl == best_slack) continue;
+            p->shape.compact_slack = sl;
+            const double m = time_runs(p, pre_stop).pre;
+            if (m < best_pre) {
+              best_pre = m;
+              best_slack = sl;
+            }
+          }
+          p->shape.compact_slack = best_slack;
+        }
         res.ms_pre_tuned = best_pre;
       }
       // post-threshold phase: whole runs at the plan's own delta-row counts, timed by the
@@ -2471,7 +2495,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
                                       p->shape.ell_chunked, p->shape.post_grid_target,
                                       p->shape.post_narrow, p->shape.post_form,
                                       (flags & SNICIT_TUNE_SAMPLE) ? p->cfg.sample_size : 0,
-                                      res.ms_tuned, res.ms_post_tuned}});
+                                      p->shape.compact_slack, res.ms_tuned, res.ms_post_tuned}});
       tune_cache_write(cache_path, cache);
     }
   }
@@ -2492,6 +2516,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
   res.post_narrow = p->shape.post_narrow;
   res.post_form = p->shape.post_form;
   res.sample_size = p->cfg.sample_size;
+  res.compact_slack = p->shape.compact_slack;
   if (!final_run) drop_graphs(p);
   p->have_run = false;
   if (info) *info = res;
@@ -2535,6 +2560,15 @@ int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
     SNICIT_REQUIRE(post_grid_target >= 1 && post_narrow >= 0, "bad post launch shape");
     plan->shape.post_grid_target = post_grid_target;
     plan->shape.post_narrow = post_narrow;
+    drop_graphs(plan);
+  });
+}
+
+int snicit_plan_set_compact_slack(snicit_plan* plan, int32_t slack) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_REQUIRE(slack >= 0 && slack < 1024, "compact_slack must be in [0, 1024)");
+    plan->shape.compact_slack = slack;
     drop_graphs(plan);
   });
 }
@@ -2787,6 +2821,12 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
         sh.grid_target = g;
         probe(p, sh);
       }
+      if (compacting(p))
+        for (int32_t sl : {0, 64}) {
+          LaunchShape sh = p->shape;
+          sh.compact_slack = sl;
+          if (sl != p->shape.compact_slack) probe(p, sh);
+        }
       if (plain_mode(p->cfg.mode) || p->cfg.threshold >= p->net->L) continue;
       for (int64_t g : {512, 1024, 4096, 8192}) {
         LaunchShape sh = p->shape;
@@ -2810,6 +2850,7 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
       res[i].post_grid_target = sh.post_grid_target;
       res[i].post_narrow = sh.post_narrow;
       res[i].post_form = sh.post_form;
+      res[i].compact_slack = sh.compact_slack;
       res[i].ms_run_default = ms_default;
       res[i].ms_run_tuned = best;
     }
@@ -2823,6 +2864,7 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
             e.second.post_grid_target = sh.post_grid_target;
             e.second.post_narrow = sh.post_narrow;
             e.second.post_form = sh.post_form;
+            e.second.compact_slack = sh.compact_slack;
           }
       }
       tune_cache_write(cache_path, cache);

@@ -1732,40 +1732,112 @@ __global__ __launch_bounds__(1024) void compact_count_kernel(const int32_t* __re
   if (t == 0) tile_cnt[blockIdx.x] = s_sum;
 }
 
+// The list kept as it is (slack compaction): map_out[i] = compose[i] (or i) and the carried
+// values copied for every position of [begin, end); flags cleared.
+__device__ __forceinline__ void keep_range(int32_t* __restrict__ flags, int begin, int end,
+                                           int32_t* __restrict__ map_out,
+                                           const int32_t* __restrict__ compose,
+                                           const int32_t* __restrict__ c0_in,
+                                           int32_t* __restrict__ c0_out,
+                                           const int32_t* __restrict__ c1_in,
+                                           int32_t* __restrict__ c1_out) {
+  for (int i = begin + (int)threadIdx.x; i < end; i += (int)blockDim.x) {
+    map_out[i] = compose ? compose[i] : i;
+    if (c0_out) c0_out[i] = c0_in ? c0_in[i] : 0;
+    if (c1_out) c1_out[i] = c1_in ? c1_in[i] : 0;
+    flags[i] = 0;
+  }
+}
+
+// slack > 0: when at most n * slack / 1024 of the n listed columns are unflagged, the list is
+// kept as it is (count_out = n) instead of compacted, so the next layer reads its columns in
+// place, aligned; the unflagged columns ride along until a later compaction drops them.
+// live_out (nullable) receives the flagged count either way.
 __global__ __launch_bounds__(1024) void compact_kernel(
     int32_t* __restrict__ flags, const int32_t* __restrict__ count_in,
     int32_t* __restrict__ count_out, int32_t* __restrict__ map_out,
     const int32_t* __restrict__ compose, const int32_t* __restrict__ c0_in,
     int32_t* __restrict__ c0_out, const int32_t* __restrict__ c1_in,
-    int32_t* __restrict__ c1_out, const int32_t* __restrict__ tile_cnt) {
+    int32_t* __restrict__ c1_out, const int32_t* __restrict__ tile_cnt, int32_t slack,
+    int32_t* __restrict__ live_out) {
   __shared__ int32_t s_cnt[kCompactSub * 16];
   __shared__ int32_t s_off[kCompactSub * 16];
   __shared__ int32_t s_tot;
   __shared__ int32_t s_base;
+  __shared__ int32_t s_live;
   const int n = *count_in;
   const int t = threadIdx.x, lane = lane_id();
+  const auto keep_list = [&](int live) {
+    return slack > 0 && (int64_t)(n - live) * 1024 <= (int64_t)n * slack;
+  };
   if (n <= kCompactOneBlock || gridDim.x == 1) {  // one workgroup walks the whole list
     if (blockIdx.x != 0) return;
+    if (slack > 0) {
+      if (t == 0) s_live = 0;
+      __syncthreads();
+      int c = 0;
+      for (int i = t; i < n; i += 1024) c += flags[i] != 0;
+      for (int d = kWave / 2; d > 0; d >>= 1) c += __shfl_xor(c, d, kWave);
+      if (lane == 0 && c) atomicAdd(&s_live, c);
+      __syncthreads();
+      const int live = s_live;
+      if (keep_list(live)) {
+        keep_range(flags, 0, n, map_out, compose, c0_in, c0_out, c1_in, c1_out);
+        if (t == 0) {
+          *count_out = n;
+          if (live_out) *live_out = live;
+        }
+        return;
+      }
+    }
     const int placed = compact_range<kCompactSub>(flags, 0, n, 0, map_out, compose, c0_in, c0_out,
                                                   c1_in, c1_out, s_cnt, s_off, &s_tot);
-    if (t == 0) *count_out = placed;
+    if (t == 0) {
+      *count_out = placed;
+      if (live_out) *live_out = placed;
+    }
     return;
   }
   const int64_t base = (int64_t)blockIdx.x * kCompactTile;
   if (base >= n) return;
-  // output slots start after the earlier tiles' columns
-  if (t == 0) s_base = 0;
+  // output slots start after the earlier tiles' columns; the total decides whether the list
+  // is kept
+  const int ntiles = (int)((n + kCompactTile - 1) / kCompactTile);
+  if (t == 0) {
+    s_base = 0;
+    s_live = 0;
+  }
   __syncthreads();
-  int part = 0;
-  for (int j = t; j < (int)blockIdx.x; j += 1024) part += tile_cnt[j];
-  for (int d = kWave / 2; d > 0; d >>= 1) part += __shfl_xor(part, d, kWave);
+  int part = 0, all = 0;
+  for (int j = t; j < ntiles; j += 1024) {
+    const int v = tile_cnt[j];
+    all += v;
+    if (j < (int)blockIdx.x) part += v;
+  }
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    part += __shfl_xor(part, d, kWave);
+    all += __shfl_xor(all, d, kWave);
+  }
   if (lane == 0 && part) atomicAdd(&s_base, part);
+  if (lane == 0 && all) atomicAdd(&s_live, all);
   __syncthreads();
   const int end = (int)min<int64_t>(base + kCompactTile, n);
+  const int live = s_live;
+  if (keep_list(live)) {
+    keep_range(flags, (int)base, end, map_out, compose, c0_in, c0_out, c1_in, c1_out);
+    if (end == n && t == 0) {
+      *count_out = n;
+      if (live_out) *live_out = live;
+    }
+    return;
+  }
   const int placed = compact_range<kCompactTileSub>(flags, (int)base, end, s_base, map_out, compose,
                                                     c0_in, c0_out, c1_in, c1_out, s_cnt, s_off,
                                                     &s_tot);
-  if (end == n && t == 0) *count_out = placed;
+  if (end == n && t == 0) {
+    *count_out = placed;
+    if (live_out) *live_out = placed;
+  }
 }
 
 // Non-empty delta rows of the compact post layout: workgroup i counts the set flags of ring
@@ -2787,7 +2859,8 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
                     const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
-                    hipStream_t s) {
+                    hipStream_t s, int32_t slack, int32_t* live_out) {
+  SNICIT_REQUIRE(slack >= 0 && slack < 1024, "compaction slack out of range");
   SNICIT_REQUIRE(max_cols >= 0 && max_cols < (int64_t(1) << 31), "compaction size out of range");
   // tiles of kCompactTile columns when the count may exceed one workgroup's walk; without
   // the tile-count scratch (lists that stay short) one workgroup walks any count
@@ -2800,7 +2873,7 @@ void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out,
     check_launch();
   }
   hipLaunchKernelGGL(compact_kernel, dim3(tiles), dim3(1024), 0, s, flags, count_in, count_out,
-                     map_out, compose, c0_in, c0_out, c1_in, c1_out, tile_cnt);
+                     map_out, compose, c0_in, c0_out, c1_in, c1_out, tile_cnt, slack, live_out);
   check_launch();
 }
 

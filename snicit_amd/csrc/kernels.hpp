@@ -69,6 +69,11 @@ struct LaunchShape {
   // launch of the layer before computed in a fifth wavefront (compact post layout, resident
   // weights; else 1)
   int32_t post_form = 2;
+  // plain layers (and SNICIT's layers < t): the compaction after a layer keeps the live list
+  // as it is when at most compact_slack / 1024 of it died, so the next layer reads aligned
+  // columns (a shifted list costs ~12% more reads) and carries the dead columns as zeros;
+  // the lists entering clustering and the last layer's are always exact
+  int32_t compact_slack = 16;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
   // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in
@@ -200,11 +205,13 @@ void launch_lossy_delta(const float* Y, int64_t ld, const float* C, int32_t kcap
                         int64_t ldD, int32_t* flags, hipStream_t s);
 
 // max_cols: upper bound of *count_in; tile_cnt: scratch of compact_tiles(max_cols) ints, or
-// null for lists expected to stay short (one workgroup then walks any count, one launch)
+// null for lists expected to stay short (one workgroup then walks any count, one launch).
+// slack (0..1023): keep the list as it is when at most count * slack / 1024 of its columns
+// are unflagged (0: always compact); live_out (nullable) receives the flagged count.
 void launch_compact(int32_t* flags, const int32_t* count_in, int32_t* count_out, int32_t* map_out,
                     const int32_t* compose, const int32_t* c0_in, int32_t* c0_out,
                     const int32_t* c1_in, int32_t* c1_out, int64_t max_cols, int32_t* tile_cnt,
-                    hipStream_t s);
+                    hipStream_t s, int32_t slack = 0, int32_t* live_out = nullptr);
 inline int64_t compact_tiles(int64_t max_cols) { return (max_cols + 4095) / 4096; }
 
 // nerow[i] = set flags of ring slot (j0 + i) % kPostRing over *count positions, i < nslots;

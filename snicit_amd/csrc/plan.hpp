@@ -37,7 +37,9 @@ struct snicit_plan {
   int32_t* d_mapD = nullptr;  // post-threshold delta list
   int32_t* d_orig[2] = {nullptr, nullptr};
   int32_t* d_mcar[2] = {nullptr, nullptr};
-  int32_t* d_counts = nullptr;  // [L + 4]: live columns entering layer l; [L+1..] scratch
+  int32_t* d_counts = nullptr;  // [L + 4]: listed columns entering layer l; [L+1..] scratch
+  int32_t* d_live = nullptr;    // [L + 4]: live columns entering layer l (the list may carry
+                                // dead ones: compact_slack)
   int32_t* d_tilecnt = nullptr; // [compact_tiles(cap)] per-tile counts of the compaction
   int32_t* d_nerow = nullptr;   // [L + 2]: delta columns entering layer l
   // compact post layout (exact clustering, one segment, every post layer fused): the delta

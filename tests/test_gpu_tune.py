@@ -188,3 +188,48 @@ def test_post_forms_give_identical_bits(ctx, distinct, per_neuron_bias):
             np.testing.assert_array_equal(plan.activations(), r["yL"])
             np.testing.assert_array_equal(plan.categories(), r["cat"])
             np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+
+
+@pytest.mark.parametrize("B", [3000, 20000])
+@pytest.mark.parametrize("slack", [0, 32, 1023])
+def test_compaction_slack_keeps_results(ctx, slack, B):
+    """Compaction slack: a plain layer's compaction keeps the live list as it is (dead columns
+    carried as zeros) when at most slack/1024 of it died; the lists entering clustering and
+    after the last layer stay exact.  BF and SNICIT (t = 3, 20) give the oracle's categories,
+    activations, live counts and clustering traces at every slack, 1023 keeping lists almost
+    always; B = 20000 takes the multi-workgroup compaction."""
+    from tests import nets
+
+    N, L = 1024, 40
+    cols, vals = nets.sdgc(N, L, seed=7)
+    y0 = nets.images(N, B, seed=11)
+    net = sa.Net.from_ell(ctx, cols, vals, -0.3)
+    yL, cat, live = orc.plain_f32(cols, vals, -0.3, 32.0, y0)
+    print(f"slack {slack} B {B}: BF", flush=True)
+    plan = sa.Plan(net, sa.default_cfg("BF"), B)
+    plan.set_compact_slack(slack)
+    plan.set_input(y0)
+    st = plan.run()
+    np.testing.assert_array_equal(plan.categories(), cat)
+    np.testing.assert_array_equal(plan.activations(), yL)
+    np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LIVE), live)
+    # work: the lists carried at least the live columns
+    assert st.rows_processed >= B + int(np.sum(live[:-1]))
+    for t in (3, 20):
+        print(f"slack {slack} B {B}: SNICIT t={t}", flush=True)
+        r = orc.snicit_exact(cols, vals, -0.3, 32.0, y0, t)
+        plan = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
+        plan.set_compact_slack(slack)
+        plan.set_input(y0)
+        st = plan.run()
+        np.testing.assert_array_equal(plan.categories(), r["cat"])
+        assert st.live_at_threshold == r["live_at_t"] and st.centroids == r["k"]
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LIVE)[:t], live[:t])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_POOL), r["pool"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_F).reshape(-1, 16), r["F"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_ASSIGN), r["assign"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_NEROW)[t:], r["nerow"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
+    with pytest.raises(sa.SnicitError):
+        plan.set_compact_slack(1024)
