@@ -266,7 +266,7 @@ def bench_batch(args, wl):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def short_rows(K):
@@ -493,7 +493,7 @@ def bench_serve(args, wl):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -537,6 +537,8 @@ def main():
     ap.add_argument("--shape-from", default=None, metavar="PATH",
                     help="set the SNICIT plan's launch shapes and post form from the single "
                          "entry of an autotuner cache instead of tuning (profiled runs)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="c3: skip the batched (c4) and serving (c5) summaries in the line")
     ap.add_argument("--short-rows", type=int, default=0, metavar="K",
                     help="panel grouping with and without, on a random net of K-input rows")
     args = ap.parse_args()
@@ -565,7 +567,7 @@ def main():
 
     wl = dict(WORKLOADS[args.config])
     if args.config == "c4":
-        bench_batch(args, wl)
+        print(json.dumps(bench_batch(args, wl)), flush=True)
         return
     for key, val in (("slots", args.slots), ("window_ms", args.window_ms),
                      ("group_samples", args.group_samples), ("round_ms", args.round_ms)):
@@ -575,7 +577,7 @@ def main():
         if args.pmc_pass:
             pass_serve_net_b(args, wl)
         else:
-            bench_serve(args, wl)
+            print(json.dumps(bench_serve(args, wl)), flush=True)
         return
     if args.threshold is not None:
         wl["t"] = args.threshold
@@ -718,6 +720,20 @@ def main():
         cpu = cpu_baseline(wl, cat_s, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.3e} edges/s on {cpu['cores']} cores; {cpu['sample']}; "
             f"categories match GPU: {cpu['categories_match_gpu']}")
+    forms = {}
+    if args.config == "c3" and not args.no_forms:
+        # the batched and serving forms on their own workloads, after this workload's plans
+        # and net are freed (summaries in this line; full lines with --config c4 / c5)
+        snic.close()
+        net.close()
+        a = bench_batch(args, dict(WORKLOADS["c4"]))
+        forms["form_a_batches"] = {k: a[k] for k in (
+            "metric", "value", "unit", "ms_per_step", "untuned_ms_per_step",
+            "plain_bf_ms_per_step", "speedup_vs_plain_bf", "categories_identical_net_a",
+            "categories_agreement_net_b_lossy", "config")}
+        b = bench_serve(args, dict(WORKLOADS["c5"]))
+        forms["form_b_serving"] = {k: b[k] for k in ("metric", "value", "unit", "config",
+                                                     "serving")}
     out = {
         "metric": "SNICIT edges/s (samples x nnz(W) x layers / time), one forward per step",
         "value": edges / (ms_s / 1000.0),
@@ -749,6 +765,7 @@ def main():
         "other_kernel_roofline": roofs[plain_name if dom == post_name else post_name],
         "cpu_baseline": cpu,
         "autotune": tune,
+        **forms,
     }
     print(json.dumps(out), flush=True)
 
