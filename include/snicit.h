@@ -284,7 +284,8 @@ int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_
  * SNICIT_TUNE_SAMPLE (exact SNICIT clustering only): also choose the clustering sample size
  *   (the pool of Alg. 1, SNICIT.hpp:228) among 8, 16, 24, 32 and the configured one, by
  *   whole-run time.  A size is kept only when the run's categories equal those of the
- *   configured size on the plan's input; the centroids, and with them the last bits of the
+ *   configured size on the input it was tuned on (a cached choice is reused under the same
+ *   key without running the check again); the centroids, and with them the last bits of the
  *   recovered activations, may differ.
  * SNICIT_TUNE_PLACEMENT: also time the whole run (issued directly, without graphs) on up to
  *   two fresh allocations of the activation ping-pong buffers (when device memory allows)
