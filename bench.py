@@ -96,7 +96,9 @@ def apply_shapes(plan, path):
     plan.set_post_shape(e["post_grid_target"], e["post_narrow"])
     plan.set_post_form(e["post_form"])
     plan.set_compact_slack(e["compact_slack"])
+    plan.set_plain_tiles(e["plain_tiles"])
     log(f"shapes from {path}: {json.dumps(e)}")
+    return e
 
 
 def time_steps(plan, steps, warmup, stages=False):
@@ -617,8 +619,9 @@ def main():
     snic = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t), B)
     snic.set_input_images(IMG_SEED, 0, B)
     tune = None
+    plain_tiles = 2  # the library's default
     if args.shape_from:
-        apply_shapes(snic, args.shape_from)
+        plain_tiles = apply_shapes(snic, args.shape_from)["plain_tiles"]
     elif args.tune != "none":
         # untuned steps for the record (not in a traced run: the plan captures its stage
         # graphs once, after tuning), then the autotuner, outside the timed region
@@ -627,6 +630,7 @@ def main():
             ms_u, _, st_u = time_steps(snic, max(2, args.steps // 2), 1)
         t0 = time.perf_counter()
         info = snic.autotune(args.tune_cache, placement=args.tune == "placement")
+        plain_tiles = info["plain_tiles"]
         tune = {"mode": args.tune, "untuned_ms_per_step": ms_u,
                 "untuned_stages_ms": st_u and {"pre": st_u.ms_pre, "post": st_u.ms_post},
                 "tuning_s": time.perf_counter() - t0,
@@ -634,7 +638,7 @@ def main():
                                         "post_narrow", "post_form", "ms_run_default",
                                         "ms_run_tuned", "ms_pre_default", "ms_pre_tuned",
                                         "ms_post_default", "ms_post_tuned", "placements",
-                                        "placement", "compact_slack")}}
+                                        "placement", "compact_slack", "plain_tiles")}}
         log(f"autotune ({args.tune}, {tune['tuning_s']:.1f} s): untuned {ms_u} ms/step; "
             f"{json.dumps(info)}")
     if args.trace_steps:
@@ -677,8 +681,9 @@ def main():
     # are counted in it).  The one with the larger total is the roofline kernel.
     ms_l, rows_l, cms_l = snic.profile()
     live_l = live_rows(snic, B, t)
-    plain_name = ("panel_spmm_kernel (plain epilogue, one chain per distinct panel row; "
-                  "layers < t)")
+    plain_name = ((f"panel_spmm_tiles_kernel<{plain_tiles}>" if plain_tiles > 1 else
+                   "panel_spmm_kernel<0>") +
+                  " (plain epilogue, one chain per distinct panel row; layers < t)")
     post_name = ("panel_post_wave_kernel (delta rows, layers >= t; the next layer's centroid "
                  "rows in a fifth wavefront)")
     pre, post = list(range(t)), list(range(t, L))

@@ -271,6 +271,7 @@ typedef struct snicit_tune_info {
   int32_t placements;   /* placements of the activation buffers timed (SNICIT_TUNE_PLACEMENT) */
   int32_t placement;    /* the one kept: 0 = the plan's own buffers */
   int32_t compact_slack; /* chosen compaction slack (snicit_plan_set_compact_slack) */
+  int32_t plain_tiles;   /* chosen tiles per wavefront (snicit_plan_set_plain_tiles) */
 } snicit_tune_info;
 
 /* Needs the plan's input (set_input); leaves the plan without a completed run.  cache_path
@@ -317,6 +318,11 @@ int snicit_plan_set_post_form(snicit_plan* plan, int32_t post_form);
  * after the last layer are always exact; results and reported live counts do not change.
  * No counterpart in the reference (BF compacts its row list every layer, BF.hpp:183-200). */
 int snicit_plan_set_compact_slack(snicit_plan* plan, int32_t slack);
+/* Plain layers on the vector kernel (panels whose rows repeat one weight row, every SDGC
+ * net): 64-column tiles each wavefront gathers per iteration, 1, 2 (default) or 4.  Tiles
+ * of one wavefront fetch a 128-byte line they share once; more tiles take more registers.
+ * Results do not change.  No counterpart in the reference. */
+int snicit_plan_set_plain_tiles(snicit_plan* plan, int32_t tiles);
 
 /* ---- batches: many uneven batches over several nets in one graph launch ------------
  * The reference runs one net over equal rounds of `batch` samples, one round after another

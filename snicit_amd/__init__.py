@@ -64,7 +64,7 @@ class TuneInfo(ctypes.Structure):
                 ("ms_run_default", ctypes.c_double), ("ms_run_tuned", ctypes.c_double),
                 ("post_form", ctypes.c_int32), ("sample_size", ctypes.c_int32),
                 ("placements", ctypes.c_int32), ("placement", ctypes.c_int32),
-                ("compact_slack", ctypes.c_int32)]
+                ("compact_slack", ctypes.c_int32), ("plain_tiles", ctypes.c_int32)]
 
     def as_dict(self):
         return {"grid_target": self.grid_target, "narrow": self.narrow,
@@ -77,7 +77,7 @@ class TuneInfo(ctypes.Structure):
                 "ms_run_default": self.ms_run_default, "ms_run_tuned": self.ms_run_tuned,
                 "post_form": self.post_form, "sample_size": self.sample_size,
                 "placements": self.placements, "placement": self.placement,
-                "compact_slack": self.compact_slack}
+                "compact_slack": self.compact_slack, "plain_tiles": self.plain_tiles}
 
 
 class ServerCfg(ctypes.Structure):
@@ -178,6 +178,7 @@ SIGNATURES = {
     "snicit_plan_set_post_shape": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
     "snicit_plan_set_post_form": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_plan_set_compact_slack": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "snicit_plan_set_plain_tiles": (ctypes.c_int, [_P, ctypes.c_int32]),
     "snicit_batch_autotune": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(TuneInfo)]),
     "snicit_server_cfg_default": (None, [ctypes.POINTER(ServerCfg)]),
     "snicit_server_create": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(RunCfg),
@@ -480,6 +481,10 @@ class Plan:
 
     def set_post_form(self, form):
         _check(lib().snicit_plan_set_post_form(self.h, form))
+
+    def set_plain_tiles(self, tiles):
+        """64-column tiles per wavefront of the plain vector kernel: 1, 2 or 4."""
+        _check(lib().snicit_plan_set_plain_tiles(self.h, tiles))
 
     def set_compact_slack(self, slack):
         """Dead columns per 1024 a plain layer's compaction may leave in the list (0: exact)."""

@@ -2121,7 +2121,7 @@ int32_t input_density(snicit_plan* p) {
 // Cache file: a JSON object, one entry per line:
 //   "key": {"grid_target": G, "narrow": W, "ell_lds_kb": E, "ell_chunked": C,
 //           "post_grid_target": Q, "post_narrow": V, "post_form": F, "sample_size": S,
-//           "compact_slack": C, "ms": T, "ms_post": U},
+//           "compact_slack": C, "plain_tiles": N, "ms": T, "ms_post": U},
 struct TuneEntry {
   int64_t grid_target;
   int32_t narrow;
@@ -2132,6 +2132,7 @@ struct TuneEntry {
   int32_t post_form;
   int32_t sample_size;
   int32_t compact_slack;
+  int32_t plain_tiles;
   double ms;
   double ms_post;
 };
@@ -2144,15 +2145,15 @@ std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const char* path)
   while (std::fgets(line, sizeof(line), f)) {
     char key[128];
     long long g = 0, q = 0;
-    int w = 0, e = 0, ch = 0, pn = 0, pf = 0, ss = 0, cs = 0;
+    int w = 0, e = 0, ch = 0, pn = 0, pf = 0, ss = 0, cs = 0, pt = 0;
     double ms = 0, msp = 0;
     if (std::sscanf(line,
                     " \"%127[^\"]\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                     "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                    "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, \"ms\": %lf, "
-                    "\"ms_post\": %lf",
-                    key, &g, &w, &e, &ch, &q, &pn, &pf, &ss, &cs, &ms, &msp) == 12)
-      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, pf, ss, cs, ms, msp}});
+                    "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, "
+                    "\"plain_tiles\": %d, \"ms\": %lf, \"ms_post\": %lf",
+                    key, &g, &w, &e, &ch, &q, &pn, &pf, &ss, &cs, &pt, &ms, &msp) == 13)
+      out.push_back({key, TuneEntry{g, w, e, ch, q, pn, pf, ss, cs, pt, ms, msp}});
   }
   std::fclose(f);
   return out;
@@ -2169,11 +2170,12 @@ void tune_cache_write(const char* path,
     std::fprintf(f,
                  "  \"%s\": {\"grid_target\": %lld, \"narrow\": %d, \"ell_lds_kb\": %d, "
                  "\"ell_chunked\": %d, \"post_grid_target\": %lld, \"post_narrow\": %d, "
-                 "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, \"ms\": %.6f, "
-                 "\"ms_post\": %.6f}%s\n",
+                 "\"post_form\": %d, \"sample_size\": %d, \"compact_slack\": %d, "
+                 "\"plain_tiles\": %d, \"ms\": %.6f, \"ms_post\": %.6f}%s\n",
                  entries[i].first.c_str(), (long long)t.grid_target, t.narrow, t.ell_lds_kb,
                  t.ell_chunked, (long long)t.post_grid_target, t.post_narrow, t.post_form,
-                 t.sample_size, t.compact_slack, t.ms, t.ms_post, i + 1 < entries.size() ? "," : "");
+                 t.sample_size, t.compact_slack, t.plain_tiles, t.ms, t.ms_post,
+                 i + 1 < entries.size() ? "," : "");
   }
   std::fprintf(f, "}\n");
   std::fclose(f);
@@ -2327,6 +2329,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
       p->shape.post_narrow = e.second.post_narrow;
       p->shape.post_form = e.second.post_form;
       p->shape.compact_slack = e.second.compact_slack;
+      p->shape.plain_tiles = e.second.plain_tiles;
       if ((flags & SNICIT_TUNE_SAMPLE) && exact && e.second.sample_size > 0)
         p->cfg.sample_size = e.second.sample_size;
       res.ms_tuned = e.second.ms;
@@ -2423,6 +2426,18 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
           }
           p->shape.compact_slack = best_slack;
         }
+        // 64-column tiles per wavefront of the plain vector kernel
+        int32_t best_tiles = p->shape.plain_tiles;
+        for (int32_t nt : {1, 2, 4}) {
+          if (nt == best_tiles) continue;
+          p->shape.plain_tiles = nt;
+          const double m = time_runs(p, pre_stop).pre;
+          if (m < best_pre) {
+            best_pre = m;
+            best_tiles = nt;
+          }
+        }
+        p->shape.plain_tiles = best_tiles;
         res.ms_pre_tuned = best_pre;
       }
       // post-threshold phase: whole runs at the plan's own delta-row counts, timed by the
@@ -2495,7 +2510,8 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
                                       p->shape.ell_chunked, p->shape.post_grid_target,
                                       p->shape.post_narrow, p->shape.post_form,
                                       (flags & SNICIT_TUNE_SAMPLE) ? p->cfg.sample_size : 0,
-                                      p->shape.compact_slack, res.ms_tuned, res.ms_post_tuned}});
+                                      p->shape.compact_slack, p->shape.plain_tiles, res.ms_tuned,
+                                      res.ms_post_tuned}});
       tune_cache_write(cache_path, cache);
     }
   }
@@ -2517,6 +2533,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
   res.post_form = p->shape.post_form;
   res.sample_size = p->cfg.sample_size;
   res.compact_slack = p->shape.compact_slack;
+  res.plain_tiles = p->shape.plain_tiles;
   if (!final_run) drop_graphs(p);
   p->have_run = false;
   if (info) *info = res;
@@ -2560,6 +2577,15 @@ int snicit_plan_set_post_shape(snicit_plan* plan, int64_t post_grid_target,
     SNICIT_REQUIRE(post_grid_target >= 1 && post_narrow >= 0, "bad post launch shape");
     plan->shape.post_grid_target = post_grid_target;
     plan->shape.post_narrow = post_narrow;
+    drop_graphs(plan);
+  });
+}
+
+int snicit_plan_set_plain_tiles(snicit_plan* plan, int32_t tiles) {
+  return guarded([&] {
+    SNICIT_REQUIRE(plan != nullptr, "null plan");
+    SNICIT_REQUIRE(tiles == 1 || tiles == 2 || tiles == 4, "plain_tiles must be 1, 2 or 4");
+    plan->shape.plain_tiles = tiles;
     drop_graphs(plan);
   });
 }
@@ -2827,6 +2853,11 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
           sh.compact_slack = sl;
           if (sl != p->shape.compact_slack) probe(p, sh);
         }
+      for (int32_t nt : {1, 4}) {
+        LaunchShape sh = p->shape;
+        sh.plain_tiles = nt;
+        if (nt != p->shape.plain_tiles) probe(p, sh);
+      }
       if (plain_mode(p->cfg.mode) || p->cfg.threshold >= p->net->L) continue;
       for (int64_t g : {512, 1024, 4096, 8192}) {
         LaunchShape sh = p->shape;
@@ -2851,6 +2882,7 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
       res[i].post_narrow = sh.post_narrow;
       res[i].post_form = sh.post_form;
       res[i].compact_slack = sh.compact_slack;
+      res[i].plain_tiles = sh.plain_tiles;
       res[i].ms_run_default = ms_default;
       res[i].ms_run_tuned = best;
     }
@@ -2865,6 +2897,7 @@ void batch_autotune(snicit_batch* batch, const char* cache_path, snicit_tune_inf
             e.second.post_narrow = sh.post_narrow;
             e.second.post_form = sh.post_form;
             e.second.compact_slack = sh.compact_slack;
+            e.second.plain_tiles = sh.plain_tiles;
           }
       }
       tune_cache_write(cache_path, cache);

@@ -200,6 +200,99 @@ __global__ __launch_bounds__(kBlock, 8) void panel_spmm_kernel(
   }
 }
 
+// Plain panel layer (EPI_PLAIN, vector chains) with TPW neighbouring 64-column tiles per
+// wavefront and iteration: a workgroup takes runs of 256 * TPW columns, wavefront w the
+// tiles base + 64 (TPW w + q), and each wavefront issues the gathers of all its tiles before
+// the chains.  When the column list is shifted, neighbouring tiles share a 128-byte line;
+// within one wavefront both requests reach the cache together and the line is fetched once,
+// while tiles of different wavefronts request it separately and a third of such lines were
+// fetched twice (L2 counters on the 16384-neuron net).  Same results as panel_spmm_kernel
+// <EPI_PLAIN>; more registers (102 at TPW 2, 158 at 4; 64 at one tile) and fewer wavefronts
+// per SIMD.  Pre-threshold stage, same plan and placement: 103.1 ms (one tile) against
+// 102.1 (2) and 101.9 (4) at N=65536, 27.44 against 27.25 (2) and 27.35 (4) at N=16384.
+template <int TPW>
+__global__ __launch_bounds__(kBlock, (TPW == 2 ? 4 : 2)) void panel_spmm_tiles_kernel(
+    const int32_t* __restrict__ pcols, const int32_t* __restrict__ pbeg,
+    const int32_t* __restrict__ outs, const float* __restrict__ w,
+    const float* __restrict__ bias_vec, float bias_s, float clip, const float* __restrict__ yin,
+    int64_t ldin, float* __restrict__ yout, int64_t ldout, const int32_t* __restrict__ count_ptr,
+    const int32_t* __restrict__ map, int32_t* __restrict__ flags, int32_t narrow,
+    const uint32_t* __restrict__ psame) {
+  constexpr int64_t kSpan = (int64_t)kBlock * TPW;
+  const int p = blockIdx.x;
+  const int64_t count = *count_ptr;
+  const int beg = pbeg[p], end = pbeg[p + 1];
+  const uint32_t same = psame ? psame[p] : 0u;
+  const int32_t* __restrict__ pc = pcols + (size_t)p * kSlots;
+  if (count <= narrow) {
+    panel_pairs<EPI_PLAIN>(beg, end, pc, outs, w, bias_vec, bias_s, clip, yin, ldin, yout,
+                           ldout, count, map, flags, nullptr, 0, nullptr, 0.f);
+    return;
+  }
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  // this workgroup row's run of kSpan-column units, [r0, r1)
+  const int64_t units = (count + kSpan - 1) / kSpan;
+  const int64_t per = (units + gridDim.y - 1) / gridDim.y;
+  const int64_t u0 = min(units, (int64_t)blockIdx.y * per);
+  const int64_t r0 = u0 * kSpan, r1 = min(count, (u0 + per) * kSpan);
+  for (int64_t base = r0 + (int64_t)wv * kWave * TPW; base < r1; base += kSpan) {
+    int64_t ps[TPW];
+    bool vs[TPW];
+    float x[TPW][kSlots];
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      ps[q] = base + kWave * q + lane;
+      vs[q] = ps[q] < r1;
+      // positions past the run read column 0 (count > 0 here) and are zeroed below
+      const int64_t src = vs[q] ? (map ? (int64_t)map[ps[q]] : ps[q]) : 0;
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) x[q][r] = yin[(int64_t)pc[r] * ldin + src];
+    }
+    bool nz = false;
+#pragma unroll
+    for (int q = 0; q < TPW; ++q)
+#pragma unroll
+      for (int r = 0; r < kSlots; ++r) {
+        x[q][r] = vs[q] ? x[q][r] : 0.f;
+        nz |= x[q][r] != 0.f;
+      }
+    const bool wave_nz = __ballot(nz) != 0ull;
+    bool fl[TPW];
+    float acc[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      fl[q] = false;
+      acc[q] = 0.f;
+    }
+    for (int j = beg; j < end; ++j) {
+      // a row repeating the previous row's weights has its chains (same inputs, same order)
+      if (!((same >> (j - beg)) & 1u)) {
+        const float* __restrict__ wj = w + (size_t)j * kSlots;
+#pragma unroll
+        for (int q = 0; q < TPW; ++q) acc[q] = 0.f;
+        if (wave_nz) {
+#pragma unroll
+          for (int r = 0; r < kSlots; ++r)
+#pragma unroll
+            for (int q = 0; q < TPW; ++q) acc[q] = __builtin_fmaf(wj[r], x[q][r], acc[q]);
+        }
+      }
+      const int o = outs[j];
+      const float b = bias_vec ? bias_vec[o] : bias_s;
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        const float y = epilogue<EPI_PLAIN>(acc[q], b, clip, vs[q], o, ps[q], 0, nullptr, 0, 0.f,
+                                            fl[q]);
+        if (vs[q]) yout[(int64_t)o * ldout + ps[q]] = y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TPW; ++q)
+      if (flags && vs[q] && fl[q]) flags[ps[q]] = 1;
+  }
+}
+
 // ---- plain panel layer on the matrix cores ----------------------------------------------
 // Same contract as panel_spmm_kernel<EPI_PLAIN> for row-major panel weights.  The workgroup
 // stages its panel's weights in LDS (rows past the panel's row count as zeros), then each
@@ -2485,6 +2578,18 @@ void launch_panel_spmm(const SpmmLaunch& a, hipStream_t s) {
                          a.flags, a.zc, a.kcap, a.mcent, a.eps_post, a.shape.narrow,           \
                          a.la.psame);                                                          \
   } while (0)
+  if (a.epilogue == EPI_PLAIN && !mfma && (a.shape.plain_tiles == 2 || a.shape.plain_tiles == 4)) {
+    if (a.shape.plain_tiles == 2)
+      hipLaunchKernelGGL(panel_spmm_tiles_kernel<2>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs,
+                         w, bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,
+                         a.flags, a.shape.narrow, a.la.psame);
+    else
+      hipLaunchKernelGGL(panel_spmm_tiles_kernel<4>, grid, dim3(kBlock), 0, s, pcols, pbeg, outs,
+                         w, bv, bs, net.clip, a.yin, a.ldin, a.yout, a.ldout, a.count, a.map,
+                         a.flags, a.shape.narrow, a.la.psame);
+    check_launch();
+    return;
+  }
   switch (a.epilogue) {
     case EPI_PLAIN: SNICIT_SPMM(EPI_PLAIN); break;
     case EPI_CENTROID: SNICIT_SPMM(EPI_CENTROID); break;

@@ -74,6 +74,9 @@ struct LaunchShape {
   // columns (a shifted list costs ~12% more reads) and carries the dead columns as zeros;
   // the lists entering clustering and the last layer's are always exact
   int32_t compact_slack = 16;
+  // plain layers on the vector kernel (panels whose rows repeat one weight row): 64-column
+  // tiles per wavefront and iteration, 1, 2 or 4
+  int32_t plain_tiles = 2;
   // LDS budget (KB) of the staged wide-row kernel's input tile; 0 = direct-gather kernel
   int32_t ell_lds_kb = 80;
   // wide-row layers with >= 128 rows: 1 = thread per row over 64-column tiles staged in

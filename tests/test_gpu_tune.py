@@ -233,3 +233,40 @@ def test_compaction_slack_keeps_results(ctx, slack, B):
         np.testing.assert_array_equal(plan.activations(), r["yL"])
     with pytest.raises(sa.SnicitError):
         plan.set_compact_slack(1024)
+
+
+@pytest.mark.parametrize("B", [3001, 20000])
+@pytest.mark.parametrize("tiles", [1, 2, 4])
+def test_plain_tiles_keep_results(ctx, tiles, B):
+    """The plain vector kernel with 1, 2 or 4 neighbouring 64-column tiles per wavefront
+    (runs of 256, 512 or 1024 columns per workgroup, a partial last unit at B = 3001) gives
+    the oracle's categories, activations and live counts in BF, and SNICIT's at t = 3, with
+    exact compaction and with lists kept almost always (slack 1023)."""
+    from tests import nets
+
+    N, L = 1024, 30
+    cols, vals = nets.sdgc(N, L, seed=7)
+    y0 = nets.images(N, B, seed=13)
+    net = sa.Net.from_ell(ctx, cols, vals, -0.3)
+    yL, cat, live = orc.plain_f32(cols, vals, -0.3, 32.0, y0)
+    r = orc.snicit_exact(cols, vals, -0.3, 32.0, y0, 3)
+    for slack in (0, 1023):
+        print(f"tiles {tiles} B {B} slack {slack}", flush=True)
+        plan = sa.Plan(net, sa.default_cfg("BF"), B)
+        plan.set_plain_tiles(tiles)
+        plan.set_compact_slack(slack)
+        plan.set_input(y0)
+        plan.run()
+        np.testing.assert_array_equal(plan.categories(), cat)
+        np.testing.assert_array_equal(plan.activations(), yL)
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_LIVE), live)
+        plan = sa.Plan(net, sa.default_cfg("SNICIT", threshold=3), B)
+        plan.set_plain_tiles(tiles)
+        plan.set_compact_slack(slack)
+        plan.set_input(y0)
+        plan.run()
+        np.testing.assert_array_equal(plan.categories(), r["cat"])
+        np.testing.assert_array_equal(plan.trace(sa.Plan.TRACE_CENTROIDS), r["centroids"])
+        np.testing.assert_array_equal(plan.activations(), r["yL"])
+    with pytest.raises(sa.SnicitError):
+        plan.set_plain_tiles(3)
