@@ -92,13 +92,18 @@ def apply_shapes(plan, path):
     if len(entries) != 1:
         raise SystemExit(f"{path}: expected one tuned entry, found {len(entries)}")
     e = next(iter(entries.values()))
+    apply_shape_dict(plan, e)
+    log(f"shapes from {path}: {json.dumps(e)}")
+    return e
+
+
+def apply_shape_dict(plan, e):
+    """Launch shapes and post form of a tuner result (cache entry or TuneInfo dict)."""
     plan.set_shape(e["grid_target"], e["narrow"], e["ell_lds_kb"], e["ell_chunked"])
     plan.set_post_shape(e["post_grid_target"], e["post_narrow"])
     plan.set_post_form(e["post_form"])
     plan.set_compact_slack(e["compact_slack"])
     plan.set_plain_tiles(e["plain_tiles"])
-    log(f"shapes from {path}: {json.dumps(e)}")
-    return e
 
 
 def time_steps(plan, steps, warmup, stages=False):
@@ -620,8 +625,10 @@ def main():
     snic.set_input_images(IMG_SEED, 0, B)
     tune = None
     plain_tiles = 2  # the library's default
+    shapes = None    # the shapes the SNICIT plan runs with, when not the defaults
     if args.shape_from:
-        plain_tiles = apply_shapes(snic, args.shape_from)["plain_tiles"]
+        shapes = apply_shapes(snic, args.shape_from)
+        plain_tiles = shapes["plain_tiles"]
     elif args.tune != "none":
         # untuned steps for the record (not in a traced run: the plan captures its stage
         # graphs once, after tuning), then the autotuner, outside the timed region
@@ -631,6 +638,7 @@ def main():
         t0 = time.perf_counter()
         info = snic.autotune(args.tune_cache, placement=args.tune == "placement")
         plain_tiles = info["plain_tiles"]
+        shapes = info
         tune = {"mode": args.tune, "untuned_ms_per_step": ms_u,
                 "untuned_stages_ms": st_u and {"pre": st_u.ms_pre, "post": st_u.ms_post},
                 "tuning_s": time.perf_counter() - t0,
@@ -672,6 +680,33 @@ def main():
         same = same and bool(np.array_equal(dn.categories(), cat_s))
         log(f"plain dense: {ms_d:.3f} ms/step")
         del dn
+    sweep = None
+    if args.config == "c3" and not args.no_forms:
+        # the workload's threshold sweep: t = 60 and 120 with the shapes the t = 30 plan runs
+        # with (no placement search), against the same plain run
+        sweep = {str(t): {"ms_per_step": ms_s, "speedup_vs_plain_bf": ms_b / ms_s,
+                          "categories_identical_snicit_vs_plain": same,
+                          "stages_ms": {"pre": st_s.ms_pre, "cluster": st_s.ms_cluster,
+                                        "post": st_s.ms_post, "recover": st_s.ms_recover},
+                          "centroids": st_s.centroids, "nerow_first": st_s.nerow_first,
+                          "nerow_last": st_s.nerow_last}}
+        for t2 in (60, 120):
+            p2 = sa.Plan(net, sa.default_cfg("SNICIT", threshold=t2), B)
+            p2.set_input_images(IMG_SEED, 0, B)
+            if shapes:
+                apply_shape_dict(p2, shapes)
+            ms2, _, st2 = time_steps(p2, max(2, args.steps // 2), 1)
+            same2 = bool(np.array_equal(p2.categories(), cat_b))
+            sweep[str(t2)] = {"ms_per_step": ms2, "speedup_vs_plain_bf": ms_b / ms2,
+                              "categories_identical_snicit_vs_plain": same2,
+                              "stages_ms": {"pre": st2.ms_pre, "cluster": st2.ms_cluster,
+                                            "post": st2.ms_post, "recover": st2.ms_recover},
+                              "centroids": st2.centroids, "nerow_first": st2.nerow_first,
+                              "nerow_last": st2.nerow_last}
+            log(f"t={t2}: SNICIT {ms2:.3f} ms/step ({ms_b / ms2:.2f}x plain); categories "
+                f"identical: {same2}; k={st2.centroids} nerow {st2.nerow_first}->"
+                f"{st2.nerow_last}")
+            p2.close()
 
     # Roofline of the two layer kernels: the plain panel layer before t (stage "pre") and the
     # delta layer after it (stage "post").  Rows per launch come from one eager profiling run
@@ -770,6 +805,7 @@ def main():
         "other_kernel_roofline": roofs[plain_name if dom == post_name else post_name],
         "cpu_baseline": cpu,
         "autotune": tune,
+        "threshold_sweep": sweep,
         **forms,
     }
     print(json.dumps(out), flush=True)
