@@ -214,7 +214,10 @@ int snicit_plan_profile(snicit_plan* plan, double* ms, int64_t* rows, double* ce
                         int64_t cap);
 
 /* One-call form mirroring SNICIT::infer(input, golden, num_inputs, batch): rounds of
- * cfg batch over B samples, categories for all B. */
+ * cfg batch over B samples, categories for all B (cat[b] = 0 or 1, int32).  Differences from
+ * the reference: the golden check is the caller's (compare cat with the golden set, as
+ * snicit_sdgc does), and there is no final-activation output; a plan returns those
+ * (snicit_plan_activations) for the batch it holds. */
 int snicit_infer(snicit_net* net, const snicit_run_cfg* cfg, const float* y0, int64_t B,
                  int64_t batch, int32_t* cat, snicit_stats* stats);
 
