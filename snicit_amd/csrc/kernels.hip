@@ -4,8 +4,10 @@
 // column pos).  A wavefront owns 64 consecutive columns, so each neuron access is one
 // contiguous 256-byte run; weights, panel inputs and output ids are wavefront-uniform and
 // are fetched with scalar loads.  The sparse layers are bound by HBM bandwidth (one read and
-// one write of every live activation per layer).  The plain panel layer and the dense input
-// and output layers run their fixed-order multiply-add chains on the matrix cores
+// one write of every live activation per layer).  Panels whose rows all repeat one weight row
+// (every SDGC net) need one chain per column and run it on the vector ALU
+// (panel_spmm_tiles_kernel); plain panels with distinct rows and the dense input and output
+// layers run their fixed-order multiply-add chains on the matrix cores
 // (v_mfma_f32_32x32x1_2b_f32: with K = 1 each element is one fused multiply-add, so the
 // chain's bits are those of the vector form); everything else runs on the vector ALU.
 #include <atomic>
