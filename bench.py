@@ -206,15 +206,16 @@ def bench_batch(args, wl):
                 batch.set_input_images(i, IMG_SEED, 100000 * i)
             else:
                 batch.set_input(i, feats[i])
-        untuned = None
-        if mode == "SNICIT":  # the default launch shapes, for the record
-            for _ in range(args.warmup):
-                batch.run()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                batch.run()
-            untuned = (time.perf_counter() - t0) * 1000.0 / args.steps
-        tune = batch.autotune(cache)
+        untuned, tune = None, None
+        if args.tune != "none":
+            if mode == "SNICIT":  # the default launch shapes, for the record
+                for _ in range(args.warmup):
+                    batch.run()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    batch.run()
+                untuned = (time.perf_counter() - t0) * 1000.0 / args.steps
+            tune = batch.autotune(cache)
         for _ in range(args.warmup):
             batch.run()
         t0 = time.perf_counter()
@@ -227,7 +228,7 @@ def bench_batch(args, wl):
         res[mode] = dict(ms=wall, ev=float(np.mean(ev)), stats=stats, cats=cats, tune=tune,
                          untuned=untuned)
         log(f"{mode} batch: {wall:.3f} ms/step (graph events {np.mean(ev):.3f}); tuned "
-            f"{[(t['key'], t['grid_target'], t['narrow']) for t in tune]}; net A k="
+            f"{[(t['key'], t['grid_target'], t['narrow']) for t in tune or []]}; net A k="
             f"{stats[0].centroids} nerow {stats[0].nerow_first}->{stats[0].nerow_last}; net B "
             f"k={stats[1].centroids} nerow {stats[1].nerow_first}->{stats[1].nerow_last}")
         del batch

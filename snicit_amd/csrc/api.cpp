@@ -2311,7 +2311,9 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
                  "unknown autotune flags");
   const bool snic = !plain_mode(p->cfg.mode) && p->cfg.threshold < p->net->L;
   const bool exact = p->cfg.mode == SNICIT_MODE_SNICIT && p->cfg.cluster == SNICIT_CLUSTER_EXACT;
-  // whole runs need the plan's own stage events (a batch's plans have none)
+  // whole runs need the plan's own stage events (the allocation-free entry point's plan has
+  // none); a batch's plans run alone here, over all their segments, before the batch refines
+  // them on whole-batch launches
   const bool whole = p->ev[0] != nullptr;
   const int32_t pre_stop = plain_mode(p->cfg.mode) ? std::min(p->net->L, 64) : -1;
   const std::string key = tune_key(p, input_density(p), flags);
