@@ -1362,16 +1362,23 @@ __global__ __launch_bounds__(kEllWaves * 64) void ell_wave_kernel(
       // this lane's columns in the chunk's buffer; an entry adds its input row's offset
       const char* __restrict__ xb =
           reinterpret_cast<const char*>(xs + (c & 1) * (kEllChunk * TC) + lane * W);
+      // every row's entry range in this chunk, loaded together rather than one row at a time
+      // just before its entries (wide-row BF layers of the 784->1024x24 net: 2.83 against
+      // 2.98 ms per layer at ~132k columns)
+      int pr[kEllRW], er[kEllRW];
 #pragma unroll
       for (int i = 0; i < kEllRW; ++i) {
-        int o = row0 + i;
-        // recomputed per chunk: hoisting every row's pointers out of the chunk loop would
-        // crowd the scalar registers (empty asm, no instruction)
-        asm volatile("" : "+s"(o));
-        if (i < rw && o < rows) {
-          const int32_t* __restrict__ rr = rng + (size_t)o * (nch + 1);
-          int p = rr[c];
-          const int e = rr[c + 1];
+        const int o = row0 + i;
+        const bool in = i < rw && o < rows;
+        const int32_t* __restrict__ rr = rng + (size_t)(in ? o : 0) * (nch + 1);
+        pr[i] = in ? rr[c] : 0;
+        er[i] = in ? rr[c + 1] : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kEllRW; ++i) {
+        {
+          int p = pr[i];
+          const int e = er[i];
           if (p < e) {
             // whole groups (runs are padded), two register sets in turn: the next group's
             // load goes out before the current group's LDS reads (the stream carries a
