@@ -292,7 +292,7 @@ int snicit_plan_autotune(snicit_plan* plan, const char* cache_path, snicit_tune_
  *   key without running the check again); the centroids, and with them the last bits of the
  *   recovered activations, may differ.
  * SNICIT_TUNE_PLACEMENT: also time the whole run (issued directly, without graphs) on up to
- *   two fresh allocations of the activation ping-pong buffers (when device memory allows)
+ *   four fresh allocations of the activation ping-pong buffers (while device memory allows)
  *   and keep the fastest.  Layer
  *   kernels stream these buffers at rates that depend on where the allocation lands in
  *   physical memory (C3: whole runs from 178 to 204 ms on different allocations of the same

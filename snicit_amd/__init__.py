@@ -460,7 +460,7 @@ class Plan:
     def autotune(self, cache_path=None, sample=False, placement=False):
         """Measures the layer-kernel launch shape on this plan's input (or takes it from the
         JSON cache); sample: also the exact clustering sample size (categories checked equal
-        to the configured size's); placement: also up to two fresh allocations of the
+        to the configured size's); placement: also up to four fresh allocations of the
         activation buffers.  Returns the TuneInfo as a dict."""
         info = TuneInfo()
         path = None if cache_path is None else os.fsencode(cache_path)

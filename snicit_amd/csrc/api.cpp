@@ -2520,7 +2520,7 @@ void plan_autotune(snicit_plan* p, const char* cache_path, uint32_t flags,
   // settings from the cache: the run before placement stands for the default
   res.placements = 1;
   if ((flags & SNICIT_TUNE_PLACEMENT) && whole) {
-    const double own = tune_placement(p, 3, res);
+    const double own = tune_placement(p, 5, res);
     if (found) res.ms_run_default = own;
   }
   // the last whole run captured the stage graphs with the final settings: they are kept
