@@ -62,20 +62,27 @@ def post_layer_bytes(N, P, rows):
 def measured_traffic(wl, kernel):
     """HBM bytes per launch of `kernel` on this workload from a committed counter summary
     (profiles/*_traffic.json: FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
-    counter reads for a copy of known size), or None when none matches."""
+    counter reads for a copy of known size), and the kernel that summary measured: the same
+    kernel, else one of its family (same name before the template arguments, e.g. another
+    tile count), else (None, None)."""
     import glob
+    family = kernel.split("<")[0]
+    found = None
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
         try:
             rec = json.load(open(path))
         except (OSError, ValueError):
             continue
         w = rec.get("workload", {})
-        if rec.get("kernel") != kernel or (w.get("N"), w.get("L"), w.get("batch"),
-                                           w.get("threshold")) != (wl["N"], wl["L"], wl["B"],
-                                                                   wl["t"]):
+        if (w.get("N"), w.get("L"), w.get("batch"), w.get("threshold")) != (
+                wl["N"], wl["L"], wl["B"], wl["t"]):
             continue
-        return rec.get("traffic_bytes_per_launch")
-    return None
+        k = rec.get("kernel", "")
+        if k == kernel:
+            return rec.get("traffic_bytes_per_launch"), k
+        if k.split("<")[0] == family and found is None:
+            found = (rec.get("traffic_bytes_per_launch"), k)
+    return found or (None, None)
 
 
 def live_rows(plan, B, n):
@@ -740,10 +747,11 @@ def main():
     def roof(name):
         d = kern[name]
         gbs = d["bytes"] / d["s"] / 1e9 if d["s"] > 0 else None
+        traffic, traffic_kernel = measured_traffic(wl, name.split(" ")[0])
         return {"kernel": name, "bound": "hbm",
                 "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None,
-                "traffic": measured_traffic(wl, name.split(" ")[0]),
+                "traffic": traffic, "traffic_profiled_kernel": traffic_kernel,
                 "bytes_per_launch": d["bytes"] / max(1, d["launches"]),
                 "avg_launch_ms": 1000.0 * d["s"] / max(1, d["launches"]),
                 "eager_avg_launch_ms": 1000.0 * d["eager_s"] / max(1, d["launches"]),
