@@ -106,6 +106,33 @@ int main() {
       check_packed(N, K, cols, vals, packed);
     }
   }
+  // grouping properties on random short-row layers: every grouping keeps each row's entries
+  // (check_packed), and the search never packs into more panels than the consecutive rule,
+  // which never packs into more than one input set per panel
+  for (int trial = 0; trial < 24; ++trial) {
+    Rng r{(uint64_t)trial * 7919u + 17u};
+    const int32_t N = 256 + (int32_t)(r.next() % 768);
+    const int32_t K = 1 + (int32_t)(r.next() % 31);
+    const int32_t span = 1 + (int32_t)(r.next() % 64);  // inputs drawn near a row-specific base
+    std::vector<int32_t> cols((size_t)N * K);
+    std::vector<float> vals((size_t)N * K);
+    for (int32_t o = 0; o < N; ++o) {
+      const int32_t base = (int32_t)(r.next() % N);
+      for (int32_t q = 0; q < K; ++q) {
+        cols[(size_t)o * K + q] = (base + (int32_t)(r.next() % (uint64_t)(span + K))) % N;
+        vals[(size_t)o * K + q] = (float)((int32_t)(r.next() % 2001) - 1000) / 512.0f;
+      }
+    }
+    size_t panels[3];
+    for (int32_t grouping : {SNICIT_GROUP_NONE, SNICIT_GROUP_CONSECUTIVE, SNICIT_GROUP_SEARCH}) {
+      HostLayerPanels packed;
+      pack_layer_ell(N, K, cols.data(), vals.data(), packed, grouping);
+      check_packed(N, K, cols, vals, packed);
+      panels[grouping] = packed.pbeg.size() - 1;
+    }
+    CHECK(panels[SNICIT_GROUP_SEARCH] <= panels[SNICIT_GROUP_CONSECUTIVE]);
+    CHECK(panels[SNICIT_GROUP_CONSECUTIVE] <= panels[SNICIT_GROUP_NONE]);
+  }
   if (failures) {
     std::fprintf(stderr, "%d checks failed\n", failures);
     return 1;
